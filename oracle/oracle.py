@@ -1,0 +1,438 @@
+"""CPU oracle for the Fluid2d timestep hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker (or the reported CPU
+baseline).  The product package ``fluid2d_amd`` never imports it.
+
+The arithmetic lives in ``f2d_oracle.c`` (FP64, ``-ffp-contract=off``); this
+module wraps it with ctypes on NumPy arrays in the reference's layout:
+row-major ``(ny + 2*nh, nx + 2*nh)`` float64 fields, int8 masks.
+
+Functions and the reference routine each restates:
+
+=====================  =====================================================
+``adv_upwind``         core/fortran_advection.f90:2-165 (+ fortran_fluxes.f90:2-170)
+``adv_centered``       core/fortran_advection.f90:169-284 (+ fortran_fluxes.f90:174-279)
+``celltocorner``       core/fortran_operators.f90:44-64
+``orthogradient``      core/fortran_operators.f90:2-39
+``add_diffusion``      core/fortran_operators.f90:125-156
+``add_torque``         core/fortran_operators.f90:330-381
+``fillhalo``           core/gmg/fortran_multigrid.f90:365-412
+``smoothtwice``        core/gmg/fortran_multigrid.f90:2-127
+``residual``           core/gmg/fortran_multigrid.f90:320-362
+``restrict``           core/gmg/fortran_multigrid.f90:501-546
+``interpolate``        core/gmg/fortran_multigrid.f90:415-498
+``norm2``              core/gmg/fortran_multigrid.f90:813-839
+``coarsenmatrix``      core/gmg/fortran_multigrid.f90:706-811
+``kemaxu`` etc.        core/fortran_diag.f90:3-30, 63-120, 155-195
+``np_sum_interior``    core/grid.py:132-140 (NumPy's buffered pairwise sum)
+``OracleGmg``          core/gmg/hierarchy.py:21-218, core/gmg/level.py:24-496
+``OracleModel``        core/euler.py, core/boussinesq.py, core/timescheme.py,
+                       core/operators.py:12-155, 214-498, core/fluid2d.py:235-268
+``make_grid``          core/grid.py:12-130 (one subdomain)
+=====================  =====================================================
+
+Parity status of each function (how it is pinned) is listed in the module
+docstring of ``tests/test_oracle_pins.py``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed recipe (oracle/Makefile)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def _opt(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    sig = {
+        "orc_adv_upwind": (_i, [_bp, _dp, _dp, _dp, _dp, _vp, _vp, _dp, _i, _i, _i, _i, _i]),
+        "orc_adv_centered": (_i, [_bp, _dp, _dp, _dp, _dp, _vp, _vp, _dp, _i, _i, _i, _i]),
+        "orc_celltocorner": (None, [_dp, _dp, _i, _i]),
+        "orc_orthogradient": (None, [_bp, _dp, _d, _d, _dp, _dp, _i, _i]),
+        "orc_add_diffusion": (None, [_bp, _dp, _d, _d, _dp, _i, _i]),
+        "orc_add_torque": (None, [_bp, _dp, _d, _i, _d, _dp, _i, _i]),
+        "orc_fillhalo": (None, [_dp, _i, _i, _i]),
+        "orc_smoothtwice": (None, [_bp, _dp, _dp, _dp, _d, _i, _i, _dp]),
+        "orc_residual": (None, [_bp, _dp, _dp, _dp, _dp, _i, _i]),
+        "orc_restrict": (None, [_bp, _dp, _i, _dp, _i, _i, _i, _i]),
+        "orc_interpolate": (_i, [_bp, _bp, _dp, _i, _dp, _i, _i, _i, _i]),
+        "orc_norm2": (_d, [_bp, _dp, _i, _i, _i]),
+        "orc_coarsenmatrix": (None, [_dp, _bp, _bp, _i, _dp, _i, _i, _i, _i]),
+        "orc_kemaxu": (None, [_bp, _dp, _dp, _i, _i, _i, C.POINTER(_d), C.POINTER(_d)]),
+        "orc_sumandnorm": (None, [_bp, _dp, _i, _i, _i, C.POINTER(_d), C.POINTER(_d)]),
+        "orc_dotprod": (_d, [_bp, _dp, _dp, _i, _i, _i]),
+        "orc_sum": (_d, [_bp, _dp, _i, _i, _i]),
+        "orc_np_sum_interior": (_d, [_dp, _i, _i, _i]),
+        "orc_gmg_nlevels": (_i, [_i, _i]),
+        "orc_gmg_create": (_vp, [_i, _i, _i, _d, _d, _d, _d, _bp]),
+        "orc_gmg_destroy": (None, [_vp]),
+        "orc_gmg_level_shape": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i)]),
+        "orc_gmg_level_ptr": (_vp, [_vp, _i, _i]),
+        "orc_gmg_vcycle": (None, [_vp, _i]),
+        "orc_gmg_fcycle": (None, [_vp, _i]),
+        "orc_gmg_solve": (_i, [_vp, _dp, _dp, _i, _d, C.POINTER(_i), C.POINTER(_d)]),
+        "orc_gmg_twovcycle": (None, [_vp, _dp, _dp]),
+        "orc_model_create": (_vp, [_i, _i, _i, _d, _d, _bp, _i, _i, _d, _i, _i, _i, _dp, _d, _d]),
+        "orc_model_destroy": (None, [_vp]),
+        "orc_model_state": (_vp, [_vp]),
+        "orc_model_tendency": (_vp, [_vp, _i]),
+        "orc_model_mskp": (_vp, [_vp]),
+        "orc_model_gmg": (_vp, [_vp]),
+        "orc_model_set_umax": (None, [_vp, _d]),
+        "orc_model_last_solve": (None, [_vp, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i)]),
+        "orc_model_set_psi_from_vorticity": (_i, [_vp]),
+        "orc_model_step": (_i, [_vp, _d]),
+        "orc_model_diagnostics": (None, [_vp, _dp, _dp, _dp]),
+        "orc_model_run": (_i, [_vp, _i, _d, _dp, _dp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _c(a, dt):
+    if not (isinstance(a, np.ndarray) and a.dtype == dt and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous %s array" % np.dtype(dt).name)
+    return a
+
+
+# --------------------------------------------------------------------------
+# kernels (in place, like the f2py intent(inplace) originals)
+# --------------------------------------------------------------------------
+def adv_upwind(msk, x, y, u, v, cst, nh, method, order, xflx=None, yflx=None):
+    m, n = x.shape
+    st = lib().orc_adv_upwind(_c(msk, np.int8), _c(x, np.float64), _c(y, np.float64),
+                              _c(u, np.float64), _c(v, np.float64), _opt(xflx), _opt(yflx),
+                              np.ascontiguousarray(cst, dtype=np.float64), nh, method, order, m, n)
+    if st:
+        raise ValueError("adv_upwind: nh must be 3")
+
+
+def adv_centered(msk, x, y, u, v, cst, nh, order, xflx=None, yflx=None):
+    m, n = x.shape
+    st = lib().orc_adv_centered(_c(msk, np.int8), _c(x, np.float64), _c(y, np.float64),
+                                _c(u, np.float64), _c(v, np.float64), _opt(xflx), _opt(yflx),
+                                np.ascontiguousarray(cst, dtype=np.float64), nh, order, m, n)
+    if st:
+        raise ValueError("adv_centered: nh must be 3")
+
+
+def celltocorner(xr, xp):
+    m, n = xr.shape
+    lib().orc_celltocorner(_c(xr, np.float64), _c(xp, np.float64), m, n)
+
+
+def orthogradient(msk, psi, dx, dy, u, v):
+    m, n = psi.shape
+    lib().orc_orthogradient(_c(msk, np.int8), _c(psi, np.float64), dx, dy,
+                            _c(u, np.float64), _c(v, np.float64), m, n)
+
+
+def add_diffusion(msk, trac, dx, kdiff, dtrac):
+    m, n = trac.shape
+    lib().orc_add_diffusion(_c(msk, np.int8), _c(trac, np.float64), dx, kdiff,
+                            _c(dtrac, np.float64), m, n)
+
+
+def add_torque(msk, buoy, dx, nh, gravity, domega):
+    m, n = buoy.shape
+    lib().orc_add_torque(_c(msk, np.int8), _c(buoy, np.float64), dx, nh, gravity,
+                         _c(domega, np.float64), m, n)
+
+
+def fillhalo(x, nh):
+    m, n = x.shape
+    lib().orc_fillhalo(_c(x, np.float64), nh, m, n)
+
+
+def smoothtwice(msk, A5, x, b, omega):
+    """A5: (5, m, n) float64 operator planes."""
+    m, n = x.shape
+    work = np.zeros_like(x)
+    lib().orc_smoothtwice(_c(msk, np.int8), _c(A5, np.float64), _c(x, np.float64),
+                          _c(b, np.float64), omega, m, n, work)
+
+
+def residual(msk, A5, x, b, r):
+    m, n = x.shape
+    lib().orc_residual(_c(msk, np.int8), _c(A5, np.float64), _c(x, np.float64),
+                       _c(b, np.float64), _c(r, np.float64), m, n)
+
+
+def restrict(msk2, x1, nh, x2):
+    m2, n2 = x2.shape
+    m1, n1 = x1.shape
+    lib().orc_restrict(_c(msk2, np.int8), _c(x1, np.float64), nh, _c(x2, np.float64),
+                       m2, n2, m1, n1)
+
+
+def interpolate(msk1, msk2, x2, nh, x1):
+    m2, n2 = x2.shape
+    m1, n1 = x1.shape
+    if lib().orc_interpolate(_c(msk1, np.int8), _c(msk2, np.int8), _c(x2, np.float64), nh,
+                             _c(x1, np.float64), m2, n2, m1, n1):
+        raise ValueError("interpolate: nh must be 3")
+
+
+def norm2(msk, x, nh):
+    m, n = x.shape
+    return lib().orc_norm2(_c(msk, np.int8), _c(x, np.float64), nh, m, n)
+
+
+def coarsenmatrix(Af9, msk1, msk2, nh):
+    m1, n1 = msk1.shape
+    m2, n2 = msk2.shape
+    Ac = np.zeros((9, m2, n2))
+    lib().orc_coarsenmatrix(_c(Af9, np.float64), _c(msk1, np.int8), _c(msk2, np.int8), nh,
+                            Ac, m1, n1, m2, n2)
+    return Ac
+
+
+def kemaxu(msk, u, v, nh):
+    m, n = u.shape
+    ke, mx = C.c_double(), C.c_double()
+    lib().orc_kemaxu(_c(msk, np.int8), _c(u, np.float64), _c(v, np.float64), nh, m, n,
+                     C.byref(ke), C.byref(mx))
+    return ke.value, mx.value
+
+
+def sumandnorm(msk, x, nh):
+    m, n = x.shape
+    s, s2 = C.c_double(), C.c_double()
+    lib().orc_sumandnorm(_c(msk, np.int8), _c(x, np.float64), nh, m, n, C.byref(s), C.byref(s2))
+    return s.value, s2.value
+
+
+def dotprod(msk, x, y, nh):
+    m, n = x.shape
+    return lib().orc_dotprod(_c(msk, np.int8), _c(x, np.float64), _c(y, np.float64), nh, m, n)
+
+
+def sum_masked(msk, x, nh):
+    m, n = x.shape
+    return lib().orc_sum(_c(msk, np.int8), _c(x, np.float64), nh, m, n)
+
+
+def np_sum_interior(x, nh):
+    m, n = x.shape
+    return lib().orc_np_sum_interior(_c(x, np.float64), nh, m, n)
+
+
+# --------------------------------------------------------------------------
+# grid (core/grid.py:12-130, one subdomain)
+# --------------------------------------------------------------------------
+def make_grid(nx, ny, Lx=1.0, Ly=1.0, geometry="perio", nh=3):
+    """Return dict(msk, xr, yr, dx, dy, area) as core/grid.py builds them."""
+    nxl, nyl = nx + 2 * nh, ny + 2 * nh
+    dx, dy = Lx / nx, Ly / ny
+    x1d = (np.arange(nxl) + 0.5 - nh) * dx
+    y1d = (np.arange(nyl) + 0.5 - nh) * dy
+    xr, yr = np.meshgrid(x1d, y1d)
+    msk = np.ones((nyl, nxl), dtype=np.int8)
+    if geometry in ("xperio", "xchannel", "closed", "disc"):
+        msk[-nh:, :] = 0
+        msk[:nh, :] = 0
+    if geometry in ("yperio", "ychannel", "closed", "disc"):
+        msk[:, -nh:] = 0
+        msk[:, :nh] = 0
+    if geometry == "disc":
+        r = np.sqrt((xr / Lx - 0.5) ** 2 + (yr / Ly - 0.5) ** 2)
+        msk[r >= 0.5] = 0
+    area = float(np.sum(msk[nh:-nh, nh:-nh]) * 1.0)
+    return dict(msk=msk, xr=np.ascontiguousarray(xr), yr=np.ascontiguousarray(yr),
+                dx=dx, dy=dy, area=area, nh=nh, nx=nx, ny=ny, Lx=Lx, Ly=Ly, geometry=geometry)
+
+
+# --------------------------------------------------------------------------
+# multigrid and models
+# --------------------------------------------------------------------------
+class OracleGmg:
+    """core/gmg/hierarchy.py Gmg with one subdomain (npre=npost=1, ndeepest=16)."""
+
+    def __init__(self, nx, ny, dx, dy, mskp, nh=3, omega=8.0 / 9.0, hydroepsilon=1.0):
+        self._h = lib().orc_gmg_create(nx, ny, nh, dx, dy, hydroepsilon, omega,
+                                       _c(np.ascontiguousarray(mskp, dtype=np.int8), np.int8))
+        if not self._h:
+            raise ValueError("bad grid size")
+        self.nlevs = lib().orc_gmg_nlevels(nx, ny)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_gmg_destroy(self._h)
+            self._h = None
+
+    def shape(self, lev):
+        m, n = C.c_int(), C.c_int()
+        lib().orc_gmg_level_shape(self._h, lev, C.byref(m), C.byref(n))
+        return m.value, n.value
+
+    def _arr(self, lev, which, dtype, extra=()):
+        m, n = self.shape(lev)
+        p = lib().orc_gmg_level_ptr(self._h, lev, which)
+        shape = tuple(extra) + (m, n)
+        cnt = int(np.prod(shape))
+        buf = (C.c_byte * (cnt * np.dtype(dtype).itemsize)).from_address(p)
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def msk(self, lev):
+        return self._arr(lev, 0, np.int8)
+
+    def A(self, lev):
+        return self._arr(lev, 1, np.float64, (9,))[:5]
+
+    def A9(self, lev):
+        return self._arr(lev, 1, np.float64, (9,))
+
+    def x(self, lev):
+        return self._arr(lev, 2, np.float64)
+
+    def b(self, lev):
+        return self._arr(lev, 3, np.float64)
+
+    def r(self, lev):
+        return self._arr(lev, 4, np.float64)
+
+    def Vcycle(self, lev1):
+        lib().orc_gmg_vcycle(self._h, lev1)
+
+    def Fcycle(self, lev1):
+        lib().orc_gmg_fcycle(self._h, lev1)
+
+    def solve(self, x, b, maxite=4, tol=1e-11):
+        nite, res = C.c_int(), C.c_double()
+        st = lib().orc_gmg_solve(self._h, _c(x, np.float64), _c(b, np.float64), maxite, tol,
+                                 C.byref(nite), C.byref(res))
+        if st:
+            raise RuntimeError("solver is not converging")
+        return nite.value, res.value
+
+    def twoVcycle(self, x, b):
+        lib().orc_gmg_twovcycle(self._h, _c(x, np.float64), _c(b, np.float64))
+        return 1, 0.0
+
+
+MODELS = {"euler": 0, "boussinesq": 1}
+SCHEMES = {"RK3_SSP": 0, "RK3": 1}
+FS_METHODS = {"minmax": 0, "parabolic": 1}
+
+
+class OracleModel:
+    """Euler / Boussinesq model with the reference's RK3_SSP (or RK3) step."""
+
+    VARS = {"euler": ["vorticity", "psi", "u", "v", "source"],
+            "boussinesq": ["vorticity", "psi", "u", "v", "buoyancy", "banom"]}
+
+    def __init__(self, modelname, nx, ny, Lx=1.0, Ly=1.0, geometry="perio", order=5,
+                 flux_splitting_method="parabolic", aparab=0.05, timestepping="RK3_SSP",
+                 diffusion=False, Kdiff=0.0, gravity=1.0, hydroepsilon=1.0, msk=None):
+        self.modelname = modelname
+        self.grid = make_grid(nx, ny, Lx, Ly, geometry)
+        if msk is not None:
+            self.grid["msk"] = np.ascontiguousarray(msk, dtype=np.int8)
+            self.grid["area"] = float(np.sum(self.grid["msk"][3:-3, 3:-3]) * 1.0)
+        self.msk = self.grid["msk"]
+        kd = Kdiff if isinstance(Kdiff, dict) else {"vorticity": Kdiff, "buoyancy": Kdiff}
+        kdiff = np.array([kd.get("vorticity", 0.0), kd.get("buoyancy", 0.0)])
+        self._h = lib().orc_model_create(
+            MODELS[modelname], nx, ny, Lx, Ly, self.msk, order, FS_METHODS[flux_splitting_method],
+            aparab, SCHEMES[timestepping], 1 if geometry == "perio" else 0,
+            1 if diffusion else 0, kdiff, gravity, hydroepsilon)
+        self.varname_list = self.VARS[modelname]
+        self.nvar = len(self.varname_list)
+        self.m, self.n = ny + 6, nx + 6
+        p = lib().orc_model_state(self._h)
+        buf = (C.c_byte * (self.nvar * self.m * self.n * 8)).from_address(p)
+        self.state = np.frombuffer(buf, dtype=np.float64).reshape(self.nvar, self.m, self.n)
+        self.gmg_handle = lib().orc_model_gmg(self._h)
+        self.diags = {"maxspeed": 0.0}
+        self.area = self.grid["area"]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_model_destroy(self._h)
+            self._h = None
+
+    def get(self, name):
+        return self.state[self.varname_list.index(name)]
+
+    def tendency(self, k):
+        p = lib().orc_model_tendency(self._h, k)
+        buf = (C.c_byte * (self.nvar * self.m * self.n * 8)).from_address(p)
+        return np.frombuffer(buf, dtype=np.float64).reshape(self.nvar, self.m, self.n)
+
+    @property
+    def mskp(self):
+        p = lib().orc_model_mskp(self._h)
+        buf = (C.c_byte * (self.m * self.n)).from_address(p)
+        return np.frombuffer(buf, dtype=np.int8).reshape(self.m, self.n)
+
+    def set_psi_from_vorticity(self):
+        st = lib().orc_model_set_psi_from_vorticity(self._h)
+        if st:
+            raise RuntimeError("solver is not converging")
+
+    def last_solve(self):
+        nite, res, st = C.c_int(), C.c_double(), C.c_int()
+        lib().orc_model_last_solve(self._h, C.byref(nite), C.byref(res), C.byref(st))
+        return nite.value, res.value
+
+    def diagnostics(self):
+        out = np.zeros(8)
+        lib().orc_model_diagnostics(self._h, self.grid["xr"], self.grid["yr"], out)
+        a = self.area
+        d = {"maxspeed": out[0], "ke": out[1] / a, "vorticity": out[2] / a,
+             "enstrophy": 0.5 * out[3] / a, "raw": out.copy()}
+        if self.modelname == "euler":
+            d.update(px=out[4] / a, py=out[5] / a, angmom=out[6] / a, source=out[7] / a)
+        else:
+            d.update(pe=out[4] / a, energy=(out[1] + out[4]) / a, buoyancy=out[5] / a,
+                     brms=np.sqrt(out[6] / a - (out[5] / a) ** 2))
+        self.diags = d
+        return d
+
+    def step(self, dt):
+        lib().orc_model_set_umax(self._h, self.diags["maxspeed"])
+        st = lib().orc_model_step(self._h, dt)
+        if st:
+            raise RuntimeError("solver is not converging")
+        return self.diagnostics()
+
+    def run(self, nsteps, dt):
+        """Fluid2d.loop body with fixed dt; returns the F-cycle count per step."""
+        log = np.zeros(nsteps, dtype=np.int32)
+        lib().orc_model_set_umax(self._h, self.diags["maxspeed"])
+        st = lib().orc_model_run(self._h, nsteps, dt, self.grid["xr"], self.grid["yr"],
+                                 log.ctypes.data_as(_vp))
+        if st:
+            raise RuntimeError("solver is not converging")
+        self.diagnostics()
+        return log
