@@ -1,0 +1,65 @@
+"""Grid: sizes, cell-centred coordinates and the geometry mask.
+
+Restates core/grid.py:12-140 for one subdomain (npx = npy = 1): the mask is
+int8, 0 in solid cells (walls are the halo rows/columns of the closed
+directions), and ``domain_integration`` sums the interior in NumPy's order.
+"""
+import numpy as np
+
+
+class Grid:
+    def __init__(self, param):
+        param.checkall()
+        self.nx, self.ny = int(param.nx), int(param.ny)
+        self.Lx, self.Ly = float(param.Lx), float(param.Ly)
+        self.nh = int(param.nh)
+        self.geometry = param.geometry
+        self.hydroepsilon = param.hydroepsilon
+        self.nxl = self.nx + 2 * self.nh
+        self.nyl = self.ny + 2 * self.nh
+        self.dx = self.Lx / self.nx
+        self.dy = self.Ly / self.ny
+        nh = self.nh
+        self.x1d = (np.arange(self.nxl) + 0.5 - nh) * self.dx
+        self.y1d = (np.arange(self.nyl) + 0.5 - nh) * self.dy
+        xr, yr = np.meshgrid(self.x1d, self.y1d)
+        self.xr, self.yr = np.ascontiguousarray(xr), np.ascontiguousarray(yr)
+        self.set_msk()
+
+    def set_msk(self):
+        """core/grid.py:67-102"""
+        nh = self.nh
+        self.msk = np.ones((self.nyl, self.nxl), dtype=np.int8)
+        if self.geometry in ("xperio", "xchannel", "closed", "disc"):
+            self.msk[-nh:, :] = 0
+            self.msk[:nh, :] = 0
+        if self.geometry in ("yperio", "ychannel", "closed", "disc"):
+            self.msk[:, -nh:] = 0
+            self.msk[:, :nh] = 0
+        if self.geometry == "disc":
+            r = np.sqrt((self.xr / self.Lx - 0.5) ** 2 + (self.yr / self.Ly - 0.5) ** 2)
+            self.msk[r >= 0.5] = 0
+        self.finalize_msk()
+
+    def finalize_msk(self):
+        """core/grid.py:104-130 (area and centred coordinates)."""
+        msk = self.msk
+        self.area = self.domain_integration(msk)
+        x0 = self.domain_integration(self.xr * msk) / self.area
+        y0 = self.domain_integration(self.yr * msk) / self.area
+        self.xr0 = (self.xr - x0) * msk
+        self.yr0 = (self.yr - y0) * msk
+        self.x0, self.y0 = x0, y0
+
+    def domain_integration(self, z2d):
+        nh = self.nh
+        return float(np.sum(z2d[nh:-nh, nh:-nh]) * 1.0)
+
+    def fill_halo(self, x):
+        """Periodic halo (core/gmg/fortran_multigrid.f90:365-412) on a host array."""
+        nh = self.nh
+        M, N = x.shape[-2] - 2 * nh, x.shape[-1] - 2 * nh
+        x[..., :nh, :] = x[..., M:M + nh, :]
+        x[..., -nh:, :] = x[..., nh:2 * nh, :]
+        x[..., :, :nh] = x[..., :, N:N + nh]
+        x[..., :, -nh:] = x[..., :, nh:2 * nh]

@@ -1,0 +1,211 @@
+/*
+ * fluid2d_amd.h -- C ABI of libf2dgpu.so, the MI355X (gfx950) implementation
+ * of the Fluid2d timestep hot path.
+ *
+ * The reference binds its Fortran kernels into Python with f2py
+ * (build.py:14-39); the Python classes Operators, Gmg, Timescheme, Euler and
+ * Boussinesq call them.  This header is the replacement boundary: plain C
+ * types, device pointers, an explicit stream, and an int status instead of
+ * Fortran `stop` / Python `exit`.
+ *
+ * Conventions
+ *  - Fields are row-major float64 arrays of m rows (ny + 2*nh) by n columns
+ *    (nx + 2*nh), x contiguous: NumPy C order.  Fortran x(j,i) of the
+ *    reference is element (j-1)*n + (i-1).  Masks are int8 0/1.  nh must be 3.
+ *  - Kernel-level entry points take device pointers and a stream
+ *    (hipStream_t, NULL = the null stream) and only enqueue work; results
+ *    are in place, as with the reference's f2py intent(inplace) arguments.
+ *    Scalar results are written to a device buffer.
+ *  - Handle-level entry points (f2d_gmg_*, f2d_model_*) own their device
+ *    buffers.  Calls on one handle must not overlap; different handles are
+ *    independent.
+ *  - Every function returns F2D_OK (0) or an F2D_E_* code;
+ *    f2d_last_error() describes the last failure of the calling thread.
+ */
+#ifndef FLUID2D_AMD_H
+#define FLUID2D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define F2D_OK 0
+#define F2D_E_ARG 1         /* invalid argument: nh != 3, bad shape, bad option */
+#define F2D_E_HIP 2         /* HIP runtime error */
+#define F2D_E_NOMEM 3       /* device allocation failed */
+#define F2D_E_DIVERGED 4    /* multigrid solve diverged (core/gmg/hierarchy.py:182-188 exits) */
+#define F2D_E_UNSUPPORTED 5 /* option not built */
+
+typedef void *f2d_stream_t; /* a hipStream_t */
+
+const char *f2d_last_error(void);
+int f2d_version(void);
+int f2d_device_count(int *count);
+
+/* ---- device memory helpers (for callers without another allocator) ---- */
+int f2d_malloc(void **ptr, size_t bytes);
+int f2d_free(void *ptr);
+int f2d_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int f2d_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int f2d_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int f2d_synchronize(void);
+
+/* ---- advection (core/operators.py:214-236 calls these) ---------------- */
+
+/* core/fortran_advection.f90:2-165 adv_upwind(msk,x,y,u,v,cst,nh,method,order);
+ * with xflx/yflx non-NULL: core/fortran_fluxes.f90:2-170 (edge fluxes stored).
+ * cst = {dx, dy, logcosh, umax, aparab} (host array, core/operators.py:112-117).
+ * method 0 minmax, 1 parabolic, 2 logcosh; order 1, 3 or 5.
+ * Writes y on the interior only (no halo fill). */
+int f2d_adv_upwind_f64(const int8_t *msk, const double *x, double *y, const double *u,
+                       const double *v, double *xflx, double *yflx, const double *cst,
+                       int nh, int method, int order, int m, int n, f2d_stream_t stream);
+
+/* core/fortran_advection.f90:169-284 adv_centered (orders 2, 4, 6); with
+ * xflx/yflx non-NULL: core/fortran_fluxes.f90:174-279 (orders 2, 4). */
+int f2d_adv_centered_f64(const int8_t *msk, const double *x, double *y, const double *u,
+                         const double *v, double *xflx, double *yflx, const double *cst,
+                         int nh, int order, int m, int n, f2d_stream_t stream);
+
+/* ---- grid operators ---------------------------------------------------- */
+
+/* core/fortran_operators.f90:44-64 celltocorner(xr,xp) */
+int f2d_celltocorner_f64(const double *xr, double *xp, int m, int n, f2d_stream_t stream);
+
+/* core/fortran_operators.f90:2-39 computeorthogradient(msk,psi,dx,dy,nh,u,v) */
+int f2d_orthogradient_f64(const int8_t *msk, const double *psi, double dx, double dy, int nh,
+                          double *u, double *v, int m, int n, f2d_stream_t stream);
+
+/* core/fortran_operators.f90:125-156 add_diffusion(msk,trac,dx,nh,Kdiff,dtrac) */
+int f2d_add_diffusion_f64(const int8_t *msk, const double *trac, double dx, int nh, double kdiff,
+                          double *dtrac, int m, int n, f2d_stream_t stream);
+
+/* core/fortran_operators.f90:330-381 add_torque(msk,buoy,dx,nh,gravity,domega) */
+int f2d_add_torque_f64(const int8_t *msk, const double *buoy, double dx, int nh, double gravity,
+                       double *domega, int m, int n, f2d_stream_t stream);
+
+/* ---- multigrid kernels (core/gmg/level.py calls these) ----------------- */
+
+/* core/gmg/fortran_multigrid.f90:365-412 fillhalo(x,nh): periodic halo */
+int f2d_fillhalo_f64(double *x, int nh, int m, int n, f2d_stream_t stream);
+
+/* core/gmg/fortran_multigrid.f90:2-127 smoothtwicewithA(msk,A,x,b,omega,yo)
+ * followed by fillhalo (core/gmg/level.py:351-365).  A: 5 planes of m*n
+ * (A(:,:,1..5) of the reference).  work: m*n scratch (replaces yo). */
+int f2d_smooth_f64(const int8_t *msk, const double *A, double *x, const double *b, double omega,
+                   double *work, int m, int n, f2d_stream_t stream);
+
+/* core/gmg/fortran_multigrid.f90:320-362 computeresidualwithA + fillhalo
+ * (core/gmg/level.py:370-387) */
+int f2d_residual_f64(const int8_t *msk, const double *A, const double *x, const double *b,
+                     double *r, int m, int n, f2d_stream_t stream);
+
+/* core/gmg/fortran_multigrid.f90:501-546 restrict(msk2,x1,nh,x2) + fillhalo
+ * on the coarse grid (core/gmg/level.py:473-496) */
+int f2d_restrict_f64(const int8_t *msk2, const double *x1, int nh, double *x2, int m2, int n2,
+                     int m1, int n1, f2d_stream_t stream);
+
+/* core/gmg/fortran_multigrid.f90:415-498 interpolate(msk1,msk2,x2,nh,x1)
+ * (core/gmg/level.py:447-470).  accumulate=1 adds into x1 instead
+ * (the x += r of core/gmg/hierarchy.py:125, fused). */
+int f2d_interpolate_f64(const int8_t *msk1, const int8_t *msk2, const double *x2, int nh,
+                        double *x1, int m2, int n2, int m1, int n1, int accumulate,
+                        f2d_stream_t stream);
+
+/* core/gmg/fortran_multigrid.f90:706-811 coarsenmatrix(Afine,msk1,msk2,nh)
+ * + fillhalo of each plane (core/gmg/level.py:321-327).  Af, Ac: 9 planes. */
+int f2d_coarsenmatrix_f64(const double *Af, const int8_t *msk1, const int8_t *msk2, int nh,
+                          double *Ac, int m1, int n1, int m2, int n2, f2d_stream_t stream);
+
+/* core/gmg/fortran_multigrid.f90:813-839 computenorm: out[0] = sum of x^2
+ * over the fluid interior (device scalar; summation order differs from the
+ * serial reference by rounding only). */
+int f2d_norm2_f64(const int8_t *msk, const double *x, int nh, int m, int n, double *out,
+                  f2d_stream_t stream);
+
+/* core/fortran_diag.f90:155-195 computekemaxu: out = {ke, maxu} (device) */
+int f2d_kemaxu_f64(const int8_t *msk, const double *u, const double *v, int nh, int m, int n,
+                   double *out, f2d_stream_t stream);
+
+/* core/grid.py:132-140 domain_integration: np.sum over the interior in
+ * NumPy's own summation order (bit-identical to NumPy 2.x); out[0] (device). */
+int f2d_domain_sum_f64(const double *x, int nh, int m, int n, double *out, f2d_stream_t stream);
+
+/* ---- multigrid hierarchy (core/gmg/hierarchy.py Gmg, one subdomain) ---- */
+typedef struct f2d_gmg f2d_gmg;
+
+/* core/gmg/hierarchy.py:23-84 + core/gmg/level.py:24-335.  mskp: level-0
+ * corner mask (host, m*n int8).  npre = npost = 1, ndeepest = 16. */
+int f2d_gmg_create(int nx, int ny, int nh, double dx, double dy, double hydroepsilon,
+                   double omega, const int8_t *mskp_host, f2d_gmg **out);
+int f2d_gmg_destroy(f2d_gmg *g);
+int f2d_gmg_nlevels(const f2d_gmg *g, int *nlev);
+/* which: 0 msk (int8), 1 A (5 planes), 2 x, 3 b, 4 r */
+int f2d_gmg_level(const f2d_gmg *g, int lev, int which, void **dev_ptr, int *m, int *n);
+/* core/gmg/hierarchy.py:98-126 Vcycle(lev1), :131-151 Fcycle(lev1) */
+int f2d_gmg_vcycle(f2d_gmg *g, int lev1);
+int f2d_gmg_fcycle(f2d_gmg *g, int lev1);
+/* core/gmg/hierarchy.py:207-218 twoVcycle(x,b): x (device, in/out), b (device) */
+int f2d_gmg_twovcycle(f2d_gmg *g, double *x, const double *b);
+/* core/gmg/hierarchy.py:154-192 solve(x,b,{maxite,tol}) -> (nite,res);
+ * returns F2D_E_DIVERGED where the reference exits. */
+int f2d_gmg_solve(f2d_gmg *g, double *x, const double *b, int maxite, double tol, int *nite,
+                  double *res);
+
+/* ---- models: Euler / Boussinesq with RK3_SSP or RK3 --------------------- */
+typedef struct f2d_model f2d_model;
+
+typedef struct {
+  int model;            /* 0 euler (core/euler.py), 1 boussinesq (core/boussinesq.py) */
+  int nx, ny;           /* interior size; powers of two down to the coarsest level */
+  double Lx, Ly;
+  int order;            /* 1..6 (core/defaults.json: 5) */
+  int flux_splitting;   /* 0 minmax, 1 parabolic (core/operators.py:124-131) */
+  double aparab;        /* 0.05 */
+  int timestepping;     /* 0 RK3_SSP, 1 RK3 (core/timescheme.py) */
+  int perio;            /* geometry == 'perio': zero-mean psi after a full solve */
+  int diffusion;        /* core/operators.py:292-301 */
+  double kdiff_vort, kdiff_buoy;
+  double gravity;
+  double hydroepsilon;
+} f2d_model_params;
+
+/* Operators.__init__ + model __init__ (core/operators.py:12-155,
+ * core/euler.py:19-101, core/boussinesq.py:17-85).  msk: cell mask (host). */
+int f2d_model_create(const f2d_model_params *p, const int8_t *msk_host, f2d_model **out);
+int f2d_model_destroy(f2d_model *M);
+/* state: nvar x m x n float64 on the device (Var.state, core/variables.py:25);
+ * Euler nvar=5 [vorticity,psi,u,v,source], Boussinesq 6 [..,buoyancy,banom] */
+int f2d_model_state(f2d_model *M, double **state_dev, int *nvar, int *m, int *n);
+/* tendency buffers dx0, dx1, dx2 of core/timescheme.py:35-41 */
+int f2d_model_tendency(f2d_model *M, int k, double **dev);
+int f2d_model_corner_mask(f2d_model *M, int8_t **dev);
+int f2d_model_gmg(f2d_model *M, f2d_gmg **g);
+/* core/euler.py:182-183 set_psi_from_vorticity (full inversion) */
+int f2d_model_set_psi_from_vorticity(f2d_model *M);
+/* Operators.invert_vorticity(x, flag) on a model-shaped device array
+ * (core/operators.py:421-498); full=0 'fast', 1 'full' */
+int f2d_model_invert_vorticity(f2d_model *M, double *x_dev, int full);
+/* Operators.rhs_adv (core/operators.py:214-236) into dxdt (device) */
+int f2d_model_rhs_adv(f2d_model *M, const double *x_dev, double *dxdt_dev);
+/* cst[3] (umax) used by the next advection (core/fluid2d.py:397) */
+int f2d_model_set_umax(f2d_model *M, double umax);
+/* model.step(t, dt) (core/euler.py:103-147, core/boussinesq.py:87-100) */
+int f2d_model_step(f2d_model *M, double dt);
+/* model.diagnostics raw sums (core/euler.py:185-221, core/boussinesq.py:123-153):
+ * out[8] = {maxspeed, ke, sum w, sum w^2, px | pe, py | sum b, sum psi | sum b^2, source | 0}
+ * xr, yr: host coordinate arrays (m*n) or NULL for the last ones given. */
+int f2d_model_diagnostics(f2d_model *M, const double *xr_host, const double *yr_host, double *out);
+/* Fluid2d.loop body with fixed dt (core/fluid2d.py:235-268): per step
+ * cst[3] = maxspeed, step, diagnostics.  nite_log (host, nsteps ints) or NULL. */
+int f2d_model_run(f2d_model *M, int nsteps, double dt, int *nite_log);
+/* F-cycles and final relative residual of the last full solve */
+int f2d_model_last_solve(f2d_model *M, int *nite, double *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
