@@ -57,7 +57,7 @@ def cpu_baseline(name, steps_hint):
         o.get(k)[...] = v
     o.set_psi_from_vorticity()
     d = o.diagnostics()
-    dt = 0.8 * g.dx / max(d["maxspeed"], 1e-12)
+    dt = workloads.time_step(name, g, d["maxspeed"])
     steps = steps_hint
     if steps <= 0:
         t0 = time.perf_counter()
@@ -94,7 +94,7 @@ def main():
         f.model.var.get(k)[...] = v
     f.model.set_psi_from_vorticity()
     d = f.model.diagnostics(f.model.var, 0.0)
-    dt = 0.8 * g.dx / max(d["maxspeed"], 1e-12)
+    dt = workloads.time_step(name, g, d["maxspeed"])
     F.load().f2d_synchronize()
     if a.warmup:
         f.model.run(a.warmup, dt)

@@ -1,0 +1,177 @@
+"""Batched engine: many independent Fluid2d instances advanced together.
+
+``Batch([spec, ...])`` builds one engine for a list of instance specs (model,
+grid, geometry, physics; one time scheme per batch).  Every kernel launch
+covers all instances taking part in that phase of the step; the step runs
+from a hipGraph with the multigrid stopping test and dt on the device.  An
+instance's FP64 result is bit-identical whether it runs alone or in a mix.
+
+A spec is a dict of :class:`fluid2d_amd.Param` keywords, or a ``(Param,
+Grid)`` pair.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .grid import Grid
+from .models import VARNAMES
+from .param import Param
+
+_p = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+
+_SIG = {
+    "f2d_batch_last_error": ([], C.c_char_p),
+    "f2d_batch_create": ([_i, C.POINTER(_lib.f2d_model_params), C.POINTER(C.c_void_p), _i,
+                          C.POINTER(C.c_void_p)], _i),
+    "f2d_batch_destroy": ([_p], _i),
+    "f2d_batch_info": ([_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_longlong)], _i),
+    "f2d_batch_set_field": ([_p, _i, _i, _p], _i),
+    "f2d_batch_get_state": ([_p, _i, _p], _i),
+    "f2d_batch_state_ptr": ([_p, _i, C.POINTER(C.c_void_p)], _i),
+    "f2d_batch_set_dt": ([_p, _i, _d], _i),
+    "f2d_batch_set_adaptive": ([_p, _i, _d, _d], _i),
+    "f2d_batch_use_graph": ([_p, _i], _i),
+    "f2d_batch_set_psi_from_vorticity": ([_p], _i),
+    "f2d_batch_step": ([_p, _i], _i),
+    "f2d_batch_step_async": ([_p, _i], _i),
+    "f2d_batch_synchronize": ([_p], _i),
+    "f2d_batch_diagnostics": ([_p, _i, C.POINTER(_d)], _i),
+    "f2d_batch_solve_info": ([_p, _i, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i)], _i),
+}
+_bound = False
+
+
+def _L():
+    global _bound
+    lib = _lib.load()
+    if not _bound:
+        for name, (args, res) in _SIG.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _bound = True
+    return lib
+
+
+def _call(name, *args):
+    rc = getattr(_L(), name)(*args)
+    if rc != 0:
+        msg = _L().f2d_batch_last_error().decode()
+        if rc == _lib.F2D_E_DIVERGED:
+            raise _lib.DivergenceError(rc, msg)
+        raise _lib.F2DError(rc, msg)
+
+
+def params_struct(param, grid):
+    p = _lib.f2d_model_params()
+    p.model = {"euler": 0, "boussinesq": 1}[param.modelname]
+    p.nx, p.ny = grid.nx, grid.ny
+    p.Lx, p.Ly = grid.Lx, grid.Ly
+    p.order = int(param.order)
+    p.flux_splitting = ["minmax", "parabolic"].index(param.flux_splitting_method)
+    p.aparab = float(param.aparab)
+    p.timestepping = ["RK3_SSP", "RK3"].index(param.timestepping)
+    p.perio = 1 if grid.geometry == "perio" else 0
+    p.diffusion = 1 if param.diffusion else 0
+    kd = param.Kdiff if isinstance(param.Kdiff, dict) else {"vorticity": param.Kdiff,
+                                                             "buoyancy": param.Kdiff}
+    p.kdiff_vort = float(kd.get("vorticity", 0.0))
+    p.kdiff_buoy = float(kd.get("buoyancy", 0.0))
+    p.gravity = float(param.gravity)
+    p.hydroepsilon = float(param.hydroepsilon)
+    return p
+
+
+class Batch:
+    def __init__(self, specs, coarse_lds_budget=0, use_graph=True):
+        self.params, self.grids = [], []
+        for s in specs:
+            if isinstance(s, tuple):
+                prm, g = s
+            else:
+                prm = Param(**s)
+                g = Grid(prm)
+            prm.checkall()
+            self.params.append(prm)
+            self.grids.append(g)
+        n = len(self.params)
+        arr = (_lib.f2d_model_params * n)(*[params_struct(p, g) for p, g in zip(self.params, self.grids)])
+        self._msks = [np.ascontiguousarray(g.msk, dtype=np.int8) for g in self.grids]
+        mp = (C.c_void_p * n)(*[m.ctypes.data for m in self._msks])
+        h = C.c_void_p()
+        _call("f2d_batch_create", n, arr, mp, int(coarse_lds_budget), C.byref(h))
+        self._h = h.value
+        self.n = n
+        _call("f2d_batch_use_graph", self._h, 1 if use_graph else 0)
+        self.varnames = [VARNAMES[p.modelname] for p in self.params]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _L().f2d_batch_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __len__(self):
+        return self.n
+
+    def info(self, i):
+        nl, nf, nlaunch = C.c_int(), C.c_int(), C.c_longlong()
+        _call("f2d_batch_info", self._h, i, C.byref(nl), C.byref(nf), C.byref(nlaunch))
+        return {"nlev": nl.value, "nfine": nf.value, "launches_per_step": nlaunch.value}
+
+    def set_field(self, i, name, a):
+        k = self.varnames[i].index(name)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        g = self.grids[i]
+        if a.shape != (g.nyl, g.nxl):
+            raise ValueError("field shape %s != %s" % (a.shape, (g.nyl, g.nxl)))
+        _call("f2d_batch_set_field", self._h, i, k, a.ctypes.data_as(C.c_void_p))
+
+    def state(self, i):
+        g = self.grids[i]
+        out = np.empty((len(self.varnames[i]), g.nyl, g.nxl))
+        _call("f2d_batch_get_state", self._h, i, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def set_dt(self, dt, i=-1):
+        _call("f2d_batch_set_dt", self._h, int(i), float(dt))
+
+    def set_adaptive(self, on=True, cfl=0.5, dtmax=5.0):
+        _call("f2d_batch_set_adaptive", self._h, 1 if on else 0, float(cfl), float(dtmax))
+
+    def set_psi_from_vorticity(self):
+        _call("f2d_batch_set_psi_from_vorticity", self._h)
+
+    def step(self, nsteps=1):
+        _call("f2d_batch_step", self._h, int(nsteps))
+
+    def step_async(self, nsteps=1):
+        _call("f2d_batch_step_async", self._h, int(nsteps))
+
+    def synchronize(self):
+        _call("f2d_batch_synchronize", self._h)
+
+    def raw_diagnostics(self, i):
+        out = (C.c_double * 8)()
+        _call("f2d_batch_diagnostics", self._h, i, out)
+        return np.array(out[:])
+
+    def diagnostics(self, i):
+        o = self.raw_diagnostics(i)
+        a = self.grids[i].area
+        d = {"maxspeed": o[0], "ke": o[1] / a, "vorticity": o[2] / a, "enstrophy": 0.5 * o[3] / a}
+        if self.params[i].modelname == "euler":
+            d.update(px=o[4] / a, py=o[5] / a, angmom=o[6] / a, source=o[7] / a)
+        else:
+            d.update(pe=o[4] / a, energy=(o[1] + o[4]) / a, buoyancy=o[5] / a)
+        return d
+
+    def solve_info(self, i):
+        nite, res, div = C.c_int(), C.c_double(), C.c_int()
+        _call("f2d_batch_solve_info", self._h, i, C.byref(nite), C.byref(res), C.byref(div))
+        return nite.value, res.value, bool(div.value)

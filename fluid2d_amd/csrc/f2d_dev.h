@@ -1,0 +1,201 @@
+// f2d_dev.h -- per-cell device arithmetic shared by the stand-alone kernels
+// (f2d_ops.hip) and the batched engine (f2d_engine.hip).  Each function
+// keeps the operation order of the reference routine it cites, so FP64
+// results are bit-identical wherever it is evaluated.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "f2d_kernels.h"
+
+namespace f2d {
+
+// Weights of the reference are float32 literal expressions stored into
+// real*8 (core/fortran_advection.f90:36-44): round in float, widen.
+#define W32(num, den) ((double)((float)(num) / (float)(den)))
+
+template <typename T>
+struct UpParams {
+  T zdx, zdy, u1, aa, bb, KK, a;
+  int method, order;
+};
+
+template <typename T>
+__device__ __forceinline__ T split_speed(T vel, const UpParams<T>& p) {
+  T UU = T(0);
+  if (p.method == 0) UU = fabs(vel);
+  if (p.method == 1) {
+    UU = fabs(vel);
+    if (UU < p.u1) UU = p.aa * (vel * vel) + p.bb;
+  }
+  if (p.method == 2) UU = T(2) * p.KK + log(cosh(vel * p.a)) / p.a;
+  return UU;
+}
+
+// Upwind flux through the edge between cell c and c+s: mask window
+// mk[-2s..3s], values q[-2s..3s], edge velocity vel.
+// core/fortran_advection.f90:73-146.  Generic over the value accessor Q
+// (Q(k) returns the tracer at offset k*s) so stage states can be formed on
+// the fly.
+template <typename T, typename Q>
+__device__ __forceinline__ T up_edge_flux_q(const int8_t* mk, ptrdiff_t s, T vel,
+                                            const UpParams<T>& p, Q q) {
+  if (mk[0] + mk[s] != 2) return T(0);
+  const T d1 = T(W32(1, 30)), d2 = T(-W32(13, 60)), d3 = T(W32(47, 60)), d4 = T(W32(9, 20)),
+          d5 = T(-W32(1, 20));
+  const T c1 = T(-W32(1, 6)), c2 = T(W32(5, 6)), c3 = T(W32(2, 6));
+  const T UU = split_speed(vel, p);
+  const T up = T(0.5) * (vel + UU), um = T(0.5) * (vel - UU);
+  const int m_2 = mk[-2 * s], m_1 = mk[-s], m0 = mk[0], m1 = mk[s], m2 = mk[2 * s],
+            m3 = mk[3 * s];
+  const int s5p = m_2 + m_1 + m0 + m1 + m2, s3p = m_1 + m0 + m1;
+  const int s5m = m_1 + m0 + m1 + m2 + m3, s3m = m0 + m1 + m2;
+  T qp, qm;
+  if (s5p == 5 && p.order == 5)
+    qp = d1 * q(-2) + d2 * q(-1) + d3 * q(0) + d4 * q(1) + d5 * q(2);
+  else if (s3p == 3 && p.order >= 3)
+    qp = c1 * q(-1) + c2 * q(0) + c3 * q(1);
+  else
+    qp = q(0);
+  if (s5m == 5 && p.order == 5)
+    qm = d5 * q(-1) + d4 * q(0) + d3 * q(1) + d2 * q(2) + d1 * q(3);
+  else if (s3m == 3 && p.order >= 3)
+    qm = c3 * q(0) + c2 * q(1) + c1 * q(2);
+  else
+    qm = q(1);
+  return up * qp + um * qm;
+}
+
+// Centred flux, core/fortran_advection.f90:169-284 (allow6) /
+// core/fortran_fluxes.f90:174-279 (no sixth order).
+template <typename T, typename Q>
+__device__ __forceinline__ T ce_edge_flux_q(const int8_t* mk, ptrdiff_t s, T vel, int order,
+                                            bool allow6, Q q) {
+  const int s2 = mk[0] + mk[s];
+  if (s2 != 2) return T(0);
+  const T e1 = T(W32(1, 60)), e2 = T(-W32(2, 15)), e3 = T(W32(37, 60));
+  const T d1 = T(-W32(1, 12)), d2 = T(W32(7, 12));
+  const T c1 = T(0.5);
+  const int s4 = mk[-s] + mk[0] + mk[s] + mk[2 * s];
+  const int s6 = mk[-2 * s] + s4 + mk[3 * s];
+  T qp = T(0);
+  if (allow6 && s6 == 6 && order == 6) {
+    qp = e1 * (q(-2) + q(3)) + e2 * (q(-1) + q(2));
+    qp = qp + e3 * (q(0) + q(1));
+  } else if (s4 == 4 && order == 4) {
+    qp = d1 * (q(-1) + q(2)) + d2 * (q(0) + q(1));
+  } else if (order >= 2) {
+    qp = c1 * (q(0) + q(1));
+  }
+  return vel * qp;
+}
+
+// Multigrid operator: A is 5 planes of stride P; A_k(j,i) couples (j,i) to
+// its lower neighbour k (SW, S, SE, W, centre); upper couplings by symmetry
+// (core/gmg/fortran_multigrid.f90:68-82, 342-350).  X(d) returns x at flat
+// offset d from the cell.
+template <typename T, typename X>
+__device__ __forceinline__ T offdiag_x(const T* A, size_t P, size_t c, int n, X x) {
+  const T* A1 = A;
+  const T* A2 = A + P;
+  const T* A3 = A + 2 * P;
+  const T* A4 = A + 3 * P;
+  T s = A1[c] * x(-n - 1);
+  s = s + A2[c] * x(-n);
+  s = s + A3[c] * x(-n + 1);
+  s = s + A4[c] * x(-1);
+  s = s + A4[c + 1] * x(1);
+  s = s + A3[c + n - 1] * x(n - 1);
+  s = s + A2[c + n] * x(n);
+  s = s + A1[c + n + 1] * x(n + 1);
+  return s;
+}
+
+// one damped-Jacobi update (core/gmg/fortran_multigrid.f90:66-85)
+template <typename T, typename X>
+__device__ __forceinline__ T jacobi_x(int8_t mk, const T* A, size_t P, size_t c, int n, T bc, T c1,
+                                      T c2, X x) {
+  if (mk == 0) return T(0);
+  const T c3 = c1 / fabs(A[4 * P + c]);
+  return x(0) * c2 + c3 * (offdiag_x(A, P, c, n, x) - bc);
+}
+
+// r = b - A x (core/gmg/fortran_multigrid.f90:340-357)
+template <typename T, typename X>
+__device__ __forceinline__ T residual_x(int8_t mk, const T* A, size_t P, size_t c, int n, T bc,
+                                        X x) {
+  if (mk == 0) return T(0);
+  const T *A1 = A, *A2 = A + P, *A3 = A + 2 * P, *A4 = A + 3 * P, *A5 = A + 4 * P;
+  T s = bc - A1[c] * x(-n - 1);
+  s = s - A2[c] * x(-n);
+  s = s - A3[c] * x(-n + 1);
+  s = s - A4[c] * x(-1);
+  s = s - A5[c] * x(0);
+  s = s - A4[c + 1] * x(1);
+  s = s - A3[c + n - 1] * x(n - 1);
+  s = s - A2[c + n] * x(n);
+  s = s - A1[c + n + 1] * x(n + 1);
+  return s;
+}
+
+// full weighting around a fine point (core/gmg/fortran_multigrid.f90:532-534);
+// R(dj, di) returns the fine residual at the offset
+template <typename T, typename R>
+__device__ __forceinline__ T restrict_r(R r) {
+  return T(0.25) * r(0, 0) + T(0.125) * (r(0, -1) + r(0, 1) + r(-1, 0) + r(1, 0)) +
+         T(0.0625) * (r(-1, -1) + r(-1, 1) + r(1, -1) + r(1, 1));
+}
+
+// Prolongation value at fine (j,i) (core/gmg/fortran_multigrid.f90:454-492,
+// nh = 3): coarse (J2,I2) = ((j+2)>>1, (i+2)>>1); row/column parity picks
+// the injected, edge (mask-weighted pair) or centre (mask-weighted quad)
+// form.  The 1/3 weight is the float32 literal of line 439 widened.
+template <typename T>
+__device__ __forceinline__ T interp_at(const int8_t* msk1, int n1, const int8_t* msk2,
+                                       const T* x2, int n2, int j, int i) {
+  if (msk1[(size_t)j * n1 + i] <= 0) return T(0);
+  const int J2 = (j + 2) >> 1, I2 = (i + 2) >> 1;
+  const size_t k = (size_t)J2 * n2 + I2;
+  const int pj = j & 1, pi = i & 1;
+  const T a = x2[k];
+  if (!pj && !pi) return a;
+  // weights c2 = {0, 1, 0.5}, c3 = {0, 1, 0.5, 0.33333334f, 0.25} selected
+  // without an indexed table (which would live in scratch memory)
+  auto c2 = [](int s) { return s == 2 ? T(0.5) : (s == 1 ? T(1) : T(0)); };
+  if (!pj) return (a + x2[k + 1]) * c2(msk2[k] + msk2[k + 1]);
+  if (!pi) return (a + x2[k + n2]) * c2(msk2[k] + msk2[k + n2]);
+  const int s4 = msk2[k] + msk2[k + 1] + msk2[k + n2] + msk2[k + n2 + 1];
+  const T w = s4 == 4 ? T(0.25)
+                      : (s4 == 3 ? T((double)0.3333333333333333333333333333f)
+                                 : (s4 == 2 ? T(0.5) : (s4 == 1 ? T(1) : T(0))));
+  return w * (a + x2[k + 1] + x2[k + n2] + x2[k + n2 + 1]);
+}
+
+// periodic image index of a halo row/column
+__device__ __forceinline__ int img(int j, int m) {
+  const int M = m - 2 * NH;
+  return j < NH ? j + M : (j >= m - NH ? j - M : j);
+}
+
+// Store at interior (J,I) and its halo images, multiplying each stored copy
+// by the mask at its own location when mk != nullptr (psi * mskp of
+// core/operators.py:481 applied to a filled array).
+template <typename T>
+__device__ __forceinline__ void store_img_mask(T* a, const int8_t* mk, int J, int I, T v, int m,
+                                               int n) {
+  const int M = m - 2 * NH, N = n - 2 * NH;
+  int jc[3], ic[3], nj = 1, ni = 1;
+  jc[0] = J;
+  ic[0] = I;
+  if (J + M < m && J + M >= m - NH) jc[nj++] = J + M;
+  if (J - M >= 0 && J - M < NH) jc[nj++] = J - M;
+  if (I + N < n && I + N >= n - NH) ic[ni++] = I + N;
+  if (I - N >= 0 && I - N < NH) ic[ni++] = I - N;
+  for (int p = 0; p < nj; p++)
+    for (int q = 0; q < ni; q++) {
+      const size_t c = at(jc[p], ic[q], n);
+      a[c] = mk ? v * T(mk[c]) : v;
+    }
+}
+
+}  // namespace f2d

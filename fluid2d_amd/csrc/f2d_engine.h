@@ -1,0 +1,61 @@
+// f2d_engine.h -- batched, graph-replayable engine for the Fluid2d step.
+//
+// A batch holds any number of independent instances (Euler or Boussinesq,
+// any power-of-two grid, any mask).  Every instance runs exactly the
+// operation sequence of the reference (core/timescheme.py RK3_SSP / RK3,
+// core/operators.py, core/gmg/hierarchy.py), so its result does not depend
+// on what else is in the batch.  Each kernel launch covers the tiles of all
+// instances that take part in that phase; a step is a fixed launch program
+// with no host round trip (solve control and dt live on the device), which
+// is captured once into a hipGraph and replayed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace f2d {
+namespace eng {
+
+constexpr int MAXLEV = 16;
+constexpr int TX = 64, TY = 16;    // fine tile (interior cells)
+constexpr int TX2 = 32, TY2 = 8;   // coarse tile of the residual/restriction
+constexpr int NDIAG = 8;
+
+struct LevD {
+  int m, n;
+  long long b, A;  // offsets into the real pool (A: 5 planes of m*n)
+  long long x[2];  // ping-pong buffers (levels >= 1; level 0 uses per-call buffers)
+  long long msk;   // offset into the byte pool
+  int pad;
+};
+
+struct InstD {
+  int model;   // 0 euler, 1 boussinesq
+  int nlev, nfine;
+  int m, n;    // finest padded dims
+  int order, method, perio, diffusion, tscheme, nvar;
+  double dx, dy, gravity, kd0, kd1, area, aparab, omega;
+  long long state, k[3], work, x0[2];  // real pool offsets
+  long long xr, yr;                    // real pool offsets (diagnostics)
+  long long msk, mskp;                 // byte pool offsets
+  LevD L[MAXLEV];
+};
+
+// device-written per-instance values
+struct DynD {
+  double dt, umax;
+  double normeb, res0, res;
+  int nite, ndiv, active, diverged;
+  double psim;
+  double diag[NDIAG];
+  double pad[2];
+};
+
+// one tile of a 2D phase
+struct Task {
+  int inst, lev, y0, x0;
+  long long xin, xout, xc;  // buffer offsets used by the phase
+  int mode, flags;          // phase variant (smoother input form, final mask)
+};
+
+}  // namespace eng
+}  // namespace f2d

@@ -1,0 +1,1740 @@
+// f2d_engine.hip -- batched engine kernels, launch program and hipGraph replay.
+//
+// Kernels (each a fused form of reference routines; FP64 operation order is
+// kept, so results are bit-identical to the unfused path and the oracle):
+//   k_rhs        stage state formed on the fly (core/timescheme.py:167-180)
+//                + advection (core/fortran_advection.f90) [+ y*=msk + torque,
+//                core/fortran_operators.f90:330-381] [+ diffusion, :125-156]
+//                + periodic images (the fills of core/operators.py)
+//   k_c2c        celltocorner (core/fortran_operators.f90:44-64)
+//   k_smooth     smoothtwicewithA + fill (core/gmg/fortran_multigrid.f90:2-127,
+//                core/gmg/level.py:340-368) in one launch through an LDS tile;
+//                input x, zero, x + I(xc) (hierarchy.py:123-126, fused
+//                prolongation and correction) or I(xc) (hierarchy.py:145-151)
+//   k_resid_restrict  residual + fill + full weighting + coarse fill
+//                (hierarchy.py:105-108), optional residual store and norms
+//   k_coarse     all levels below the fine part of one instance in one
+//                workgroup, operands in LDS (V-cycle or F-cycle tail)
+//   k_ortho      computeorthogradient (core/fortran_operators.f90:2-39)
+//   k_combine    final RK combination (core/timescheme.py:180)
+//   k_update, k_fold, k_npsum, k_finalize_psi, k_diag: solve control,
+//                norms, zero-mean psi and diagnostics, all on the device.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/fluid2d_amd.h"
+#include "f2d_dev.h"
+#include "f2d_engine.h"
+
+namespace f2d {
+namespace eng {
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool skip(const DynD* dyn, int inst, int pred) {
+  return pred && !dyn[inst].active;
+}
+
+// stage state (core/timescheme.py:167-180, :152-164), NumPy order
+// stage 0: x; RK3_SSP 1: x + dt*k0; 2: x + (0.25 dt)(k0 + k1);
+// RK3 1: x + (dt/3) k0; 2: x + (0.5 dt) k1
+template <typename T>
+struct Stage {
+  const T *x, *k0, *k1;
+  T c;
+  int kind;  // 0 x, 1 x + c*k0, 2 x + c*(k0+k1), 3 x + c*k1
+  __device__ __forceinline__ T operator()(size_t q) const {
+    switch (kind) {
+      case 0: return x[q];
+      case 1: return x[q] + c * k0[q];
+      case 2: return x[q] + c * (k0[q] + k1[q]);
+      default: return x[q] + c * k1[q];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// k_rhs: tendencies of the tracers for one RK stage
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
+                                             const InstD* __restrict__ I, T* P,
+                                             const int8_t* __restrict__ B,
+                                             const DynD* __restrict__ dyn, int stage, int out_k) {
+  const Task tk = tasks[blockIdx.x];
+  const InstD& in = I[tk.inst];
+  const int m = in.m, n = in.n;
+  const size_t F = (size_t)m * n;
+  const DynD& d = dyn[tk.inst];
+  const double dt = d.dt;
+  Stage<T> S;
+  S.x = P + in.state;
+  S.k0 = P + in.k[0];
+  S.k1 = P + in.k[1];
+  if (stage == 0) {
+    S.kind = 0;
+    S.c = T(0);
+  } else if (in.tscheme == 0) {
+    S.kind = stage == 1 ? 1 : 2;
+    S.c = stage == 1 ? T(dt) : T(0.25 * dt);
+  } else {
+    S.kind = stage == 1 ? 1 : 3;
+    S.c = stage == 1 ? T(dt / 3.) : T(0.5 * dt);
+  }
+  const int8_t* msk = B + in.msk;
+  T* out = P + in.k[out_k];
+  // advection parameters (core/fortran_advection.f90:46-60), cst[3] = umax
+  UpParams<T> p;
+  const double umax = d.umax, logcosh = 0.05;
+  p.zdx = T(1. / in.dx);
+  p.zdy = T(1. / in.dy);
+  {
+    const double a = logcosh / umax;
+    p.a = T(a);
+    p.KK = T((umax - log(cosh(umax * a)) / a) * 0.5);
+    const double u1 = in.aparab * umax;
+    p.u1 = T(u1);
+    p.aa = T(1. / (2. * u1));
+    p.bb = T(u1 * 0.5);
+  }
+  p.method = in.method;
+  p.order = in.order;
+  const bool upwind = in.order % 2 == 1;
+  const bool kforce = stage == (in.tscheme == 0 ? 2 : 0);
+  const int ntr = in.model == 0 ? 1 : 2;
+  const size_t fu = 2 * F, fv = 3 * F;
+  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+    const int J = tk.y0 + r, Ic = tk.x0 + threadIdx.x;
+    if (J >= m - NH || Ic >= n - NH) continue;
+    const size_t c = at(J, Ic, n);
+    for (int t = 0; t < ntr; t++) {
+      const size_t ft = t == 0 ? 0 : 4 * F;  // vorticity, buoyancy
+      auto qx = [&](size_t base) { return [&, base](int k) { return S(ft + base + k); }; };
+      auto qy = [&](size_t base) {
+        return [&, base](int k) { return S(ft + base + (ptrdiff_t)k * n); };
+      };
+      T fe, fw, fn, fs;
+      if (upwind) {
+        fe = up_edge_flux_q(msk + c, 1, S(fu + c), p, qx(c));
+        fw = up_edge_flux_q(msk + c - 1, 1, S(fu + c - 1), p, qx(c - 1));
+        fn = up_edge_flux_q(msk + c, (ptrdiff_t)n, S(fv + c), p, qy(c));
+        fs = up_edge_flux_q(msk + c - n, (ptrdiff_t)n, S(fv + c - n), p, qy(c - n));
+      } else {
+        fe = ce_edge_flux_q(msk + c, 1, S(fu + c), in.order, true, qx(c));
+        fw = ce_edge_flux_q(msk + c - 1, 1, S(fu + c - 1), in.order, true, qx(c - 1));
+        fn = ce_edge_flux_q(msk + c, (ptrdiff_t)n, S(fv + c), in.order, true, qy(c));
+        fs = ce_edge_flux_q(msk + c - n, (ptrdiff_t)n, S(fv + c - n), in.order, true, qy(c - n));
+      }
+      T val = -p.zdx * (fe - fw) - p.zdy * (fn - fs);
+      if (in.model == 1 && t == 0) {  // y *= msk; add_torque
+        const int8_t mc = msk[c];
+        val = val * T(mc);
+        int ml = mc + msk[c - 1], mr = msk[c + 1] + mc;
+        ml = ml < 1 ? 1 : ml;
+        mr = mr < 1 ? 1 : mr;
+        if (ml + mr == 4)
+          val = val + (S(4 * F + c + 1) - S(4 * F + c - 1)) * T(0.5 * in.gravity / in.dx) * T(mc);
+      }
+      if (in.diffusion && kforce && msk[c] == 1) {  // add_diffusion
+        const double coefd =
+            (in.model == 0 ? 1. : (in.tscheme == 0 ? 1.5 : 1.)) * (t == 0 ? in.kd0 : in.kd1);
+        const T coef = T(coefd / (in.dx * in.dx));
+        const T tc = S(ft + c);
+        T s = T(msk[c - 1]) * (S(ft + c - 1) - tc);
+        s = s + T(msk[c + 1]) * (S(ft + c + 1) - tc);
+        s = s + T(msk[c - n]) * (S(ft + c - n) - tc);
+        s = s + T(msk[c + n]) * (S(ft + c + n) - tc);
+        val = val + coef * s;
+      }
+      store_img_mask<T>(out + ft, nullptr, J, Ic, val, m, n);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_c2c: dst = celltocorner(src) on [0, m-2] x [0, n-2]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
+                                             const InstD* __restrict__ I, T* P,
+                                             const DynD* __restrict__ dyn, int pred) {
+  const Task tk = tasks[blockIdx.x];
+  if (skip(dyn, tk.inst, pred)) return;
+  const InstD& in = I[tk.inst];
+  const int m = in.m, n = in.n;
+  const T* src = P + tk.xin;
+  T* dst = P + tk.xout;
+  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+    const int j = tk.y0 + r, i = tk.x0 + threadIdx.x;
+    if (j >= m - 1 || i >= n - 1) continue;
+    const size_t c = at(j, i, n);
+    dst[c] = T(0.25) * (src[c] + src[c + 1] + src[c + n] + src[c + n + 1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_smooth: two damped-Jacobi sweeps + fill, LDS tile.
+// MODE 0: x = xin; 1: x = 0; 2: x = xin + I(xc); 3: x = I(xc).
+// FINAL: each stored copy multiplied by mskp (psi * mskp).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
+                                                const InstD* __restrict__ I, T* P,
+                                                const int8_t* __restrict__ B,
+                                                const DynD* __restrict__ dyn, int pred) {
+  __shared__ T xs[TY + 4][TX + 4];
+  __shared__ T j1[TY + 2][TX + 2];
+  const Task tk = tasks[blockIdx.x];
+  if (skip(dyn, tk.inst, pred)) return;
+  const int MODE = tk.mode;
+  const int final_mask = tk.flags & 1;
+  const InstD& in = I[tk.inst];
+  const LevD& L = in.L[tk.lev];
+  const int m = L.m, n = L.n;
+  const size_t PL = (size_t)m * n;
+  const int8_t* msk = B + L.msk;
+  const T* A = P + L.A;
+  const T* b = P + L.b;
+  const T* xin = P + tk.xin;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+  const int y0 = tk.y0, x0 = tk.x0;
+  // x on rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1], clipped to [1, m-2]
+  for (int q = tid; q < (TY + 4) * (TX + 4); q += nt) {
+    const int rr = q / (TX + 4), cc = q % (TX + 4);
+    const int j = y0 - 2 + rr, i = x0 - 2 + cc;
+    T v = T(0);
+    if (j <= m - 2 && i <= n - 2) {
+      if (MODE == 0 || MODE == 2) v = xin[at(j, i, n)];
+      if (MODE == 2 || MODE == 3) {
+        const LevD& C = in.L[tk.lev + 1];
+        const T r = interp_at<T>(msk, n, B + C.msk, P + tk.xc, C.n, j, i);
+        v = MODE == 2 ? v + r : r;
+      }
+    }
+    xs[rr][cc] = v;
+  }
+  __syncthreads();
+  const T c1 = T(in.omega), c2 = T(1. - in.omega);
+  // first sweep on rows [y0-1, y0+TY] x cols [x0-1, x0+TX], clipped to [.., m-NH]
+  for (int q = tid; q < (TY + 2) * (TX + 2); q += nt) {
+    const int rr = q / (TX + 2), cc = q % (TX + 2);
+    const int j = y0 - 1 + rr, i = x0 - 1 + cc;
+    T v = T(0);
+    if (j <= m - NH && i <= n - NH) {
+      const size_t c = at(j, i, n);
+      v = jacobi_x<T>(msk[c], A, PL, c, n, b[c], c1, c2,
+                      [&](int dd) {
+                        const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+                        const int di = dd - dj * n;
+                        return xs[rr + 1 + dj][cc + 1 + di];
+                      });
+    }
+    j1[rr][cc] = v;
+  }
+  __syncthreads();
+  T* xout = P + tk.xout;
+  const int8_t* mp = final_mask ? B + in.mskp : nullptr;
+  for (int rr = threadIdx.y; rr < TY; rr += blockDim.y) {
+    const int J = y0 + rr, Ic = x0 + threadIdx.x;
+    if (J >= m - NH || Ic >= n - NH) continue;
+    const int cc = threadIdx.x;
+    const size_t c = at(J, Ic, n);
+    const T v = jacobi_x<T>(msk[c], A, PL, c, n, b[c], c1, c2, [&](int dd) {
+      const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+      const int di = dd - dj * n;
+      return j1[rr + 1 + dj][cc + 1 + di];
+    });
+    store_img_mask<T>(xout, mp, J, Ic, v, m, n);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_resid_restrict: r = b - A x on the fine points a coarse tile restricts
+// (halo points through their periodic images), then the full weighting onto
+// the coarse interior + images.
+// flags: 1 store r on the owned fine cells (+ images) into rout (tk.xc),
+//        2 restrict b itself instead of the residual (F-cycle descent),
+//        4 norm partial of r, 8 norm partial of b (owned interior, msk != 0)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__ tasks,
+                                                        const InstD* __restrict__ I, T* P,
+                                                        const int8_t* __restrict__ B,
+                                                        const DynD* __restrict__ dyn, int pred,
+                                                        int flags, double* partial) {
+  constexpr int RY = 2 * TY2 + 1, RX = 2 * TX2 + 1;
+  __shared__ T rs[RY][RX];
+  __shared__ double red[2][8];
+  const Task tk = tasks[blockIdx.x];
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+  if (skip(dyn, tk.inst, pred)) {
+    if ((flags & 12) && tid == 0) {
+      partial[2 * blockIdx.x] = 0.;
+      partial[2 * blockIdx.x + 1] = 0.;
+    }
+    return;
+  }
+  const InstD& in = I[tk.inst];
+  const LevD& Lf = in.L[tk.lev];
+  const LevD& Lc = in.L[tk.lev + 1];
+  const int m1 = Lf.m, n1 = Lf.n, m2 = Lc.m, n2 = Lc.n;
+  const size_t PL = (size_t)m1 * n1;
+  const int8_t* msk1 = B + Lf.msk;
+  const int8_t* msk2 = B + Lc.msk;
+  const T* A = P + Lf.A;
+  const T* x = P + tk.xin;
+  // rhs: level b, or an explicit rhs buffer (tk.xout >= 0 with flag 16)
+  const T* b = P + ((flags & 16) ? tk.xout : Lf.b);
+  const int fy0 = 2 * tk.y0 - 3, fx0 = 2 * tk.x0 - 3;
+  double nr = 0., nb = 0.;
+  for (int q = tid; q < RY * RX; q += nt) {
+    const int rr = q / RX, cc = q % RX;
+    const int jf = fy0 + rr, if_ = fx0 + cc;
+    if (jf > m1 - NH || if_ > n1 - NH) continue;  // beyond the last needed point
+    const int j = img(jf, m1), i = img(if_, n1);
+    const size_t c = at(j, i, n1);
+    T v;
+    if (flags & 2)
+      v = b[c];
+    else
+      v = residual_x<T>(msk1[c], A, PL, c, n1, b[c], [&](int dd) { return x[c + dd]; });
+    rs[rr][cc] = v;
+    // owned fine cells: local rows/cols [0, 2*T-1] that are interior
+    if (rr < 2 * TY2 && cc < 2 * TX2 && jf == j && if_ == i && jf >= NH && if_ >= NH &&
+        jf < m1 - NH && if_ < n1 - NH && (tk.y0 + rr / 2) < m2 - NH && (tk.x0 + cc / 2) < n2 - NH) {
+      if (flags & 1) store_img_mask<T>(P + tk.xc, nullptr, jf, if_, v, m1, n1);
+      if (msk1[c] != 0) {
+        nr += (double)v * (double)v;
+        const double bb = (double)b[c];
+        nb += bb * bb;
+      }
+    }
+  }
+  if (flags & 12) {
+    for (int o = 32; o > 0; o >>= 1) {
+      nr += __shfl_xor(nr, o, 64);
+      nb += __shfl_xor(nb, o, 64);
+    }
+    if ((tid & 63) == 0) {
+      red[0][tid >> 6] = nr;
+      red[1][tid >> 6] = nb;
+    }
+  }
+  __syncthreads();
+  if ((flags & 12) && tid == 0) {
+    double a = 0., c = 0.;
+    for (int w = 0; w < nt / 64; w++) {
+      a += red[0][w];
+      c += red[1][w];
+    }
+    partial[2 * blockIdx.x] = a;
+    partial[2 * blockIdx.x + 1] = c;
+  }
+  T* b2 = P + Lc.b;
+  for (int q = tid; q < TY2 * TX2; q += nt) {
+    const int ry = q / TX2, rx = q % TX2;
+    const int J2 = tk.y0 + ry, I2 = tk.x0 + rx;
+    if (J2 >= m2 - NH || I2 >= n2 - NH) continue;
+    T v = T(0);
+    if (msk2[at(J2, I2, n2)] != 0) {
+      const int cy = 2 * ry + 1, cx = 2 * rx + 1;
+      v = restrict_r<T>([&](int dj, int di) { return rs[cy + dj][cx + di]; });
+    }
+    store_img_mask<T>(b2, nullptr, J2, I2, v, m2, n2);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_coarse: every level from nfine to the coarsest of one instance, in LDS.
+// mode 0: V-cycle tail with x(nfine) = 0 (hierarchy.py:98-126 from a level
+// above); mode 1: F-cycle tail (hierarchy.py:131-151 from nfine).
+// Input b(nfine) from global, output x(nfine) (filled) to tk.xout.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct CL {
+  int m, n;
+  int8_t* msk;
+  T *A, *x, *b, *t, *r;
+};
+
+template <typename T>
+__device__ void c_smooth(const CL<T>& L, T c1, T c2, int tid, int nt) {
+  const int m = L.m, n = L.n;
+  const size_t P = (size_t)m * n;
+  const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2;
+  for (int q = tid; q < wj * wi; q += nt) {
+    const int j = NH - 1 + q / wi, i = NH - 1 + q % wi;
+    const size_t c = at(j, i, n);
+    L.t[c] = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
+                         [&](int dd) { return L.x[c + dd]; });
+  }
+  __syncthreads();
+  const int N = n - 2 * NH, M = m - 2 * NH;
+  for (int q = tid; q < M * N; q += nt) {
+    const int J = NH + q / N, I2 = NH + q % N;
+    const size_t c = at(J, I2, n);
+    const T v = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
+                            [&](int dd) { return L.t[c + dd]; });
+    store_img_mask<T>(L.x, nullptr, J, I2, v, m, n);
+  }
+  __syncthreads();
+}
+template <typename T>
+__device__ void c_zero(T* a, int cnt, int tid, int nt) {
+  for (int q = tid; q < cnt; q += nt) a[q] = T(0);
+  __syncthreads();
+}
+template <typename T>
+__device__ void c_resid_restrict(const CL<T>& F, const CL<T>& C, bool restrict_b, int tid,
+                                 int nt) {
+  const int m = F.m, n = F.n;
+  const size_t P = (size_t)m * n;
+  const int N = n - 2 * NH, M = m - 2 * NH;
+  if (!restrict_b) {
+    for (int q = tid; q < M * N; q += nt) {
+      const int J = NH + q / N, I2 = NH + q % N;
+      const size_t c = at(J, I2, n);
+      const T v = residual_x<T>(F.msk[c], F.A, P, c, n, F.b[c], [&](int dd) { return F.x[c + dd]; });
+      store_img_mask<T>(F.r, nullptr, J, I2, v, m, n);
+    }
+    __syncthreads();
+  }
+  const T* src = restrict_b ? F.b : F.r;
+  const int N2 = C.n - 2 * NH, M2 = C.m - 2 * NH;
+  for (int q = tid; q < M2 * N2; q += nt) {
+    const int J2 = NH + q / N2, I2 = NH + q % N2;
+    T v = T(0);
+    if (C.msk[at(J2, I2, C.n)] != 0) {
+      const size_t c = at(2 * J2 + 1 - NH, 2 * I2 + 1 - NH, n);
+      v = restrict_r<T>([&](int dj, int di) { return src[c + (ptrdiff_t)dj * n + di]; });
+    }
+    store_img_mask<T>(C.b, nullptr, J2, I2, v, C.m, C.n);
+  }
+  __syncthreads();
+}
+template <typename T>
+__device__ void c_interp(const CL<T>& F, const CL<T>& C, bool acc, int tid, int nt) {
+  const int cnt = F.m * F.n;
+  for (int q = tid; q < cnt; q += nt) {
+    const int j = q / F.n, i = q % F.n;
+    const T r = interp_at<T>(F.msk, F.n, C.msk, C.x, C.n, j, i);
+    F.x[q] = acc ? F.x[q] + r : r;
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void c_vcycle(CL<T>* Ls, int lev1, int deep, bool zero_first, T c1, T c2, int tid,
+                         int nt) {
+  for (int l = lev1; l < deep; l++) {
+    if (l > lev1 || zero_first) c_zero(Ls[l].x, Ls[l].m * Ls[l].n, tid, nt);
+    c_smooth(Ls[l], c1, c2, tid, nt);
+    c_resid_restrict(Ls[l], Ls[l + 1], false, tid, nt);
+  }
+  c_zero(Ls[deep].x, Ls[deep].m * Ls[deep].n, tid, nt);
+  for (int k = 0; k < 16; k++) c_smooth(Ls[deep], c1, c2, tid, nt);
+  for (int l = deep - 1; l >= lev1; l--) {
+    c_interp(Ls[l], Ls[l + 1], true, tid, nt);
+    c_smooth(Ls[l], c1, c2, tid, nt);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_coarse(const Task* __restrict__ tasks,
+                                                const InstD* __restrict__ I, T* P,
+                                                const int8_t* __restrict__ B,
+                                                const DynD* __restrict__ dyn, int pred, int mode) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const Task tk = tasks[blockIdx.x];
+  if (skip(dyn, tk.inst, pred)) return;
+  const InstD& in = I[tk.inst];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int k0 = in.nfine, deep = in.nlev - 1, nl = deep - k0 + 1;
+  __shared__ CL<T> Ls[MAXLEV];
+  // carve LDS: per level A(5), x, b, t, r (T), then masks
+  size_t off = 0;
+  if (tid == 0)
+  for (int l = 0; l < nl; l++) {
+    const LevD& G = in.L[k0 + l];
+    const int sz = G.m * G.n;
+    Ls[l].m = G.m;
+    Ls[l].n = G.n;
+    Ls[l].A = (T*)(lds + off);
+    off += sizeof(T) * 5 * sz;
+    Ls[l].x = (T*)(lds + off);
+    off += sizeof(T) * sz;
+    Ls[l].b = (T*)(lds + off);
+    off += sizeof(T) * sz;
+    Ls[l].t = (T*)(lds + off);
+    off += sizeof(T) * sz;
+    Ls[l].r = (T*)(lds + off);
+    off += sizeof(T) * sz;
+  }
+  if (tid == 0)
+  for (int l = 0; l < nl; l++) {
+    const LevD& G = in.L[k0 + l];
+    Ls[l].msk = (int8_t*)(lds + off);
+    off += (G.m * G.n + 15) & ~15;
+  }
+  __syncthreads();
+  for (int l = 0; l < nl; l++) {
+    const LevD& G = in.L[k0 + l];
+    const int sz = G.m * G.n;
+    const T* Ag = P + G.A;
+    for (int q = tid; q < 5 * sz; q += nt) Ls[l].A[q] = Ag[q];
+    const int8_t* mg = B + G.msk;
+    for (int q = tid; q < sz; q += nt) Ls[l].msk[q] = mg[q];
+    for (int q = tid; q < sz; q += nt) Ls[l].t[q] = T(0);
+  }
+  {
+    const int sz = Ls[0].m * Ls[0].n;
+    const T* bg = P + in.L[k0].b;
+    for (int q = tid; q < sz; q += nt) Ls[0].b[q] = bg[q];
+  }
+  __syncthreads();
+  const T c1 = T(in.omega), c2 = T(1. - in.omega);
+  if (mode == 0) {
+    c_vcycle(Ls, 0, nl - 1, true, c1, c2, tid, nt);
+  } else {
+    for (int l = 0; l < nl - 1; l++) c_resid_restrict(Ls[l], Ls[l + 1], true, tid, nt);
+    c_zero(Ls[nl - 1].x, Ls[nl - 1].m * Ls[nl - 1].n, tid, nt);
+    for (int k = 0; k < 16; k++) c_smooth(Ls[nl - 1], c1, c2, tid, nt);
+    for (int l = nl - 2; l >= 0; l--) {
+      c_interp(Ls[l], Ls[l + 1], false, tid, nt);
+      c_vcycle(Ls, l, nl - 1, false, c1, c2, tid, nt);
+    }
+  }
+  T* xg = P + tk.xout;
+  const int sz = Ls[0].m * Ls[0].n;
+  for (int q = tid; q < sz; q += nt) xg[q] = Ls[0].x[q];
+}
+
+// ---------------------------------------------------------------------------
+// k_ortho: (u,v) from psi on [1, m-2] (computeorthogradient)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_ortho(const Task* __restrict__ tasks,
+                                               const InstD* __restrict__ I, T* P,
+                                               const int8_t* __restrict__ B) {
+  const Task tk = tasks[blockIdx.x];
+  const InstD& in = I[tk.inst];
+  const int m = in.m, n = in.n;
+  const size_t F = (size_t)m * n;
+  const int8_t* msk = B + in.msk;
+  T* base = P + tk.xout;  // state-shaped (psi at +F, u at +2F, v at +3F)
+  const T* psi = base + F;
+  const T zdx = T(1. / in.dx), zdy = T(1. / in.dy);
+  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+    const int j = tk.y0 + r, i = tk.x0 + threadIdx.x;
+    if (j < 1 || i < 1 || j >= m - 1 || i >= n - 1) continue;
+    const size_t c = at(j, i, n);
+    base[2 * F + c] = (msk[c] + msk[c + 1] == 2) ? zdy * (psi[c - n] - psi[c]) : T(0);
+    base[3 * F + c] = (msk[c] + msk[c + n] == 2) ? zdx * (psi[c] - psi[c - 1]) : T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// flat (element-wise) kernels: task.y0 = first element, task.x0 = count
+// ---------------------------------------------------------------------------
+constexpr int FLAT = 4096;
+
+// final RK combination of the fields in fmask (bit k: field k)
+template <typename T>
+__global__ void __launch_bounds__(256) k_combine(const Task* __restrict__ tasks,
+                                                 const InstD* __restrict__ I, T* P,
+                                                 const DynD* __restrict__ dyn) {
+  const Task tk = tasks[blockIdx.x];
+  const InstD& in = I[tk.inst];
+  const size_t F = (size_t)in.m * in.n;
+  const double dt = dyn[tk.inst].dt;
+  T* x = P + in.state;
+  const T *k0 = P + in.k[0], *k1 = P + in.k[1], *k2 = P + in.k[2];
+  const int field = tk.lev;
+  const size_t base = (size_t)field * F;
+  for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
+    const size_t e = base + q;
+    if (in.tscheme == 0) {
+      const T c6 = T(dt / 6.);
+      x[e] = x[e] + c6 * ((k0[e] + k1[e]) + T(4) * k2[e]);
+    } else {
+      x[e] = x[e] + T(dt) * k2[e];
+    }
+    if (in.model == 1 && field == 4) x[5 * F + q] = x[e] - T(0);  // banom = b - bref(=0)
+  }
+}
+
+// psi += x0 over the whole array (hierarchy.py:171)
+template <typename T>
+__global__ void __launch_bounds__(256) k_update(const Task* __restrict__ tasks,
+                                                const InstD* __restrict__ I, T* P,
+                                                const DynD* __restrict__ dyn, int pred) {
+  const Task tk = tasks[blockIdx.x];
+  if (skip(dyn, tk.inst, pred)) return;
+  T* x = P + tk.xout;
+  const T* y = P + tk.xin;
+  for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) x[q] = x[q] + y[q];
+}
+
+// psi = (psi - psim) * mskp (core/operators.py:474-481), psim used if perio
+template <typename T>
+__global__ void __launch_bounds__(256) k_finalize_psi(const Task* __restrict__ tasks,
+                                                      const InstD* __restrict__ I, T* P,
+                                                      const int8_t* __restrict__ B,
+                                                      const DynD* __restrict__ dyn) {
+  const Task tk = tasks[blockIdx.x];
+  const InstD& in = I[tk.inst];
+  const size_t F = (size_t)in.m * in.n;
+  T* psi = P + in.state + F;
+  const int8_t* mp = B + in.mskp;
+  const T psim = T(dyn[tk.inst].psim);
+  for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
+    T v = psi[q];
+    if (in.perio) v = v - psim;
+    psi[q] = v * T(mp[q]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+// NumPy pairwise sum of interior elements [s, s+cnt) (flattened C order)
+template <typename T>
+__device__ double np_pairwise_flat(const T* x, int n, int N, size_t start, size_t cnt) {
+  struct Fr {
+    size_t s, c;
+    double left;
+    int stage;
+  };
+  Fr st[16];
+  int sp = 0;
+  st[0] = {start, cnt, 0., 0};
+  double ret = 0.;
+  auto val = [&](size_t q) { return (double)x[at(NH + (int)(q / N), NH + (int)(q % N), n)]; };
+  while (sp >= 0) {
+    Fr& f = st[sp];
+    if (f.c <= 128) {
+      double r;
+      if (f.c < 8) {
+        r = -0.0;
+        for (size_t k = 0; k < f.c; k++) r += val(f.s + k);
+      } else {
+        double a[8];
+        for (int k = 0; k < 8; k++) a[k] = val(f.s + k);
+        size_t k;
+        for (k = 8; k < f.c - (f.c % 8); k += 8)
+          for (int q = 0; q < 8; q++) a[q] += val(f.s + k + q);
+        r = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+        for (; k < f.c; k++) r += val(f.s + k);
+      }
+      ret = r;
+      sp--;
+      while (sp >= 0) {
+        Fr& pp = st[sp];
+        if (pp.stage == 1) {
+          pp.left = ret;
+          pp.stage = 2;
+          size_t h = pp.c / 2;
+          h -= h % 8;
+          st[++sp] = {pp.s + h, pp.c - h, 0., 0};
+          break;
+        } else {
+          ret = pp.left + ret;
+          sp--;
+        }
+      }
+      continue;
+    }
+    size_t h = f.c / 2;
+    h -= h % 8;
+    f.stage = 1;
+    st[sp + 1] = {f.s, h, 0., 0};
+    sp++;
+  }
+  return ret;
+}
+
+// one thread per 8192-element chunk of psi's interior; task.y0 = first chunk
+// index, task.x0 = chunk count; results at partial[task-local slot]
+template <typename T>
+__global__ void k_npsum(const Task* __restrict__ tasks, const InstD* __restrict__ I, T* P,
+                        double* partial, const int* __restrict__ slot) {
+  const Task tk = tasks[blockIdx.x];
+  const InstD& in = I[tk.inst];
+  if (!in.perio) return;
+  const int ch = threadIdx.x;
+  if (ch >= tk.x0) return;
+  const int m = in.m, n = in.n, N = n - 2 * NH;
+  const size_t tot = (size_t)(m - 2 * NH) * N;
+  const size_t s = (size_t)(tk.y0 + ch) * 8192;
+  const size_t cnt = tot - s < 8192 ? tot - s : 8192;
+  partial[slot[blockIdx.x] + ch] = np_pairwise_flat<T>(P + in.state + (size_t)m * n, n, N, s, cnt);
+}
+
+// diagnostics partials: {maxu, ke, sum w, sum w^2, d4, d5, d6, d7}
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
+                                              const InstD* __restrict__ I, T* P,
+                                              const int8_t* __restrict__ B, double* partial) {
+  __shared__ double red[8][4];
+  const Task tk = tasks[blockIdx.x];
+  const InstD& in = I[tk.inst];
+  const int m = in.m, n = in.n;
+  const size_t F = (size_t)m * n;
+  const int8_t* msk = B + in.msk;
+  const T* s = P + in.state;
+  const T *w = s, *psi = s + F, *u = s + 2 * F, *v = s + 3 * F, *f4 = s + 4 * F;
+  const double* xr = (const double*)(P) + in.xr;
+  const double* yr = (const double*)(P) + in.yr;
+  double a[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+    const int J = tk.y0 + r, Ic = tk.x0 + threadIdx.x;
+    if (J >= m - NH || Ic >= n - NH) continue;
+    const size_t c = at(J, Ic, n);
+    const int mk = msk[c];
+    if (mk == 1) {
+      const double zu = (double)u[c] * u[c] + (double)u[c - 1] * u[c - 1];
+      const double zv = (double)v[c] * v[c] + (double)v[c - n] * v[c - n];
+      a[0] = fmax(a[0], fabs((double)u[c] + u[c - 1]) + fabs((double)v[c] + v[c - n]));
+      a[1] += zu + zv;
+      a[2] += w[c];
+      a[3] += (double)w[c] * w[c];
+    }
+    if (in.model == 0) {
+      if (mk != 0) {
+        a[4] += (double)w[c] * xr[c];
+        a[5] += (double)w[c] * yr[c];
+        a[6] += psi[c];
+        a[7] += (double)w[c] * f4[c];
+      }
+    } else {
+      if (mk != 0) a[4] += (double)f4[c] * yr[c];
+      if (mk == 1) {
+        a[5] += f4[c];
+        a[6] += (double)f4[c] * f4[c];
+      }
+    }
+  }
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int k = 0; k < 8; k++) {
+    double x = a[k];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(x, o, 64);
+      x = k == 0 ? fmax(x, y) : x + y;
+    }
+    if ((tid & 63) == 0) red[k][tid >> 6] = x;
+  }
+  __syncthreads();
+  if (tid < 8) {
+    double x = red[tid][0];
+    for (int q = 1; q < (int)(blockDim.x * blockDim.y) / 64; q++)
+      x = tid == 0 ? fmax(x, red[tid][q]) : x + red[tid][q];
+    partial[(size_t)blockIdx.x * 8 + tid] = x;
+  }
+}
+
+// Per-instance fold of partial sums plus solve control.  256 threads; thread
+// t folds slots t, t+256, ... in order, then a fixed tree combines the
+// threads, so the result is the same on every run.
+// what: 0 initial norms (slot pairs {r, b}), 1 iteration norms,
+//       2 diagnostics (8 per slot), 3 NumPy chunks -> psim (sequential,
+//       as NumPy adds its chunk sums in order)
+struct FoldSpan {
+  int first, count;  // partial slots of this instance
+};
+__global__ void __launch_bounds__(256) k_fold(const FoldSpan* __restrict__ spans,
+                                              const InstD* __restrict__ I, DynD* dyn,
+                                              const double* __restrict__ partial, int what,
+                                              int maxite, double tol, int pred, int adapt_dt,
+                                              double cfl, double dtmax) {
+  __shared__ double sh[8][256];
+  const int inst = blockIdx.x, tid = threadIdx.x;
+  DynD& d = dyn[inst];
+  const FoldSpan sp = spans[inst];
+  if (sp.count == 0) return;
+  if (what == 3) {
+    if (tid != 0 || !I[inst].perio) return;
+    double acc = 0.;
+    for (int k = 0; k < sp.count; k++) acc = acc + partial[sp.first + k];
+    d.psim = acc * 1. / I[inst].area;
+    return;
+  }
+  if (what != 2 && pred && !d.active) return;
+  const int nv = what == 2 ? 8 : 2;
+  double a[8];
+  for (int k = 0; k < nv; k++) a[k] = 0.;
+  for (int q = tid; q < sp.count; q += 256)
+    for (int k = 0; k < nv; k++) {
+      const double y = partial[(size_t)(sp.first + q) * nv + k];
+      a[k] = (what == 2 && k == 0) ? fmax(a[k], y) : a[k] + y;
+    }
+  for (int k = 0; k < nv; k++) sh[k][tid] = a[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h)
+      for (int k = 0; k < nv; k++)
+        sh[k][tid] = (what == 2 && k == 0) ? fmax(sh[k][tid], sh[k][tid + h])
+                                           : sh[k][tid] + sh[k][tid + h];
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  for (int k = 0; k < nv; k++) a[k] = sh[k][0];
+  if (what == 2) {
+    a[0] *= 0.5;
+    a[1] *= 0.25;
+    if (I[inst].model == 1) a[4] = -I[inst].gravity * a[4];
+    for (int k = 0; k < 8; k++) d.diag[k] = a[k];
+    d.umax = a[0];  // cst[3] of the next step (core/fluid2d.py:397)
+    if (adapt_dt && a[0] != 0.) {  // core/fluid2d.py:378-391
+      double dt = cfl * fmin(I[inst].dx, I[inst].dy) / a[0];
+      if (dt > dtmax) dt = dtmax;
+      d.dt = dt;
+    }
+    return;
+  }
+  const double nr = a[0], nb = a[1];
+  if (what == 0) {  // core/gmg/hierarchy.py:159-169
+    d.normeb = sqrt(nb / 1.);
+    d.nite = 0;
+    d.ndiv = 0;
+    d.diverged = 0;
+    if (!(d.normeb > 0)) {
+      d.res = 0.;
+      d.res0 = 0.;
+      d.active = 0;
+      return;
+    }
+    d.res0 = sqrt(nr / 1.) / d.normeb;
+    d.res = d.res0;
+    d.active = (maxite > 0 && d.res0 > tol) ? 1 : 0;
+    return;
+  }
+  // what == 1: core/gmg/hierarchy.py:172-188
+  const double res = sqrt(nr / 1.) / d.normeb;
+  const double conv = d.res0 / res;
+  d.res0 = res;
+  d.res = res;
+  d.nite += 1;
+  if (conv < 1) d.ndiv += 1;
+  if (d.ndiv > 4) d.diverged = 1;
+  d.active = (d.nite < maxite && d.res0 > tol && !d.diverged) ? 1 : 0;
+}
+
+}  // namespace eng
+}  // namespace f2d
+
+// ===========================================================================
+// host side: batch layout, launch program, graph
+// ===========================================================================
+extern int f2d_gmg_build_for_engine(int nx, int ny, double dx, double dy, double hydroepsilon,
+                                    const int8_t* mskp_host, void** gmg_out);
+extern int f2d_gmg_level_raw(void* g, int lev, int8_t** msk, double** A, int* m, int* n);
+extern void f2d_gmg_free_raw(void* g);
+
+namespace f2d {
+namespace eng {
+
+static thread_local std::string e_err;
+static int efail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int efail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  e_err = buf;
+  return code;
+}
+#define ECHK(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return efail(F2D_E_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #expr,              \
+                   hipGetErrorString(e_));                                             \
+  } while (0)
+#define ETRY(expr)                   \
+  do {                               \
+    int rc_ = (expr);                \
+    if (rc_ != F2D_OK) return rc_;   \
+  } while (0)
+
+enum Kind {
+  L_RHS,
+  L_C2C,
+  L_SMOOTH,
+  L_RR,
+  L_COARSE,
+  L_ORTHO,
+  L_COMBINE,
+  L_UPDATE,
+  L_FINPSI,
+  L_NPSUM,
+  L_DIAG,
+  L_FOLD
+};
+
+struct Launch {
+  int kind = 0;
+  int toff = 0, ntask = 0;   // task table slice
+  int pred = 0, flags = 0;   // predication, variant flags
+  int stage = 0, outk = 0;   // RK stage, tendency slot (k_rhs)
+  int mode = 0;              // coarse mode / fold what
+  int span_off = 0;          // fold spans (ninst entries)
+  int slot_off = 0;          // npsum slots
+  size_t shmem = 0;          // k_coarse dynamic LDS
+};
+
+struct HostInst {
+  InstD d;
+  f2d_model_params p;
+  long long x0alt = 0;
+  int cur[MAXLEV];
+  long long buf[MAXLEV][2];  // x buffers per level (level 0 set per phase)
+  size_t coarse_lds = 0;
+};
+
+struct Batch {
+  int ninst = 0;
+  std::vector<HostInst> H;
+  double* P = nullptr;
+  int8_t* B = nullptr;
+  size_t nP = 0, nB = 0;
+  InstD* dI = nullptr;
+  DynD* dD = nullptr;
+  DynD* hD = nullptr;  // pinned mirror
+  Task* dT = nullptr;
+  FoldSpan* dS = nullptr;
+  int* dSlot = nullptr;
+  double* partial = nullptr;
+  size_t npartial = 0;
+  std::vector<Task> tasks;
+  std::vector<FoldSpan> spans;
+  std::vector<int> slots;
+  std::vector<Launch> prog_init, prog_step;
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool use_graph = true;
+  int adapt = 0;
+  double cfl = 0.8, dtmax = 5.0;
+  int maxite = 4;
+  double tol = 1e-11;
+  size_t max_shmem = 0;
+};
+
+// ---- task helpers ----------------------------------------------------------
+static Task mk_task(int inst, int lev, int y0, int x0) {
+  Task t;
+  memset(&t, 0, sizeof t);
+  t.inst = inst;
+  t.lev = lev;
+  t.y0 = y0;
+  t.x0 = x0;
+  return t;
+}
+
+// tiles of TY x TX over rows [r0, r1) x cols [c0, c1)
+static void tiles(std::vector<Task>& out, const Task& proto, int r0, int r1, int c0, int c1) {
+  for (int y = r0; y < r1; y += TY)
+    for (int x = c0; x < c1; x += TX) {
+      Task t = proto;
+      t.y0 = y;
+      t.x0 = x;
+      out.push_back(t);
+    }
+}
+
+struct Builder {
+  Batch& b;
+  std::vector<Launch>* prog;
+  explicit Builder(Batch& bb) : b(bb), prog(nullptr) {}
+
+  Launch& add(int kind, const std::vector<Task>& ts) {
+    Launch L;
+    L.kind = kind;
+    L.toff = (int)b.tasks.size();
+    L.ntask = (int)ts.size();
+    b.tasks.insert(b.tasks.end(), ts.begin(), ts.end());
+    prog->push_back(L);
+    return prog->back();
+  }
+  // spans of a reduction launch (tasks grouped by instance, contiguous)
+  int spans_of(const std::vector<Task>& ts) {
+    const int off = (int)b.spans.size();
+    std::vector<FoldSpan> sp(b.ninst, FoldSpan{0, 0});
+    for (int k = 0; k < (int)ts.size(); k++) {
+      FoldSpan& f = sp[ts[k].inst];
+      if (f.count == 0) f.first = k;
+      f.count++;
+    }
+    b.spans.insert(b.spans.end(), sp.begin(), sp.end());
+    return off;
+  }
+  void fold(int what, int span_off, int pred) {
+    Launch L;
+    L.kind = L_FOLD;
+    L.mode = what;
+    L.span_off = span_off;
+    L.pred = pred;
+    prog->push_back(L);
+  }
+
+  const LevD& lev(int i, int l) { return b.H[i].d.L[l]; }
+
+  // smoother on level l of instance i: input mode, output the other buffer
+  Task smooth_task(int i, int l, int mode, int final_mask, std::vector<Task>& out) {
+    HostInst& h = b.H[i];
+    Task t = mk_task(i, l, 0, 0);
+    t.mode = mode;
+    t.flags = final_mask;
+    t.xin = h.buf[l][h.cur[l]];
+    t.xout = h.buf[l][h.cur[l] ^ 1];
+    if (mode == 2 || mode == 3) t.xc = h.buf[l + 1][h.cur[l + 1]];
+    const LevD& L = lev(i, l);
+    tiles(out, t, NH, L.m - NH, NH, L.n - NH);
+    h.cur[l] ^= 1;
+    return t;
+  }
+  // residual (or b) of level l restricted onto l+1
+  void rr_tasks(int i, int l, long long xin, long long rhs, long long rstore,
+                std::vector<Task>& out) {
+    const LevD& C = lev(i, l + 1);
+    Task t = mk_task(i, l, 0, 0);
+    t.xin = xin;
+    t.xout = rhs;
+    t.xc = rstore;
+    for (int y = NH; y < C.m - NH; y += TY2)
+      for (int x = NH; x < C.n - NH; x += TX2) {
+        Task u = t;
+        u.y0 = y;
+        u.x0 = x;
+        out.push_back(u);
+      }
+  }
+
+  // V-cycles of the instances in `who`; instance i starts at top-down level
+  // start[i] (its bottom-aligned index D_i = nfine_i - 1 - start[i]) with
+  // smoother mode m0 there; levels below are zeroed; the coarse part runs in
+  // k_coarse.  final_top: mask the last smooth of the start level by mskp.
+  void vcycle(const std::vector<int>& who, const std::vector<int>& start, int m0, int pred,
+              bool final_top) {
+    int Dmax = -1;
+    for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1 - start[i]);
+    for (int dd = Dmax; dd >= 0; dd--) {
+      std::vector<Task> ts, rs;
+      for (int i : who) {
+        const int D = b.H[i].d.nfine - 1 - start[i];
+        if (dd > D) continue;
+        const int l = b.H[i].d.nfine - 1 - dd;
+        smooth_task(i, l, dd == D ? m0 : 1, 0, ts);
+        HostInst& h = b.H[i];
+        rr_tasks(i, l, h.buf[l][h.cur[l]], -1, -1, rs);
+      }
+      if (!ts.empty()) {
+        add(L_SMOOTH, ts).pred = pred;
+        add(L_RR, rs).pred = pred;
+      }
+    }
+    coarse(who, 0, pred);
+    for (int dd = 0; dd <= Dmax; dd++) {
+      std::vector<Task> ts;
+      for (int i : who) {
+        const int D = b.H[i].d.nfine - 1 - start[i];
+        if (dd > D) continue;
+        const int l = b.H[i].d.nfine - 1 - dd;
+        smooth_task(i, l, 2, (final_top && dd == D) ? 1 : 0, ts);
+      }
+      if (!ts.empty()) add(L_SMOOTH, ts).pred = pred;
+    }
+  }
+
+  void coarse(const std::vector<int>& who, int mode, int pred) {
+    std::vector<Task> ts;
+    size_t sh = 0;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      const int k = h.d.nfine;
+      Task t = mk_task(i, k, 0, 0);
+      h.cur[k] ^= 1;
+      t.xout = h.buf[k][h.cur[k]];
+      ts.push_back(t);
+      sh = std::max(sh, h.coarse_lds);
+    }
+    Launch& L = add(L_COARSE, ts);
+    L.mode = mode;
+    L.pred = pred;
+    L.shmem = sh;
+  }
+
+  using Off = std::function<long long(HostInst&)>;
+
+  void c2c(const std::vector<int>& who, Off src, Off dst, int pred) {
+    std::vector<Task> ts;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      Task t = mk_task(i, 0, 0, 0);
+      t.xin = src(h);
+      t.xout = dst(h);
+      tiles(ts, t, 0, h.d.m - 1, 0, h.d.n - 1);
+    }
+    add(L_C2C, ts).pred = pred;
+  }
+
+  void ortho(const std::vector<int>& who, Off base) {
+    std::vector<Task> ts;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      Task t = mk_task(i, 0, 0, 0);
+      t.xout = base(h);
+      tiles(ts, t, 1, h.d.m - 1, 1, h.d.n - 1);
+    }
+    add(L_ORTHO, ts);
+  }
+
+  // flat element ranges [0, cnt) per instance
+  std::vector<Task> flat(const std::vector<int>& who, Off cnt, int field) {
+    std::vector<Task> ts;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      const long long c = cnt(h);
+      for (long long q = 0; q < c; q += FLAT) {
+        Task t = mk_task(i, field, (int)q, (int)std::min<long long>(FLAT, c - q));
+        ts.push_back(t);
+      }
+    }
+    return ts;
+  }
+
+  std::vector<int> all() {
+    std::vector<int> v(b.ninst);
+    for (int i = 0; i < b.ninst; i++) v[i] = i;
+    return v;
+  }
+
+  // fast inversion of tendency slot k (core/operators.py:421-498, 'fast')
+  void invert_fast(const std::vector<int>& who, int k) {
+    if (who.empty()) return;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      const size_t F = (size_t)h.d.m * h.d.n;
+      h.buf[0][0] = h.d.k[k] + F;  // psi tendency: the guess and the result
+      h.buf[0][1] = h.x0alt;
+      h.cur[0] = 0;
+    }
+    c2c(who, [k](HostInst& h) { return h.d.k[k]; }, [](HostInst& h) { return h.d.L[0].b; }, 0);
+    std::vector<int> st(b.ninst, 0);
+    vcycle(who, st, 0, 0, false);
+    vcycle(who, st, 0, 0, true);
+    for (int i : who)
+      if (b.H[i].cur[0] != 0) fprintf(stderr, "engine: fast inversion parity\n");
+    ortho(who, [k](HostInst& h) { return h.d.k[k]; });
+  }
+
+  // full solve of the state's psi (core/operators.py:470-481)
+  void solve_full(const std::vector<int>& who) {
+    if (who.empty()) return;
+    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0);
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      h.buf[0][0] = h.d.x0[0];
+      h.buf[0][1] = h.d.x0[1];
+      h.cur[0] = 0;
+    }
+    auto resid_norm = [&](int what, int pred) {
+      std::vector<Task> rs;
+      for (int i : who) {
+        HostInst& h = b.H[i];
+        const size_t F = (size_t)h.d.m * h.d.n;
+        rr_tasks(i, 0, h.d.state + F, h.d.work, h.d.L[0].b, rs);
+      }
+      const int sp = spans_of(rs);
+      Launch& L = add(L_RR, rs);
+      L.flags = 1 | 4 | 8 | 16;
+      L.pred = pred;
+      fold(what, sp, pred);
+    };
+    resid_norm(0, 0);
+    for (int it = 0; it < b.maxite; it++) {
+      // F-cycle: b restricted down the fine levels (b0 -> b1 came with the
+      // residual), coarse tail, then V-cycles from each fine level upward
+      int Dmax = 0;
+      for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1);
+      for (int dd = Dmax - 1; dd >= 0; dd--) {
+        std::vector<Task> rs;
+        for (int i : who) {
+          const int l = b.H[i].d.nfine - 1 - dd;
+          if (l < 1) continue;
+          rr_tasks(i, l, 0, -1, -1, rs);
+        }
+        if (!rs.empty()) {
+          Launch& L = add(L_RR, rs);
+          L.flags = 2;
+          L.pred = 1;
+        }
+      }
+      coarse(who, 1, 1);
+      for (int dd = 0; dd <= Dmax; dd++) {
+        std::vector<int> sub, st(b.ninst, 0);
+        for (int i : who)
+          if (b.H[i].d.nfine - 1 >= dd) {
+            sub.push_back(i);
+            st[i] = b.H[i].d.nfine - 1 - dd;
+          }
+        vcycle(sub, st, 3, 1, false);
+      }
+      std::vector<Task> up;
+      for (int i : who) {
+        HostInst& h = b.H[i];
+        const size_t F = (size_t)h.d.m * h.d.n;
+        for (size_t q = 0; q < F; q += FLAT) {
+          Task t = mk_task(i, 0, (int)q, (int)std::min<size_t>(FLAT, F - q));
+          t.xout = h.d.state + F;
+          t.xin = h.buf[0][h.cur[0]];
+          up.push_back(t);
+        }
+      }
+      add(L_UPDATE, up).pred = 1;
+      resid_norm(1, 1);
+    }
+    // zero mean (perio) and mask, then (u, v)
+    std::vector<Task> ns;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      const long long tot = (long long)(h.d.m - 2 * NH) * (h.d.n - 2 * NH);
+      const int nch = (int)((tot + 8191) / 8192);
+      for (int c = 0; c < nch; c += 256) {
+        Task t = mk_task(i, 0, c, std::min(256, nch - c));
+        ns.push_back(t);
+      }
+    }
+    {
+      const int soff = (int)b.slots.size();
+      std::vector<FoldSpan> sp(b.ninst, FoldSpan{0, 0});
+      int slot = 0;
+      for (const Task& t : ns) {
+        b.slots.push_back(slot);
+        if (sp[t.inst].count == 0) sp[t.inst].first = slot;
+        sp[t.inst].count += t.x0;
+        slot += t.x0;
+      }
+      const int spoff = (int)b.spans.size();
+      b.spans.insert(b.spans.end(), sp.begin(), sp.end());
+      Launch& L = add(L_NPSUM, ns);
+      L.slot_off = soff;
+      fold(3, spoff, 0);
+    }
+    add(L_FINPSI, flat(who, [](HostInst& h) { return (long long)h.d.m * h.d.n; }, 1));
+    ortho(who, [](HostInst& h) { return h.d.state; });
+  }
+
+  void diag() {
+    std::vector<Task> ts;
+    for (int i = 0; i < b.ninst; i++) {
+      HostInst& h = b.H[i];
+      tiles(ts, mk_task(i, 0, 0, 0), NH, h.d.m - NH, NH, h.d.n - NH);
+    }
+    const int sp = spans_of(ts);
+    add(L_DIAG, ts);
+    fold(2, sp, 0);
+  }
+
+  void rk_step() {
+    std::vector<int> who = all(), eul;
+    const bool ssp = b.H[0].d.tscheme == 0;
+    const int kforcing = ssp ? 2 : 0;
+    for (int stage = 0; stage < 3; stage++) {
+      std::vector<Task> ts;
+      for (int i : who) {
+        HostInst& h = b.H[i];
+        tiles(ts, mk_task(i, 0, 0, 0), NH, h.d.m - NH, NH, h.d.n - NH);
+      }
+      Launch& L = add(L_RHS, ts);
+      L.stage = stage;
+      L.outk = stage;
+      // Euler inverts except at stage kforcing (unless diffusion);
+      // Boussinesq inverts at every stage (core/euler.py:156-169,
+      // core/boussinesq.py:114)
+      std::vector<int> inv;
+      for (int i : who) {
+        const InstD& d = b.H[i].d;
+        if (d.model == 1 || stage != kforcing || d.diffusion) inv.push_back(i);
+      }
+      invert_fast(inv, stage);
+    }
+    // final combination: Euler omega, psi; Boussinesq all fields (+ banom)
+    std::vector<Task> cs;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      const int nf = h.d.model == 0 ? 2 : 5;
+      for (int f = 0; f < nf; f++) {
+        auto t = flat({i}, [](HostInst& hh) { return (long long)hh.d.m * hh.d.n; }, f);
+        cs.insert(cs.end(), t.begin(), t.end());
+      }
+      if (h.d.model == 0) eul.push_back(i);
+    }
+    add(L_COMBINE, cs);
+    solve_full(eul);
+    diag();
+  }
+};
+
+// ---- execution ------------------------------------------------------------
+static int run_prog(Batch& b, const std::vector<Launch>& prog) {
+  const dim3 B2(64, 4), B1(256);
+  for (const Launch& L : prog) {
+    const Task* T = b.dT + L.toff;
+    const unsigned g = (unsigned)L.ntask;
+    if (L.kind != L_FOLD && g == 0) continue;
+    switch (L.kind) {
+      case L_RHS:
+        hipLaunchKernelGGL(k_rhs<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B, b.dD, L.stage,
+                           L.outk);
+        break;
+      case L_C2C:
+        hipLaunchKernelGGL(k_c2c<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.dD, L.pred);
+        break;
+      case L_SMOOTH:
+        hipLaunchKernelGGL(k_smooth<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B, b.dD,
+                           L.pred);
+        break;
+      case L_RR:
+        hipLaunchKernelGGL(k_resid_restrict<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B,
+                           b.dD, L.pred, L.flags, b.partial);
+        break;
+      case L_COARSE:
+        hipLaunchKernelGGL(k_coarse<double>, dim3(g), dim3(256), L.shmem, b.s, T, b.dI, b.P, b.B,
+                           b.dD, L.pred, L.mode);
+        break;
+      case L_ORTHO:
+        hipLaunchKernelGGL(k_ortho<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B);
+        break;
+      case L_COMBINE:
+        hipLaunchKernelGGL(k_combine<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.dD);
+        break;
+      case L_UPDATE:
+        hipLaunchKernelGGL(k_update<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.dD, L.pred);
+        break;
+      case L_FINPSI:
+        hipLaunchKernelGGL(k_finalize_psi<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.B, b.dD);
+        break;
+      case L_NPSUM:
+        hipLaunchKernelGGL(k_npsum<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.partial,
+                           b.dSlot + L.slot_off);
+        break;
+      case L_DIAG:
+        hipLaunchKernelGGL(k_diag<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B, b.partial);
+        break;
+      case L_FOLD:
+        hipLaunchKernelGGL(k_fold, dim3(b.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
+                           b.partial, L.mode, b.maxite, b.tol, L.pred, b.adapt, b.cfl, b.dtmax);
+        break;
+    }
+  }
+  ECHK(hipGetLastError());
+  return F2D_OK;
+}
+
+static void batch_free(Batch* b) {
+  if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->P) (void)hipFree(b->P);
+  if (b->B) (void)hipFree(b->B);
+  if (b->dI) (void)hipFree(b->dI);
+  if (b->dD) (void)hipFree(b->dD);
+  if (b->hD) (void)hipHostFree(b->hD);
+  if (b->dT) (void)hipFree(b->dT);
+  if (b->dS) (void)hipFree(b->dS);
+  if (b->dSlot) (void)hipFree(b->dSlot);
+  if (b->partial) (void)hipFree(b->partial);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
+}
+
+static int build_programs(Batch& b) {
+  b.tasks.clear();
+  b.spans.clear();
+  b.slots.clear();
+  b.prog_init.clear();
+  b.prog_step.clear();
+  Builder bd(b);
+  bd.prog = &b.prog_init;
+  // initial inversion of every instance's state + diagnostics.  Euler and
+  // Boussinesq both start from set_psi_from_vorticity (a full solve).
+  bd.solve_full(bd.all());
+  bd.diag();
+  bd.prog = &b.prog_step;
+  bd.rk_step();
+  size_t np = 0;
+  for (auto* pr : {&b.prog_init, &b.prog_step})
+    for (const Launch& L : *pr)
+      if (L.kind == L_RR || L.kind == L_DIAG) np = std::max(np, (size_t)L.ntask * 8);
+  np = std::max(np, (size_t)b.slots.size() * 256 + 256);
+  if (b.partial) (void)hipFree(b.partial);
+  if (b.dT) (void)hipFree(b.dT);
+  if (b.dS) (void)hipFree(b.dS);
+  if (b.dSlot) (void)hipFree(b.dSlot);
+  ECHK(hipMalloc(&b.partial, np * sizeof(double)));
+  ECHK(hipMalloc(&b.dT, std::max<size_t>(1, b.tasks.size()) * sizeof(Task)));
+  ECHK(hipMalloc(&b.dS, std::max<size_t>(1, b.spans.size()) * sizeof(FoldSpan)));
+  ECHK(hipMalloc(&b.dSlot, std::max<size_t>(1, b.slots.size()) * sizeof(int)));
+  ECHK(hipMemcpy(b.dT, b.tasks.data(), b.tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
+  ECHK(hipMemcpy(b.dS, b.spans.data(), b.spans.size() * sizeof(FoldSpan), hipMemcpyHostToDevice));
+  if (!b.slots.empty())
+    ECHK(hipMemcpy(b.dSlot, b.slots.data(), b.slots.size() * sizeof(int), hipMemcpyHostToDevice));
+  return F2D_OK;
+}
+
+static size_t coarse_bytes(const InstD& d, int k0) {
+  size_t sz = 0;
+  for (int l = k0; l < d.nlev; l++) sz += (size_t)d.L[l].m * d.L[l].n * (9 * sizeof(double));
+  for (int l = k0; l < d.nlev; l++) sz += ((size_t)d.L[l].m * d.L[l].n + 15) & ~(size_t)15;
+  return sz;
+}
+
+}  // namespace eng
+}  // namespace f2d
+
+using namespace f2d;
+using namespace f2d::eng;
+
+struct f2d_batch : public Batch {};
+
+extern "C" {
+
+const char* f2d_batch_last_error(void) { return e_err.c_str(); }
+
+int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* const* msks,
+                     int coarse_lds_budget, f2d_batch** out) {
+  if (ninst < 1) return efail(F2D_E_ARG, "ninst must be >= 1");
+  f2d_batch* b = new f2d_batch();
+  auto bail = [&](int rc) {
+    batch_free(b);
+    return rc;
+  };
+  b->ninst = ninst;
+  b->H.resize(ninst);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int maxsh = 65536;
+  (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  size_t budget = coarse_lds_budget > 0 ? (size_t)coarse_lds_budget : 65536;
+  budget = std::min(budget, (size_t)maxsh);
+  const int tscheme0 = params[0].timestepping;
+  // layout
+  size_t nP = 0, nB = 0;
+  auto takeP = [&](size_t cnt) {
+    const size_t o = nP;
+    nP += (cnt + 31) & ~(size_t)31;
+    return (long long)o;
+  };
+  auto takeB = [&](size_t cnt) {
+    const size_t o = nB;
+    nB += (cnt + 255) & ~(size_t)255;
+    return (long long)o;
+  };
+  for (int i = 0; i < ninst; i++) {
+    const f2d_model_params& p = params[i];
+    HostInst& h = b->H[i];
+    h.p = p;
+    InstD& d = h.d;
+    memset(&d, 0, sizeof d);
+    if (p.model != 0 && p.model != 1) return bail(efail(F2D_E_ARG, "model"));
+    if (p.timestepping != tscheme0)
+      return bail(efail(F2D_E_ARG, "one time scheme per batch"));
+    if (p.order < 1 || p.order > 6 || p.flux_splitting < 0 || p.flux_splitting > 1)
+      return bail(efail(F2D_E_ARG, "order/flux_splitting"));
+    d.model = p.model;
+    d.m = p.ny + 2 * NH;
+    d.n = p.nx + 2 * NH;
+    d.order = p.order;
+    d.method = p.flux_splitting;
+    d.perio = p.perio;
+    d.diffusion = p.diffusion;
+    d.tscheme = p.timestepping;
+    d.nvar = p.model == 0 ? 5 : 6;
+    d.dx = p.Lx / p.nx;
+    d.dy = p.Ly / p.ny;
+    d.gravity = p.gravity;
+    d.kd0 = p.kdiff_vort;
+    d.kd1 = p.kdiff_buoy;
+    d.aparab = p.order % 2 == 0 ? 0. : p.aparab;
+    d.omega = 8. / 9.;
+    // levels (core/gmg/level.py:62-115)
+    int n = p.nx, m = p.ny, nl = 0;
+    for (;;) {
+      if (nl > 0) {
+        if (n % 2 || m % 2) return bail(efail(F2D_E_ARG, "grid %dx%d not halvable", p.nx, p.ny));
+        n /= 2;
+        m /= 2;
+      }
+      if (n < NH || m < NH) return bail(efail(F2D_E_ARG, "grid too small"));
+      if (nl >= MAXLEV) return bail(efail(F2D_E_ARG, "too many levels"));
+      d.L[nl].m = m + 2 * NH;
+      d.L[nl].n = n + 2 * NH;
+      nl++;
+      if (n <= 4 || m <= 4) break;
+    }
+    if (nl < 2) return bail(efail(F2D_E_ARG, "grid must have at least two levels"));
+    d.nlev = nl;
+    int k0 = nl - 1;
+    while (k0 > 1 && coarse_bytes(d, k0 - 1) <= budget) k0--;
+    if (coarse_bytes(d, k0) > budget)
+      return bail(efail(F2D_E_ARG, "coarse levels exceed the LDS budget"));
+    d.nfine = k0;
+    h.coarse_lds = coarse_bytes(d, k0);
+    const size_t F = (size_t)d.m * d.n;
+    d.state = takeP(d.nvar * F);
+    for (int k = 0; k < 3; k++) d.k[k] = takeP(d.nvar * F);
+    d.work = takeP(F);
+    d.x0[0] = takeP(F);
+    d.x0[1] = takeP(F);
+    h.x0alt = takeP(F);
+    d.xr = takeP(F);
+    d.yr = takeP(F);
+    d.msk = takeB(F);
+    d.mskp = takeB(F);
+    for (int l = 0; l < nl; l++) {
+      LevD& L = d.L[l];
+      const size_t S = (size_t)L.m * L.n;
+      L.A = takeP(5 * S);
+      L.b = takeP(S);
+      if (l > 0) {
+        L.x[0] = takeP(S);
+        L.x[1] = takeP(S);
+      }
+      L.msk = l == 0 ? d.mskp : takeB(S);
+      h.buf[l][0] = L.x[0];
+      h.buf[l][1] = L.x[1];
+      h.cur[l] = 0;
+    }
+    long long a = 0;
+    for (int j = NH; j < d.m - NH; j++)
+      for (int q = NH; q < d.n - NH; q++) a += msks[i][(size_t)j * d.n + q];
+    d.area = (double)a;
+  }
+  b->nP = nP;
+  b->nB = nB;
+  if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess)
+    return bail(efail(F2D_E_HIP, "stream"));
+  if (hipMalloc(&b->P, nP * sizeof(double)) != hipSuccess ||
+      hipMalloc(&b->B, std::max<size_t>(nB, 1)) != hipSuccess)
+    return bail(efail(F2D_E_NOMEM, "pools (%zu doubles, %zu bytes)", nP, nB));
+  if (hipMemset(b->P, 0, nP * sizeof(double)) != hipSuccess ||
+      hipMemset(b->B, 0, std::max<size_t>(nB, 1)) != hipSuccess)
+    return bail(efail(F2D_E_HIP, "memset pools"));
+  // masks, operators: level hierarchy built by the multigrid set-up kernels
+  for (int i = 0; i < ninst; i++) {
+    HostInst& h = b->H[i];
+    InstD& d = h.d;
+    const f2d_model_params& p = h.p;
+    const size_t F = (size_t)d.m * d.n;
+    if (hipMemcpy(b->B + d.msk, msks[i], F, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(efail(F2D_E_HIP, "mask upload"));
+    // corner mask on the host (celltocorner of the float mask == 1)
+    std::vector<int8_t> mp(F, 0);
+    const int8_t* mk = msks[i];
+    for (int j = 0; j < d.m - 1; j++)
+      for (int q = 0; q < d.n - 1; q++) {
+        const size_t c = (size_t)j * d.n + q;
+        const double s4 = 0.25 * ((double)mk[c] + (double)mk[c + 1] + (double)mk[c + d.n] +
+                                  (double)mk[c + d.n + 1]);
+        mp[c] = s4 == 1.0 ? 1 : 0;
+      }
+    if (hipMemcpy(b->B + d.mskp, mp.data(), F, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(efail(F2D_E_HIP, "mskp upload"));
+    void* g = nullptr;
+    int rc = f2d_gmg_build_for_engine(p.nx, p.ny, d.dx, d.dy, p.hydroepsilon, mp.data(), &g);
+    if (rc) return bail(efail(rc, "multigrid set-up failed"));
+    for (int l = 0; l < d.nlev; l++) {
+      int8_t* gm;
+      double* gA;
+      int gmm, gnn;
+      f2d_gmg_level_raw(g, l, &gm, &gA, &gmm, &gnn);
+      const size_t S = (size_t)gmm * gnn;
+      if (gmm != d.L[l].m || gnn != d.L[l].n) {
+        f2d_gmg_free_raw(g);
+        return bail(efail(F2D_E_ARG, "level shape mismatch"));
+      }
+      if (hipMemcpy(b->P + d.L[l].A, gA, 5 * S * sizeof(double), hipMemcpyDeviceToDevice) !=
+              hipSuccess ||
+          (l > 0 && hipMemcpy(b->B + d.L[l].msk, gm, S, hipMemcpyDeviceToDevice) != hipSuccess)) {
+        f2d_gmg_free_raw(g);
+        return bail(efail(F2D_E_HIP, "level copy"));
+      }
+    }
+    f2d_gmg_free_raw(g);
+    // coordinates for the diagnostics (core/grid.py:49-52)
+    std::vector<double> xr(F), yr(F);
+    for (int j = 0; j < d.m; j++)
+      for (int q = 0; q < d.n; q++) {
+        xr[(size_t)j * d.n + q] = ((double)q + 0.5 - NH) * d.dx;
+        yr[(size_t)j * d.n + q] = ((double)j + 0.5 - NH) * d.dy;
+      }
+    if (hipMemcpy(b->P + d.xr, xr.data(), F * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->P + d.yr, yr.data(), F * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(efail(F2D_E_HIP, "coords"));
+  }
+  std::vector<InstD> di(ninst);
+  for (int i = 0; i < ninst; i++) di[i] = b->H[i].d;
+  if (hipMalloc(&b->dI, ninst * sizeof(InstD)) != hipSuccess ||
+      hipMemcpy(b->dI, di.data(), ninst * sizeof(InstD), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(efail(F2D_E_HIP, "descriptors"));
+  if (hipMalloc(&b->dD, ninst * sizeof(DynD)) != hipSuccess ||
+      hipMemset(b->dD, 0, ninst * sizeof(DynD)) != hipSuccess)
+    return bail(efail(F2D_E_HIP, "dyn"));
+  if (hipHostMalloc((void**)&b->hD, ninst * sizeof(DynD), hipHostMallocDefault) != hipSuccess)
+    return bail(efail(F2D_E_NOMEM, "pinned dyn"));
+  memset(b->hD, 0, ninst * sizeof(DynD));
+  size_t maxlds = 0;
+  for (auto& h : b->H) maxlds = std::max(maxlds, h.coarse_lds);
+  if (maxlds > 65536 &&
+      hipFuncSetAttribute((const void*)k_coarse<double>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds) != hipSuccess)
+    return bail(efail(F2D_E_HIP, "coarse LDS %zu bytes", maxlds));
+  int rc = build_programs(*b);
+  if (rc) return bail(rc);
+  *out = b;
+  return F2D_OK;
+}
+
+int f2d_batch_destroy(f2d_batch* b) {
+  batch_free(b);
+  return F2D_OK;
+}
+
+int f2d_batch_info(f2d_batch* b, int inst, int* nlev, int* nfine, long long* launches_per_step) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  *nlev = b->H[inst].d.nlev;
+  *nfine = b->H[inst].d.nfine;
+  *launches_per_step = (long long)b->prog_step.size();
+  return F2D_OK;
+}
+
+int f2d_batch_set_field(f2d_batch* b, int inst, int field, const double* host) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  const InstD& d = b->H[inst].d;
+  if (field < 0 || field >= d.nvar) return efail(F2D_E_ARG, "field out of range");
+  const size_t F = (size_t)d.m * d.n;
+  ECHK(hipMemcpy(b->P + d.state + field * F, host, F * 8, hipMemcpyHostToDevice));
+  return F2D_OK;
+}
+
+int f2d_batch_get_state(f2d_batch* b, int inst, double* host) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  const InstD& d = b->H[inst].d;
+  ECHK(hipStreamSynchronize(b->s));
+  ECHK(hipMemcpy(host, b->P + d.state, (size_t)d.nvar * d.m * d.n * 8, hipMemcpyDeviceToHost));
+  return F2D_OK;
+}
+
+int f2d_batch_state_ptr(f2d_batch* b, int inst, double** dev) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  *dev = b->P + b->H[inst].d.state;
+  return F2D_OK;
+}
+
+static int sync_dyn_to_device(f2d_batch* b) {
+  ECHK(hipMemcpyAsync(b->dD, b->hD, b->ninst * sizeof(DynD), hipMemcpyHostToDevice, b->s));
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+static int sync_dyn_to_host(f2d_batch* b) {
+  ECHK(hipMemcpyAsync(b->hD, b->dD, b->ninst * sizeof(DynD), hipMemcpyDeviceToHost, b->s));
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+
+int f2d_batch_set_dt(f2d_batch* b, int inst, double dt) {
+  ETRY(sync_dyn_to_host(b));
+  if (inst < 0) {
+    for (int i = 0; i < b->ninst; i++) b->hD[i].dt = dt;
+  } else {
+    if (inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+    b->hD[inst].dt = dt;
+  }
+  return sync_dyn_to_device(b);
+}
+
+int f2d_batch_set_adaptive(f2d_batch* b, int on, double cfl, double dtmax) {
+  b->adapt = on;
+  b->cfl = cfl;
+  b->dtmax = dtmax;
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+  }
+  return F2D_OK;
+}
+
+int f2d_batch_use_graph(f2d_batch* b, int on) {
+  b->use_graph = on != 0;
+  return F2D_OK;
+}
+
+int f2d_batch_set_psi_from_vorticity(f2d_batch* b) {
+  ETRY(run_prog(*b, b->prog_init));
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+
+int f2d_batch_step(f2d_batch* b, int nsteps) {
+  if (b->use_graph) {
+    if (!b->gexec) {
+      ECHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+      int rc = run_prog(*b, b->prog_step);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->s, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return efail(F2D_E_HIP, "capture: %s", hipGetErrorString(e));
+      b->graph = g;
+      ECHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    }
+    for (int k = 0; k < nsteps; k++) ECHK(hipGraphLaunch(b->gexec, b->s));
+  } else {
+    for (int k = 0; k < nsteps; k++) ETRY(run_prog(*b, b->prog_step));
+  }
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+
+int f2d_batch_step_async(f2d_batch* b, int nsteps) {
+  if (!b->use_graph || !b->gexec) return f2d_batch_step(b, nsteps);
+  for (int k = 0; k < nsteps; k++) ECHK(hipGraphLaunch(b->gexec, b->s));
+  return F2D_OK;
+}
+
+int f2d_batch_synchronize(f2d_batch* b) {
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+
+int f2d_batch_diagnostics(f2d_batch* b, int inst, double* out8) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  ETRY(sync_dyn_to_host(b));
+  for (int k = 0; k < NDIAG; k++) out8[k] = b->hD[inst].diag[k];
+  return F2D_OK;
+}
+
+int f2d_batch_solve_info(f2d_batch* b, int inst, int* nite, double* res, int* diverged) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  ETRY(sync_dyn_to_host(b));
+  *nite = b->hD[inst].nite;
+  *res = b->hD[inst].res;
+  *diverged = b->hD[inst].diverged;
+  return F2D_OK;
+}
+
+}  // extern "C"

@@ -889,3 +889,24 @@ int f2d_model_last_solve(f2d_model* M, int* nite, double* res) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// multigrid set-up shared with the batched engine (f2d_engine.hip): builds a
+// hierarchy on the device and hands out its level masks and operators.
+// ---------------------------------------------------------------------------
+int f2d_gmg_build_for_engine(int nx, int ny, double dx, double dy, double hydroepsilon,
+                             const int8_t* mskp_host, void** gmg_out) {
+  f2d_gmg* g = nullptr;
+  int rc = gmg_build(nx, ny, dx, dy, hydroepsilon, 8. / 9., mskp_host, &g);
+  *gmg_out = g;
+  return rc;
+}
+int f2d_gmg_level_raw(void* gv, int lev, int8_t** msk, double** A, int* m, int* n) {
+  f2d_gmg* g = (f2d_gmg*)gv;
+  *msk = g->L[lev].msk;
+  *A = g->L[lev].A;
+  *m = g->L[lev].m;
+  *n = g->L[lev].n;
+  return F2D_OK;
+}
+void f2d_gmg_free_raw(void* g) { gmg_free((f2d_gmg*)g); }

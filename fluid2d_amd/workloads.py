@@ -98,5 +98,14 @@ def initial_fields(name, grid, seed=None):
             "buoyancy": rayleigh_taylor(grid, 3 if seed is None else seed)}
 
 
+def time_step(name, grid, maxspeed0):
+    """Fixed dt at CFL 0.8: of the initial maxspeed, or for the
+    Rayleigh-Taylor start (at rest) of the buoyancy velocity scale
+    sqrt(g * db * Ly) = 1."""
+    kw, _, _ = CONFIGS[name]
+    u = maxspeed0 if kw["modelname"] == "euler" else 1.0
+    return 0.8 * min(grid.dx, grid.dy) / u
+
+
 def build_grid(name, **over):
     return Grid(make_param(name, **over))

@@ -205,6 +205,42 @@ int f2d_model_run(f2d_model *M, int nsteps, double dt, int *nite_log);
 /* F-cycles and final relative residual of the last full solve */
 int f2d_model_last_solve(f2d_model *M, int *nite, double *res);
 
+/* ---- batched engine ------------------------------------------------------
+ * Any number of independent Euler/Boussinesq instances (different sizes,
+ * masks and physics; one time scheme per batch) advanced together.  Every
+ * kernel launch covers all instances taking part in a phase; one step is a
+ * fixed launch program without host round trips (the solve stopping test of
+ * core/gmg/hierarchy.py:169 and dt live on the device), captured once into
+ * a hipGraph and replayed.  Each instance performs exactly the reference's
+ * operations, so its FP64 result is independent of the batch it runs in.
+ */
+typedef struct f2d_batch f2d_batch;
+const char *f2d_batch_last_error(void);
+/* coarse_lds_budget: bytes of LDS for the single-workgroup coarse-level
+ * kernel (0 = 64 KiB); levels are handed to it from the coarsest up while
+ * they fit.  msks[i]: host cell mask of instance i. */
+int f2d_batch_create(int ninst, const f2d_model_params *params, const int8_t *const *msks,
+                     int coarse_lds_budget, f2d_batch **out);
+int f2d_batch_destroy(f2d_batch *b);
+int f2d_batch_info(f2d_batch *b, int inst, int *nlev, int *nfine, long long *launches_per_step);
+int f2d_batch_set_field(f2d_batch *b, int inst, int field, const double *host);
+int f2d_batch_get_state(f2d_batch *b, int inst, double *host);
+int f2d_batch_state_ptr(f2d_batch *b, int inst, double **dev);
+/* fixed dt of instance inst (inst < 0: all) */
+int f2d_batch_set_dt(f2d_batch *b, int inst, double dt);
+/* adaptive dt = cfl*min(dx,dy)/maxspeed capped at dtmax (core/fluid2d.py:376-397) */
+int f2d_batch_set_adaptive(f2d_batch *b, int on, double cfl, double dtmax);
+int f2d_batch_use_graph(f2d_batch *b, int on);
+/* set_psi_from_vorticity of every instance, then diagnostics (maxspeed) */
+int f2d_batch_set_psi_from_vorticity(f2d_batch *b);
+/* nsteps x (RK step + diagnostics), synchronous / enqueue only */
+int f2d_batch_step(f2d_batch *b, int nsteps);
+int f2d_batch_step_async(f2d_batch *b, int nsteps);
+int f2d_batch_synchronize(f2d_batch *b);
+/* out8 as f2d_model_diagnostics (raw sums; maxspeed and ke scaled) */
+int f2d_batch_diagnostics(f2d_batch *b, int inst, double *out8);
+int f2d_batch_solve_info(f2d_batch *b, int inst, int *nite, double *res, int *diverged);
+
 #ifdef __cplusplus
 }
 #endif
