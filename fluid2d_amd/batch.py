@@ -7,7 +7,9 @@ from a hipGraph with the multigrid stopping test and dt on the device.  An
 instance's FP64 result is bit-identical whether it runs alone or in a mix.
 
 A spec is a dict of :class:`fluid2d_amd.Param` keywords, or a ``(Param,
-Grid)`` pair.
+Grid)`` pair.  ``precision="mixed"`` keeps the state, the RK combination,
+the full-solve residual and correction and the coarse-level solve in FP64
+and runs advection, tendencies and the fine-level V-cycles in FP32.
 """
 import ctypes as C
 
@@ -24,7 +26,7 @@ _d = C.c_double
 
 _SIG = {
     "f2d_batch_last_error": ([], C.c_char_p),
-    "f2d_batch_create": ([_i, C.POINTER(_lib.f2d_model_params), C.POINTER(C.c_void_p), _i,
+    "f2d_batch_create": ([_i, C.POINTER(_lib.f2d_model_params), C.POINTER(C.c_void_p), _i, _i,
                           C.POINTER(C.c_void_p)], _i),
     "f2d_batch_destroy": ([_p], _i),
     "f2d_batch_info": ([_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_longlong)], _i),
@@ -86,7 +88,7 @@ def params_struct(param, grid):
 
 
 class Batch:
-    def __init__(self, specs, coarse_lds_budget=0, use_graph=True):
+    def __init__(self, specs, precision="fp64", coarse_lds_budget=0, use_graph=True):
         self.params, self.grids = [], []
         for s in specs:
             if isinstance(s, tuple):
@@ -102,7 +104,9 @@ class Batch:
         self._msks = [np.ascontiguousarray(g.msk, dtype=np.int8) for g in self.grids]
         mp = (C.c_void_p * n)(*[m.ctypes.data for m in self._msks])
         h = C.c_void_p()
-        _call("f2d_batch_create", n, arr, mp, int(coarse_lds_budget), C.byref(h))
+        self.precision = precision
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        _call("f2d_batch_create", n, arr, mp, prec, int(coarse_lds_budget), C.byref(h))
         self._h = h.value
         self.n = n
         _call("f2d_batch_use_graph", self._h, 1 if use_graph else 0)

@@ -90,52 +90,80 @@ __device__ __forceinline__ T ce_edge_flux_q(const int8_t* mk, ptrdiff_t s, T vel
   return vel * qp;
 }
 
-// Multigrid operator: A is 5 planes of stride P; A_k(j,i) couples (j,i) to
-// its lower neighbour k (SW, S, SE, W, centre); upper couplings by symmetry
-// (core/gmg/fortran_multigrid.f90:68-82, 342-350).  X(d) returns x at flat
-// offset d from the cell.
-template <typename T, typename X>
-__device__ __forceinline__ T offdiag_x(const T* A, size_t P, size_t c, int n, X x) {
-  const T* A1 = A;
-  const T* A2 = A + P;
-  const T* A3 = A + 2 * P;
-  const T* A4 = A + 3 * P;
-  T s = A1[c] * x(-n - 1);
-  s = s + A2[c] * x(-n);
-  s = s + A3[c] * x(-n + 1);
-  s = s + A4[c] * x(-1);
-  s = s + A4[c + 1] * x(1);
-  s = s + A3[c + n - 1] * x(n - 1);
-  s = s + A2[c + n] * x(n);
-  s = s + A1[c + n + 1] * x(n + 1);
+// Multigrid operator.  AF(k, d) returns coefficient plane k (A_1..A_5 of
+// the reference) at flat offset d from the cell: A_k couples the cell to
+// its lower neighbour k (SW, S, SE, W, centre); the upper couplings are
+// read from the neighbours by symmetry (core/gmg/fortran_multigrid.f90:
+// 68-82, 342-350).  X(d) returns x at flat offset d.
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T offdiag_f(int n, AF A, X x) {
+  T s = A(0, 0) * x(-n - 1);
+  s = s + A(1, 0) * x(-n);
+  s = s + A(2, 0) * x(-n + 1);
+  s = s + A(3, 0) * x(-1);
+  s = s + A(3, 1) * x(1);
+  s = s + A(2, n - 1) * x(n - 1);
+  s = s + A(1, n) * x(n);
+  s = s + A(0, n + 1) * x(n + 1);
   return s;
 }
 
 // one damped-Jacobi update (core/gmg/fortran_multigrid.f90:66-85)
-template <typename T, typename X>
-__device__ __forceinline__ T jacobi_x(int8_t mk, const T* A, size_t P, size_t c, int n, T bc, T c1,
-                                      T c2, X x) {
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T jacobi_f(int8_t mk, int n, T bc, T c1, T c2, AF A, X x) {
   if (mk == 0) return T(0);
-  const T c3 = c1 / fabs(A[4 * P + c]);
-  return x(0) * c2 + c3 * (offdiag_x(A, P, c, n, x) - bc);
+  const T c3 = c1 / fabs(A(4, 0));
+  return x(0) * c2 + c3 * (offdiag_f<T>(n, A, x) - bc);
 }
 
 // r = b - A x (core/gmg/fortran_multigrid.f90:340-357)
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T residual_f(int8_t mk, int n, T bc, AF A, X x) {
+  if (mk == 0) return T(0);
+  T s = bc - A(0, 0) * x(-n - 1);
+  s = s - A(1, 0) * x(-n);
+  s = s - A(2, 0) * x(-n + 1);
+  s = s - A(3, 0) * x(-1);
+  s = s - A(4, 0) * x(0);
+  s = s - A(3, 1) * x(1);
+  s = s - A(2, n - 1) * x(n - 1);
+  s = s - A(1, n) * x(n);
+  s = s - A(0, n + 1) * x(n + 1);
+  return s;
+}
+
+// explicit planes of stride P at cell c
+template <typename T>
+struct APlanes {
+  const T* A;
+  size_t P, c;
+  __device__ __forceinline__ T operator()(int k, ptrdiff_t d) const { return A[k * P + c + d]; }
+};
+// constant weights w_k where bit k of the cell's stencil byte is set (the
+// finest-level operator: stencil * coef * msk_nb * msk, core/gmg/level.py:292)
+template <typename T>
+struct ABits {
+  const uint8_t* sb;
+  const T* w;
+  size_t c;
+  __device__ __forceinline__ T operator()(int k, ptrdiff_t d) const {
+    return ((sb[c + d] >> k) & 1) ? w[k] : T(0);
+  }
+};
+
+template <typename T, typename X>
+__device__ __forceinline__ T offdiag_x(const T* A, size_t P, size_t c, int n, X x) {
+  return offdiag_f<T>(n, APlanes<T>{A, P, c}, x);
+}
+template <typename T, typename X>
+__device__ __forceinline__ T jacobi_x(int8_t mk, const T* A, size_t P, size_t c, int n, T bc, T c1,
+                                      T c2, X x) {
+  return jacobi_f<T>(mk, n, bc, c1, c2, APlanes<T>{A, P, c}, x);
+}
 template <typename T, typename X>
 __device__ __forceinline__ T residual_x(int8_t mk, const T* A, size_t P, size_t c, int n, T bc,
                                         X x) {
-  if (mk == 0) return T(0);
-  const T *A1 = A, *A2 = A + P, *A3 = A + 2 * P, *A4 = A + 3 * P, *A5 = A + 4 * P;
-  T s = bc - A1[c] * x(-n - 1);
-  s = s - A2[c] * x(-n);
-  s = s - A3[c] * x(-n + 1);
-  s = s - A4[c] * x(-1);
-  s = s - A5[c] * x(0);
-  s = s - A4[c + 1] * x(1);
-  s = s - A3[c + n - 1] * x(n - 1);
-  s = s - A2[c + n] * x(n);
-  s = s - A1[c + n + 1] * x(n + 1);
-  return s;
+  return residual_f<T>(mk, n, bc, APlanes<T>{A, P, c}, x);
 }
 
 // full weighting around a fine point (core/gmg/fortran_multigrid.f90:532-534);

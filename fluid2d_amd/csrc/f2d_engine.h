@@ -34,9 +34,18 @@ struct InstD {
   int m, n;    // finest padded dims
   int order, method, perio, diffusion, tscheme, nvar;
   double dx, dy, gravity, kd0, kd1, area, aparab, omega;
-  long long state, k[3], work, x0[2];  // real pool offsets
-  long long xr, yr;                    // real pool offsets (diagnostics)
+  // FP64 pool: state, work (full-solve rhs), xr, yr, A0d (FP64 level-0
+  // operator of the outer residual); T pool: tendencies k, level arrays,
+  // the level-0 correction pair x0 (T is double or float)
+  long long state, work, xr, yr, A0d;
+  long long k[3], x0[2];
   long long msk, mskp;                 // byte pool offsets
+  // finest-level operator from stencil bits: A_k(cell) = w0[k] where bit k
+  // of sb0[cell] is set, else 0 (verified against the explicit operator at
+  // set-up; a0fly = 0 keeps the explicit planes)
+  long long sb0;
+  int a0fly, pad0;
+  double w0[5];
   LevD L[MAXLEV];
 };
 

@@ -46,29 +46,41 @@ __device__ __forceinline__ bool skip(const DynD* dyn, int inst, int pred) {
 // stage state (core/timescheme.py:167-180, :152-164), NumPy order
 // stage 0: x; RK3_SSP 1: x + dt*k0; 2: x + (0.25 dt)(k0 + k1);
 // RK3 1: x + (dt/3) k0; 2: x + (0.5 dt) k1
+// The state is FP64; tendencies are T (double, or float in mixed mode) and
+// are widened, so in FP64 mode this is exactly the reference's expression.
 template <typename T>
 struct Stage {
-  const T *x, *k0, *k1;
-  T c;
+  const double* x;
+  const T *k0, *k1;
+  double c;
   int kind;  // 0 x, 1 x + c*k0, 2 x + c*(k0+k1), 3 x + c*k1
-  __device__ __forceinline__ T operator()(size_t q) const {
+  __device__ __forceinline__ double operator()(size_t q) const {
     switch (kind) {
       case 0: return x[q];
-      case 1: return x[q] + c * k0[q];
-      case 2: return x[q] + c * (k0[q] + k1[q]);
-      default: return x[q] + c * k1[q];
+      case 1: return x[q] + c * (double)k0[q];
+      case 2: return x[q] + c * ((double)k0[q] + (double)k1[q]);
+      default: return x[q] + c * (double)k1[q];
     }
   }
 };
 
 // ---------------------------------------------------------------------------
-// k_rhs: tendencies of the tracers for one RK stage
+// k_rhs: tendencies of the tracers for one RK stage.  The stage state of the
+// tracer(s) and of (u, v) is formed once per cell into LDS tiles (3-cell
+// halo for the tracer, 1 for the velocities); fluxes, torque and diffusion
+// then read the tiles.
 // ---------------------------------------------------------------------------
+constexpr int QW = TX + 6, QH = TY + 6;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
-                                             const InstD* __restrict__ I, T* P,
+                                             const InstD* __restrict__ I, double* P, T* Q,
                                              const int8_t* __restrict__ B,
                                              const DynD* __restrict__ dyn, int stage, int out_k) {
+  __shared__ T qs[2][QH * QW];
+  __shared__ T us[TY * (TX + 1)];
+  __shared__ T vs[(TY + 1) * TX];
+  __shared__ int8_t ms[QH * QW];
   const Task tk = tasks[blockIdx.x];
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
@@ -77,20 +89,42 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   const double dt = d.dt;
   Stage<T> S;
   S.x = P + in.state;
-  S.k0 = P + in.k[0];
-  S.k1 = P + in.k[1];
+  S.k0 = Q + in.k[0];
+  S.k1 = Q + in.k[1];
   if (stage == 0) {
     S.kind = 0;
-    S.c = T(0);
+    S.c = 0.;
   } else if (in.tscheme == 0) {
     S.kind = stage == 1 ? 1 : 2;
-    S.c = stage == 1 ? T(dt) : T(0.25 * dt);
+    S.c = stage == 1 ? dt : 0.25 * dt;
   } else {
     S.kind = stage == 1 ? 1 : 3;
-    S.c = stage == 1 ? T(dt / 3.) : T(0.5 * dt);
+    S.c = stage == 1 ? dt / 3. : 0.5 * dt;
   }
   const int8_t* msk = B + in.msk;
-  T* out = P + in.k[out_k];
+  const int ntr = in.model == 0 ? 1 : 2;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+  const int y0 = tk.y0, x0 = tk.x0;
+  // tiles: tracer rows [y0-3, y0+TY+2] x cols [x0-3, x0+TX+2]
+  for (int q = tid; q < QH * QW; q += nt) {
+    const int r = q / QW, c = q % QW;
+    const int j = y0 - 3 + r, i = x0 - 3 + c;
+    const bool ok = j < m && i < n;
+    const size_t e = ok ? at(j, i, n) : 0;
+    ms[q] = ok ? msk[e] : 0;
+    for (int t = 0; t < ntr; t++) qs[t][q] = ok ? T(S((t == 0 ? 0 : 4 * F) + e)) : T(0);
+  }
+  for (int q = tid; q < TY * (TX + 1); q += nt) {  // u on rows y0.., cols x0-1..
+    const int r = q / (TX + 1), c = q % (TX + 1);
+    const int j = y0 + r, i = x0 - 1 + c;
+    us[q] = (j < m && i < n) ? T(S(2 * F + at(j, i, n))) : T(0);
+  }
+  for (int q = tid; q < (TY + 1) * TX; q += nt) {  // v on rows y0-1.., cols x0..
+    const int r = q / TX, c = q % TX;
+    const int j = y0 - 1 + r, i = x0 + c;
+    vs[q] = (j < m && i < n) ? T(S(3 * F + at(j, i, n))) : T(0);
+  }
+  __syncthreads();
   // advection parameters (core/fortran_advection.f90:46-60), cst[3] = umax
   UpParams<T> p;
   const double umax = d.umax, logcosh = 0.05;
@@ -109,52 +143,51 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   p.order = in.order;
   const bool upwind = in.order % 2 == 1;
   const bool kforce = stage == (in.tscheme == 0 ? 2 : 0);
-  const int ntr = in.model == 0 ? 1 : 2;
-  const size_t fu = 2 * F, fv = 3 * F;
+  T* out = Q + in.k[out_k];
   for (int r = threadIdx.y; r < TY; r += blockDim.y) {
-    const int J = tk.y0 + r, Ic = tk.x0 + threadIdx.x;
+    const int J = y0 + r, Ic = x0 + threadIdx.x;
     if (J >= m - NH || Ic >= n - NH) continue;
-    const size_t c = at(J, Ic, n);
+    const int lq = (r + 3) * QW + threadIdx.x + 3;  // cell in the tracer/mask tile
+    const T ue = us[r * (TX + 1) + threadIdx.x + 1], uw = us[r * (TX + 1) + threadIdx.x];
+    const T vn = vs[(r + 1) * TX + threadIdx.x], vsn = vs[r * TX + threadIdx.x];
     for (int t = 0; t < ntr; t++) {
-      const size_t ft = t == 0 ? 0 : 4 * F;  // vorticity, buoyancy
-      auto qx = [&](size_t base) { return [&, base](int k) { return S(ft + base + k); }; };
-      auto qy = [&](size_t base) {
-        return [&, base](int k) { return S(ft + base + (ptrdiff_t)k * n); };
-      };
+      const T* qt = qs[t];
+      auto qx = [&](int base) { return [qt, base](int k) { return qt[base + k]; }; };
+      auto qy = [&](int base) { return [qt, base](int k) { return qt[base + k * QW]; }; };
       T fe, fw, fn, fs;
       if (upwind) {
-        fe = up_edge_flux_q(msk + c, 1, S(fu + c), p, qx(c));
-        fw = up_edge_flux_q(msk + c - 1, 1, S(fu + c - 1), p, qx(c - 1));
-        fn = up_edge_flux_q(msk + c, (ptrdiff_t)n, S(fv + c), p, qy(c));
-        fs = up_edge_flux_q(msk + c - n, (ptrdiff_t)n, S(fv + c - n), p, qy(c - n));
+        fe = up_edge_flux_q(ms + lq, 1, ue, p, qx(lq));
+        fw = up_edge_flux_q(ms + lq - 1, 1, uw, p, qx(lq - 1));
+        fn = up_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, p, qy(lq));
+        fs = up_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, p, qy(lq - QW));
       } else {
-        fe = ce_edge_flux_q(msk + c, 1, S(fu + c), in.order, true, qx(c));
-        fw = ce_edge_flux_q(msk + c - 1, 1, S(fu + c - 1), in.order, true, qx(c - 1));
-        fn = ce_edge_flux_q(msk + c, (ptrdiff_t)n, S(fv + c), in.order, true, qy(c));
-        fs = ce_edge_flux_q(msk + c - n, (ptrdiff_t)n, S(fv + c - n), in.order, true, qy(c - n));
+        fe = ce_edge_flux_q(ms + lq, 1, ue, in.order, true, qx(lq));
+        fw = ce_edge_flux_q(ms + lq - 1, 1, uw, in.order, true, qx(lq - 1));
+        fn = ce_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, in.order, true, qy(lq));
+        fs = ce_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, in.order, true, qy(lq - QW));
       }
       T val = -p.zdx * (fe - fw) - p.zdy * (fn - fs);
+      const int8_t mc = ms[lq];
       if (in.model == 1 && t == 0) {  // y *= msk; add_torque
-        const int8_t mc = msk[c];
         val = val * T(mc);
-        int ml = mc + msk[c - 1], mr = msk[c + 1] + mc;
+        int ml = mc + ms[lq - 1], mr = ms[lq + 1] + mc;
         ml = ml < 1 ? 1 : ml;
         mr = mr < 1 ? 1 : mr;
         if (ml + mr == 4)
-          val = val + (S(4 * F + c + 1) - S(4 * F + c - 1)) * T(0.5 * in.gravity / in.dx) * T(mc);
+          val = val + (qs[1][lq + 1] - qs[1][lq - 1]) * T(0.5 * in.gravity / in.dx) * T(mc);
       }
-      if (in.diffusion && kforce && msk[c] == 1) {  // add_diffusion
+      if (in.diffusion && kforce && mc == 1) {  // add_diffusion
         const double coefd =
             (in.model == 0 ? 1. : (in.tscheme == 0 ? 1.5 : 1.)) * (t == 0 ? in.kd0 : in.kd1);
         const T coef = T(coefd / (in.dx * in.dx));
-        const T tc = S(ft + c);
-        T s = T(msk[c - 1]) * (S(ft + c - 1) - tc);
-        s = s + T(msk[c + 1]) * (S(ft + c + 1) - tc);
-        s = s + T(msk[c - n]) * (S(ft + c - n) - tc);
-        s = s + T(msk[c + n]) * (S(ft + c + n) - tc);
-        val = val + coef * s;
+        const T tc = qt[lq];
+        T sm = T(ms[lq - 1]) * (qt[lq - 1] - tc);
+        sm = sm + T(ms[lq + 1]) * (qt[lq + 1] - tc);
+        sm = sm + T(ms[lq - QW]) * (qt[lq - QW] - tc);
+        sm = sm + T(ms[lq + QW]) * (qt[lq + QW] - tc);
+        val = val + coef * sm;
       }
-      store_img_mask<T>(out + ft, nullptr, J, Ic, val, m, n);
+      store_img_mask<T>(out + (t == 0 ? 0 : 4 * F), nullptr, J, Ic, val, m, n);
     }
   }
 }
@@ -164,14 +197,14 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
-                                             const InstD* __restrict__ I, T* P,
+                                             const InstD* __restrict__ I, const T* SP, T* DP,
                                              const DynD* __restrict__ dyn, int pred) {
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
-  const T* src = P + tk.xin;
-  T* dst = P + tk.xout;
+  const T* src = SP + tk.xin;
+  T* dst = DP + tk.xout;
   for (int r = threadIdx.y; r < TY; r += blockDim.y) {
     const int j = tk.y0 + r, i = tk.x0 + threadIdx.x;
     if (j >= m - 1 || i >= n - 1) continue;
@@ -187,7 +220,7 @@ __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
-                                                const InstD* __restrict__ I, T* P,
+                                                const InstD* __restrict__ I, T* Q,
                                                 const int8_t* __restrict__ B,
                                                 const DynD* __restrict__ dyn, int pred) {
   __shared__ T xs[TY + 4][TX + 4];
@@ -201,9 +234,9 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   const int m = L.m, n = L.n;
   const size_t PL = (size_t)m * n;
   const int8_t* msk = B + L.msk;
-  const T* A = P + L.A;
-  const T* b = P + L.b;
-  const T* xin = P + tk.xin;
+  const T* A = Q + L.A;
+  const T* b = Q + L.b;
+  const T* xin = Q + tk.xin;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
   const int y0 = tk.y0, x0 = tk.x0;
   // x on rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1], clipped to [1, m-2]
@@ -215,7 +248,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       if (MODE == 0 || MODE == 2) v = xin[at(j, i, n)];
       if (MODE == 2 || MODE == 3) {
         const LevD& C = in.L[tk.lev + 1];
-        const T r = interp_at<T>(msk, n, B + C.msk, P + tk.xc, C.n, j, i);
+        const T r = interp_at<T>(msk, n, B + C.msk, Q + tk.xc, C.n, j, i);
         v = MODE == 2 ? v + r : r;
       }
     }
@@ -223,37 +256,45 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   }
   __syncthreads();
   const T c1 = T(in.omega), c2 = T(1. - in.omega);
-  // first sweep on rows [y0-1, y0+TY] x cols [x0-1, x0+TX], clipped to [.., m-NH]
-  for (int q = tid; q < (TY + 2) * (TX + 2); q += nt) {
-    const int rr = q / (TX + 2), cc = q % (TX + 2);
-    const int j = y0 - 1 + rr, i = x0 - 1 + cc;
-    T v = T(0);
-    if (j <= m - NH && i <= n - NH) {
-      const size_t c = at(j, i, n);
-      v = jacobi_x<T>(msk[c], A, PL, c, n, b[c], c1, c2,
-                      [&](int dd) {
-                        const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-                        const int di = dd - dj * n;
-                        return xs[rr + 1 + dj][cc + 1 + di];
-                      });
-    }
-    j1[rr][cc] = v;
-  }
-  __syncthreads();
-  T* xout = P + tk.xout;
+  T* xout = Q + tk.xout;
   const int8_t* mp = final_mask ? B + in.mskp : nullptr;
-  for (int rr = threadIdx.y; rr < TY; rr += blockDim.y) {
-    const int J = y0 + rr, Ic = x0 + threadIdx.x;
-    if (J >= m - NH || Ic >= n - NH) continue;
-    const int cc = threadIdx.x;
-    const size_t c = at(J, Ic, n);
-    const T v = jacobi_x<T>(msk[c], A, PL, c, n, b[c], c1, c2, [&](int dd) {
-      const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-      const int di = dd - dj * n;
-      return j1[rr + 1 + dj][cc + 1 + di];
-    });
-    store_img_mask<T>(xout, mp, J, Ic, v, m, n);
-  }
+  T w[5];
+  for (int k = 0; k < 5; k++) w[k] = T(in.w0[k]);
+  const uint8_t* sb = (const uint8_t*)(B + in.sb0);
+  auto sweeps = [&](auto Aat) {
+    // first sweep on rows [y0-1, y0+TY] x cols [x0-1, x0+TX], clipped to [.., m-NH]
+    for (int q = tid; q < (TY + 2) * (TX + 2); q += nt) {
+      const int rr = q / (TX + 2), cc = q % (TX + 2);
+      const int j = y0 - 1 + rr, i = x0 - 1 + cc;
+      T v = T(0);
+      if (j <= m - NH && i <= n - NH) {
+        const size_t c = at(j, i, n);
+        v = jacobi_f<T>(msk[c], n, b[c], c1, c2, Aat(c), [&](int dd) {
+          const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+          const int di = dd - dj * n;
+          return xs[rr + 1 + dj][cc + 1 + di];
+        });
+      }
+      j1[rr][cc] = v;
+    }
+    __syncthreads();
+    for (int rr = threadIdx.y; rr < TY; rr += blockDim.y) {
+      const int J = y0 + rr, Ic = x0 + threadIdx.x;
+      if (J >= m - NH || Ic >= n - NH) continue;
+      const int cc = threadIdx.x;
+      const size_t c = at(J, Ic, n);
+      const T v = jacobi_f<T>(msk[c], n, b[c], c1, c2, Aat(c), [&](int dd) {
+        const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+        const int di = dd - dj * n;
+        return j1[rr + 1 + dj][cc + 1 + di];
+      });
+      store_img_mask<T>(xout, mp, J, Ic, v, m, n);
+    }
+  };
+  if (tk.lev == 0 && in.a0fly)
+    sweeps([&](size_t c) { return ABits<T>{sb, w, c}; });
+  else
+    sweeps([&](size_t c) { return APlanes<T>{A, PL, c}; });
 }
 
 // ---------------------------------------------------------------------------
@@ -264,14 +305,15 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
 //        2 restrict b itself instead of the residual (F-cycle descent),
 //        4 norm partial of r, 8 norm partial of b (owned interior, msk != 0)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__ tasks,
-                                                        const InstD* __restrict__ I, T* P,
+                                                        const InstD* __restrict__ I,
+                                                        const R* RP, T* Q,
                                                         const int8_t* __restrict__ B,
                                                         const DynD* __restrict__ dyn, int pred,
                                                         int flags, double* partial) {
   constexpr int RY = 2 * TY2 + 1, RX = 2 * TX2 + 1;
-  __shared__ T rs[RY][RX];
+  __shared__ R rs[RY][RX];
   __shared__ double red[2][8];
   const Task tk = tasks[blockIdx.x];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
@@ -289,11 +331,15 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   const size_t PL = (size_t)m1 * n1;
   const int8_t* msk1 = B + Lf.msk;
   const int8_t* msk2 = B + Lc.msk;
-  const T* A = P + Lf.A;
-  const T* x = P + tk.xin;
-  // rhs: level b, or an explicit rhs buffer (tk.xout >= 0 with flag 16)
-  const T* b = P + ((flags & 16) ? tk.xout : Lf.b);
+  // operands in RP: the level's A, x and b, or (flag 32) the FP64 level-0
+  // operator with an explicit rhs buffer (flag 16) -- the outer residual
+  const R* A = RP + ((flags & 32) ? in.A0d : Lf.A);
+  const R* x = RP + tk.xin;
+  const R* b = RP + ((flags & 16) ? tk.xout : Lf.b);
   const int fy0 = 2 * tk.y0 - 3, fx0 = 2 * tk.x0 - 3;
+  R w[5];
+  for (int k = 0; k < 5; k++) w[k] = R(in.w0[k]);
+  const uint8_t* sb = (const uint8_t*)(B + in.sb0);
   double nr = 0., nb = 0.;
   for (int q = tid; q < RY * RX; q += nt) {
     const int rr = q / RX, cc = q % RX;
@@ -301,16 +347,18 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     if (jf > m1 - NH || if_ > n1 - NH) continue;  // beyond the last needed point
     const int j = img(jf, m1), i = img(if_, n1);
     const size_t c = at(j, i, n1);
-    T v;
+    R v;
     if (flags & 2)
       v = b[c];
+    else if (tk.lev == 0 && in.a0fly)
+      v = residual_f<R>(msk1[c], n1, b[c], ABits<R>{sb, w, c}, [&](int dd) { return x[c + dd]; });
     else
-      v = residual_x<T>(msk1[c], A, PL, c, n1, b[c], [&](int dd) { return x[c + dd]; });
+      v = residual_x<R>(msk1[c], A, PL, c, n1, b[c], [&](int dd) { return x[c + dd]; });
     rs[rr][cc] = v;
     // owned fine cells: local rows/cols [0, 2*T-1] that are interior
     if (rr < 2 * TY2 && cc < 2 * TX2 && jf == j && if_ == i && jf >= NH && if_ >= NH &&
         jf < m1 - NH && if_ < n1 - NH && (tk.y0 + rr / 2) < m2 - NH && (tk.x0 + cc / 2) < n2 - NH) {
-      if (flags & 1) store_img_mask<T>(P + tk.xc, nullptr, jf, if_, v, m1, n1);
+      if (flags & 1) store_img_mask<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
       if (msk1[c] != 0) {
         nr += (double)v * (double)v;
         const double bb = (double)b[c];
@@ -338,17 +386,17 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     partial[2 * blockIdx.x] = a;
     partial[2 * blockIdx.x + 1] = c;
   }
-  T* b2 = P + Lc.b;
+  T* b2 = Q + Lc.b;
   for (int q = tid; q < TY2 * TX2; q += nt) {
     const int ry = q / TX2, rx = q % TX2;
     const int J2 = tk.y0 + ry, I2 = tk.x0 + rx;
     if (J2 >= m2 - NH || I2 >= n2 - NH) continue;
-    T v = T(0);
+    R v = R(0);
     if (msk2[at(J2, I2, n2)] != 0) {
       const int cy = 2 * ry + 1, cx = 2 * rx + 1;
-      v = restrict_r<T>([&](int dj, int di) { return rs[cy + dj][cx + di]; });
+      v = restrict_r<R>([&](int dj, int di) { return rs[cy + dj][cx + di]; });
     }
-    store_img_mask<T>(b2, nullptr, J2, I2, v, m2, n2);
+    store_img_mask<T>(b2, nullptr, J2, I2, T(v), m2, n2);
   }
 }
 
@@ -362,159 +410,204 @@ template <typename T>
 struct CL {
   int m, n;
   int8_t* msk;
-  T *A, *x, *b, *t, *r;
+  T *A, *x, *b, *t;  // the residual reuses t (never live at the same time)
+};
+
+// Phases over a level: levels of at most SMALL cells run on wavefront 0
+// alone and end with a wavefront-level barrier (LDS operations of one
+// wavefront execute in order); larger levels use the whole workgroup and a
+// workgroup barrier.  `prev_small` (uniform) inserts the workgroup barrier
+// needed when a large phase follows small ones.
+constexpr int SMALL = 256;
+
+struct Sync {
+  int tid, nt;
+  bool prev_small;
+  __device__ __forceinline__ bool begin(int cells) {
+    const bool small = cells <= SMALL;
+    if (!small && prev_small) __syncthreads();
+    prev_small = small;
+    return small ? tid < 64 : true;
+  }
+  __device__ __forceinline__ int stride(int cells) const { return cells <= SMALL ? 64 : nt; }
+  __device__ __forceinline__ void end(int cells) {
+    if (cells <= SMALL) {
+      if (tid < 64) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    } else {
+      __syncthreads();
+    }
+  }
 };
 
 template <typename T>
-__device__ void c_smooth(const CL<T>& L, T c1, T c2, int tid, int nt) {
-  const int m = L.m, n = L.n;
+__device__ void c_smooth(const CL<T>& L, T c1, T c2, Sync& sy) {
+  const int m = L.m, n = L.n, cells = m * n;
   const size_t P = (size_t)m * n;
   const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2;
-  for (int q = tid; q < wj * wi; q += nt) {
-    const int j = NH - 1 + q / wi, i = NH - 1 + q % wi;
-    const size_t c = at(j, i, n);
-    L.t[c] = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
-                         [&](int dd) { return L.x[c + dd]; });
+  if (sy.begin(cells)) {
+    const int st = sy.stride(cells);
+    for (int q = sy.tid; q < wj * wi; q += st) {
+      const int j = NH - 1 + q / wi, i = NH - 1 + q % wi;
+      const size_t c = at(j, i, n);
+      L.t[c] = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
+                           [&](int dd) { return L.x[c + dd]; });
+    }
   }
-  __syncthreads();
+  sy.end(cells);
   const int N = n - 2 * NH, M = m - 2 * NH;
-  for (int q = tid; q < M * N; q += nt) {
-    const int J = NH + q / N, I2 = NH + q % N;
-    const size_t c = at(J, I2, n);
-    const T v = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
-                            [&](int dd) { return L.t[c + dd]; });
-    store_img_mask<T>(L.x, nullptr, J, I2, v, m, n);
+  if (sy.begin(cells)) {
+    const int st = sy.stride(cells);
+    for (int q = sy.tid; q < M * N; q += st) {
+      const int J = NH + q / N, I2 = NH + q % N;
+      const size_t c = at(J, I2, n);
+      const T v = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
+                              [&](int dd) { return L.t[c + dd]; });
+      store_img_mask<T>(L.x, nullptr, J, I2, v, m, n);
+    }
   }
-  __syncthreads();
+  sy.end(cells);
 }
 template <typename T>
-__device__ void c_zero(T* a, int cnt, int tid, int nt) {
-  for (int q = tid; q < cnt; q += nt) a[q] = T(0);
-  __syncthreads();
+__device__ void c_zero(const CL<T>& L, Sync& sy) {
+  const int cnt = L.m * L.n;
+  if (sy.begin(cnt))
+    for (int q = sy.tid; q < cnt; q += sy.stride(cnt)) L.x[q] = T(0);
+  sy.end(cnt);
 }
 template <typename T>
-__device__ void c_resid_restrict(const CL<T>& F, const CL<T>& C, bool restrict_b, int tid,
-                                 int nt) {
-  const int m = F.m, n = F.n;
+__device__ void c_resid_restrict(const CL<T>& F, const CL<T>& C, bool restrict_b, Sync& sy) {
+  const int m = F.m, n = F.n, cells = m * n;
   const size_t P = (size_t)m * n;
   const int N = n - 2 * NH, M = m - 2 * NH;
   if (!restrict_b) {
-    for (int q = tid; q < M * N; q += nt) {
-      const int J = NH + q / N, I2 = NH + q % N;
-      const size_t c = at(J, I2, n);
-      const T v = residual_x<T>(F.msk[c], F.A, P, c, n, F.b[c], [&](int dd) { return F.x[c + dd]; });
-      store_img_mask<T>(F.r, nullptr, J, I2, v, m, n);
-    }
-    __syncthreads();
+    if (sy.begin(cells))
+      for (int q = sy.tid; q < M * N; q += sy.stride(cells)) {
+        const int J = NH + q / N, I2 = NH + q % N;
+        const size_t c = at(J, I2, n);
+        const T v = residual_x<T>(F.msk[c], F.A, P, c, n, F.b[c],
+                                  [&](int dd) { return F.x[c + dd]; });
+        store_img_mask<T>(F.t, nullptr, J, I2, v, m, n);
+      }
+    sy.end(cells);
   }
-  const T* src = restrict_b ? F.b : F.r;
-  const int N2 = C.n - 2 * NH, M2 = C.m - 2 * NH;
-  for (int q = tid; q < M2 * N2; q += nt) {
-    const int J2 = NH + q / N2, I2 = NH + q % N2;
-    T v = T(0);
-    if (C.msk[at(J2, I2, C.n)] != 0) {
-      const size_t c = at(2 * J2 + 1 - NH, 2 * I2 + 1 - NH, n);
-      v = restrict_r<T>([&](int dj, int di) { return src[c + (ptrdiff_t)dj * n + di]; });
+  const T* src = restrict_b ? F.b : F.t;
+  const int N2 = C.n - 2 * NH, M2 = C.m - 2 * NH, ccells = C.m * C.n;
+  if (sy.begin(ccells))
+    for (int q = sy.tid; q < M2 * N2; q += sy.stride(ccells)) {
+      const int J2 = NH + q / N2, I2 = NH + q % N2;
+      T v = T(0);
+      if (C.msk[at(J2, I2, C.n)] != 0) {
+        const size_t c = at(2 * J2 + 1 - NH, 2 * I2 + 1 - NH, n);
+        v = restrict_r<T>([&](int dj, int di) { return src[c + (ptrdiff_t)dj * n + di]; });
+      }
+      store_img_mask<T>(C.b, nullptr, J2, I2, v, C.m, C.n);
     }
-    store_img_mask<T>(C.b, nullptr, J2, I2, v, C.m, C.n);
-  }
-  __syncthreads();
+  sy.end(ccells);
 }
 template <typename T>
-__device__ void c_interp(const CL<T>& F, const CL<T>& C, bool acc, int tid, int nt) {
+__device__ void c_interp(const CL<T>& F, const CL<T>& C, bool acc, Sync& sy) {
   const int cnt = F.m * F.n;
-  for (int q = tid; q < cnt; q += nt) {
-    const int j = q / F.n, i = q % F.n;
-    const T r = interp_at<T>(F.msk, F.n, C.msk, C.x, C.n, j, i);
-    F.x[q] = acc ? F.x[q] + r : r;
-  }
-  __syncthreads();
+  if (sy.begin(cnt))
+    for (int q = sy.tid; q < cnt; q += sy.stride(cnt)) {
+      const int j = q / F.n, i = q % F.n;
+      const T r = interp_at<T>(F.msk, F.n, C.msk, C.x, C.n, j, i);
+      F.x[q] = acc ? F.x[q] + r : r;
+    }
+  sy.end(cnt);
 }
 
 template <typename T>
-__device__ void c_vcycle(CL<T>* Ls, int lev1, int deep, bool zero_first, T c1, T c2, int tid,
-                         int nt) {
+__device__ void c_vcycle(CL<T>* Ls, int lev1, int deep, bool zero_first, T c1, T c2, Sync& sy) {
   for (int l = lev1; l < deep; l++) {
-    if (l > lev1 || zero_first) c_zero(Ls[l].x, Ls[l].m * Ls[l].n, tid, nt);
-    c_smooth(Ls[l], c1, c2, tid, nt);
-    c_resid_restrict(Ls[l], Ls[l + 1], false, tid, nt);
+    if (l > lev1 || zero_first) c_zero(Ls[l], sy);
+    c_smooth(Ls[l], c1, c2, sy);
+    c_resid_restrict(Ls[l], Ls[l + 1], false, sy);
   }
-  c_zero(Ls[deep].x, Ls[deep].m * Ls[deep].n, tid, nt);
-  for (int k = 0; k < 16; k++) c_smooth(Ls[deep], c1, c2, tid, nt);
+  c_zero(Ls[deep], sy);
+  for (int k = 0; k < 16; k++) c_smooth(Ls[deep], c1, c2, sy);
   for (int l = deep - 1; l >= lev1; l--) {
-    c_interp(Ls[l], Ls[l + 1], true, tid, nt);
-    c_smooth(Ls[l], c1, c2, tid, nt);
+    c_interp(Ls[l], Ls[l + 1], true, sy);
+    c_smooth(Ls[l], c1, c2, sy);
   }
 }
 
+// Operands come from the T pool and are solved in FP64 in LDS (the
+// coarsest-grid solve stays FP64 in mixed mode).
 template <typename T>
 __global__ void __launch_bounds__(256) k_coarse(const Task* __restrict__ tasks,
-                                                const InstD* __restrict__ I, T* P,
+                                                const InstD* __restrict__ I, T* Q,
                                                 const int8_t* __restrict__ B,
                                                 const DynD* __restrict__ dyn, int pred, int mode) {
+  typedef double C;
   extern __shared__ __align__(16) unsigned char lds[];
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
   const InstD& in = I[tk.inst];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int k0 = in.nfine, deep = in.nlev - 1, nl = deep - k0 + 1;
-  __shared__ CL<T> Ls[MAXLEV];
-  // carve LDS: per level A(5), x, b, t, r (T), then masks
-  size_t off = 0;
-  if (tid == 0)
-  for (int l = 0; l < nl; l++) {
-    const LevD& G = in.L[k0 + l];
-    const int sz = G.m * G.n;
-    Ls[l].m = G.m;
-    Ls[l].n = G.n;
-    Ls[l].A = (T*)(lds + off);
-    off += sizeof(T) * 5 * sz;
-    Ls[l].x = (T*)(lds + off);
-    off += sizeof(T) * sz;
-    Ls[l].b = (T*)(lds + off);
-    off += sizeof(T) * sz;
-    Ls[l].t = (T*)(lds + off);
-    off += sizeof(T) * sz;
-    Ls[l].r = (T*)(lds + off);
-    off += sizeof(T) * sz;
-  }
-  if (tid == 0)
-  for (int l = 0; l < nl; l++) {
-    const LevD& G = in.L[k0 + l];
-    Ls[l].msk = (int8_t*)(lds + off);
-    off += (G.m * G.n + 15) & ~15;
+  __shared__ CL<C> Ls[MAXLEV];
+  // carve LDS: per level A(5), x, b, t (FP64), then masks
+  if (tid == 0) {
+    size_t off = 0;
+    for (int l = 0; l < nl; l++) {
+      const LevD& G = in.L[k0 + l];
+      const int sz = G.m * G.n;
+      Ls[l].m = G.m;
+      Ls[l].n = G.n;
+      Ls[l].A = (C*)(lds + off);
+      off += sizeof(C) * 5 * sz;
+      Ls[l].x = (C*)(lds + off);
+      off += sizeof(C) * sz;
+      Ls[l].b = (C*)(lds + off);
+      off += sizeof(C) * sz;
+      Ls[l].t = (C*)(lds + off);
+      off += sizeof(C) * sz;
+    }
+    for (int l = 0; l < nl; l++) {
+      const LevD& G = in.L[k0 + l];
+      Ls[l].msk = (int8_t*)(lds + off);
+      off += (G.m * G.n + 15) & ~15;
+    }
   }
   __syncthreads();
   for (int l = 0; l < nl; l++) {
     const LevD& G = in.L[k0 + l];
     const int sz = G.m * G.n;
-    const T* Ag = P + G.A;
-    for (int q = tid; q < 5 * sz; q += nt) Ls[l].A[q] = Ag[q];
+    const T* Ag = Q + G.A;
+    for (int q = tid; q < 5 * sz; q += nt) Ls[l].A[q] = (C)Ag[q];
     const int8_t* mg = B + G.msk;
     for (int q = tid; q < sz; q += nt) Ls[l].msk[q] = mg[q];
-    for (int q = tid; q < sz; q += nt) Ls[l].t[q] = T(0);
+    for (int q = tid; q < sz; q += nt) Ls[l].t[q] = C(0);
   }
   {
     const int sz = Ls[0].m * Ls[0].n;
-    const T* bg = P + in.L[k0].b;
-    for (int q = tid; q < sz; q += nt) Ls[0].b[q] = bg[q];
+    const T* bg = Q + in.L[k0].b;
+    for (int q = tid; q < sz; q += nt) Ls[0].b[q] = (C)bg[q];
   }
   __syncthreads();
-  const T c1 = T(in.omega), c2 = T(1. - in.omega);
+  const C c1 = C(in.omega), c2 = C(1. - in.omega);
+  Sync sy;
+  sy.tid = tid;
+  sy.nt = nt;
+  sy.prev_small = false;
   if (mode == 0) {
-    c_vcycle(Ls, 0, nl - 1, true, c1, c2, tid, nt);
+    c_vcycle(Ls, 0, nl - 1, true, c1, c2, sy);
   } else {
-    for (int l = 0; l < nl - 1; l++) c_resid_restrict(Ls[l], Ls[l + 1], true, tid, nt);
-    c_zero(Ls[nl - 1].x, Ls[nl - 1].m * Ls[nl - 1].n, tid, nt);
-    for (int k = 0; k < 16; k++) c_smooth(Ls[nl - 1], c1, c2, tid, nt);
+    for (int l = 0; l < nl - 1; l++) c_resid_restrict(Ls[l], Ls[l + 1], true, sy);
+    c_zero(Ls[nl - 1], sy);
+    for (int k = 0; k < 16; k++) c_smooth(Ls[nl - 1], c1, c2, sy);
     for (int l = nl - 2; l >= 0; l--) {
-      c_interp(Ls[l], Ls[l + 1], false, tid, nt);
-      c_vcycle(Ls, l, nl - 1, false, c1, c2, tid, nt);
+      c_interp(Ls[l], Ls[l + 1], false, sy);
+      c_vcycle(Ls, l, nl - 1, false, c1, c2, sy);
     }
   }
-  T* xg = P + tk.xout;
+  __syncthreads();
+  T* xg = Q + tk.xout;
   const int sz = Ls[0].m * Ls[0].n;
-  for (int q = tid; q < sz; q += nt) xg[q] = Ls[0].x[q];
+  for (int q = tid; q < sz; q += nt) xg[q] = T(Ls[0].x[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -524,6 +617,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_ortho(const Task* __restrict__ tasks,
                                                const InstD* __restrict__ I, T* P,
                                                const int8_t* __restrict__ B) {
+  // P: the pool holding the state-shaped array (FP64 state or T tendency)
   const Task tk = tasks[blockIdx.x];
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
@@ -549,56 +643,56 @@ constexpr int FLAT = 4096;
 // final RK combination of the fields in fmask (bit k: field k)
 template <typename T>
 __global__ void __launch_bounds__(256) k_combine(const Task* __restrict__ tasks,
-                                                 const InstD* __restrict__ I, T* P,
+                                                 const InstD* __restrict__ I, double* P, T* Q,
                                                  const DynD* __restrict__ dyn) {
   const Task tk = tasks[blockIdx.x];
   const InstD& in = I[tk.inst];
   const size_t F = (size_t)in.m * in.n;
   const double dt = dyn[tk.inst].dt;
-  T* x = P + in.state;
-  const T *k0 = P + in.k[0], *k1 = P + in.k[1], *k2 = P + in.k[2];
+  double* x = P + in.state;
+  const T *k0 = Q + in.k[0], *k1 = Q + in.k[1], *k2 = Q + in.k[2];
   const int field = tk.lev;
   const size_t base = (size_t)field * F;
   for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
     const size_t e = base + q;
     if (in.tscheme == 0) {
-      const T c6 = T(dt / 6.);
-      x[e] = x[e] + c6 * ((k0[e] + k1[e]) + T(4) * k2[e]);
+      const double c6 = dt / 6.;
+      x[e] = x[e] + c6 * (((double)k0[e] + (double)k1[e]) + 4. * (double)k2[e]);
     } else {
-      x[e] = x[e] + T(dt) * k2[e];
+      x[e] = x[e] + dt * (double)k2[e];
     }
-    if (in.model == 1 && field == 4) x[5 * F + q] = x[e] - T(0);  // banom = b - bref(=0)
+    if (in.model == 1 && field == 4) x[5 * F + q] = x[e] - 0.;  // banom = b - bref(=0)
   }
 }
 
 // psi += x0 over the whole array (hierarchy.py:171)
 template <typename T>
 __global__ void __launch_bounds__(256) k_update(const Task* __restrict__ tasks,
-                                                const InstD* __restrict__ I, T* P,
+                                                const InstD* __restrict__ I, double* P, T* Q,
                                                 const DynD* __restrict__ dyn, int pred) {
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
-  T* x = P + tk.xout;
-  const T* y = P + tk.xin;
-  for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) x[q] = x[q] + y[q];
+  double* x = P + tk.xout;
+  const T* y = Q + tk.xin;
+  for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x)
+    x[q] = x[q] + (double)y[q];
 }
 
 // psi = (psi - psim) * mskp (core/operators.py:474-481), psim used if perio
-template <typename T>
 __global__ void __launch_bounds__(256) k_finalize_psi(const Task* __restrict__ tasks,
-                                                      const InstD* __restrict__ I, T* P,
+                                                      const InstD* __restrict__ I, double* P,
                                                       const int8_t* __restrict__ B,
                                                       const DynD* __restrict__ dyn) {
   const Task tk = tasks[blockIdx.x];
   const InstD& in = I[tk.inst];
   const size_t F = (size_t)in.m * in.n;
-  T* psi = P + in.state + F;
+  double* psi = P + in.state + F;
   const int8_t* mp = B + in.mskp;
-  const T psim = T(dyn[tk.inst].psim);
+  const double psim = dyn[tk.inst].psim;
   for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
-    T v = psi[q];
+    double v = psi[q];
     if (in.perio) v = v - psim;
-    psi[q] = v * T(mp[q]);
+    psi[q] = v * (double)mp[q];
   }
 }
 
@@ -661,28 +755,56 @@ __device__ double np_pairwise_flat(const T* x, int n, int N, size_t start, size_
   return ret;
 }
 
-// one thread per 8192-element chunk of psi's interior; task.y0 = first chunk
-// index, task.x0 = chunk count; results at partial[task-local slot]
-template <typename T>
-__global__ void k_npsum(const Task* __restrict__ tasks, const InstD* __restrict__ I, T* P,
-                        double* partial, const int* __restrict__ slot) {
+// One wavefront per 8192-element chunk of psi's interior (task.y0 = first
+// chunk, task.x0 = chunk count, one block per task with 64 threads per
+// chunk).  When the chunk length is 128 * 2^k -- every chunk of a
+// power-of-two grid -- NumPy's recursion splits it into equal halves down to
+// 128-element leaves, so lane L sums leaf L with NumPy's eight-accumulator
+// loop and the leaves are combined as the same balanced tree.  Other
+// lengths run NumPy's recursion on lane 0.
+__global__ void __launch_bounds__(256) k_npsum(const Task* __restrict__ tasks,
+                                               const InstD* __restrict__ I, double* P,
+                                               double* partial, const int* __restrict__ slot) {
   const Task tk = tasks[blockIdx.x];
   const InstD& in = I[tk.inst];
   if (!in.perio) return;
-  const int ch = threadIdx.x;
-  if (ch >= tk.x0) return;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int m = in.m, n = in.n, N = n - 2 * NH;
   const size_t tot = (size_t)(m - 2 * NH) * N;
-  const size_t s = (size_t)(tk.y0 + ch) * 8192;
-  const size_t cnt = tot - s < 8192 ? tot - s : 8192;
-  partial[slot[blockIdx.x] + ch] = np_pairwise_flat<T>(P + in.state + (size_t)m * n, n, N, s, cnt);
+  const double* psi = P + in.state + (size_t)m * n;
+  for (int ch = wv; ch < tk.x0; ch += blockDim.x >> 6) {
+    const size_t s0 = (size_t)(tk.y0 + ch) * 8192;
+    const size_t cnt = tot - s0 < 8192 ? tot - s0 : 8192;
+    const size_t leaves = cnt / 128;
+    const bool balanced = cnt >= 128 && cnt % 128 == 0 && (leaves & (leaves - 1)) == 0;
+    double r = 0.;
+    if (!balanced) {
+      if (lane == 0) r = np_pairwise_flat<double>(psi, n, N, s0, cnt);
+    } else {
+      if ((size_t)lane < leaves) {
+        const size_t b0 = s0 + (size_t)lane * 128;
+        auto val = [&](size_t q) { return psi[at(NH + (int)(q / N), NH + (int)(q % N), n)]; };
+        double a[8];
+        for (int k = 0; k < 8; k++) a[k] = val(b0 + k);
+        for (int k = 8; k < 128; k += 8)
+          for (int q = 0; q < 8; q++) a[q] += val(b0 + k + q);
+        r = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+      }
+      // balanced tree: at width w, lane L (L % 2w == 0) adds lane L + w
+      for (size_t w = 1; w < leaves; w <<= 1) {
+        const double o = __shfl_down(r, (unsigned)w, 64);
+        if ((lane % (2 * w)) == 0) r = r + o;
+      }
+    }
+    if (lane == 0) partial[slot[blockIdx.x] + ch] = r;
+  }
 }
 
 // diagnostics partials: {maxu, ke, sum w, sum w^2, d4, d5, d6, d7}
-template <typename T>
 __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
-                                              const InstD* __restrict__ I, T* P,
+                                              const InstD* __restrict__ I, const double* P,
                                               const int8_t* __restrict__ B, double* partial) {
+  typedef double T;
   __shared__ double red[8][4];
   const Task tk = tasks[blockIdx.x];
   const InstD& in = I[tk.inst];
@@ -691,8 +813,8 @@ __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
   const int8_t* msk = B + in.msk;
   const T* s = P + in.state;
   const T *w = s, *psi = s + F, *u = s + 2 * F, *v = s + 3 * F, *f4 = s + 4 * F;
-  const double* xr = (const double*)(P) + in.xr;
-  const double* yr = (const double*)(P) + in.yr;
+  const double* xr = P + in.xr;
+  const double* yr = P + in.yr;
   double a[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   for (int r = threadIdx.y; r < TY; r += blockDim.y) {
     const int J = tk.y0 + r, Ic = tk.x0 + threadIdx.x;
@@ -902,10 +1024,12 @@ struct HostInst {
 
 struct Batch {
   int ninst = 0;
+  int prec = 0;  // 0: FP64 throughout; 1: FP32 tendencies and V-cycles
   std::vector<HostInst> H;
-  double* P = nullptr;
+  double* P = nullptr;  // FP64 pool
+  void* Q = nullptr;    // T pool (double or float)
   int8_t* B = nullptr;
-  size_t nP = 0, nB = 0;
+  size_t nP = 0, nQ = 0, nB = 0;
   InstD* dI = nullptr;
   DynD* dD = nullptr;
   DynD* hD = nullptr;  // pinned mirror
@@ -1075,7 +1199,8 @@ struct Builder {
 
   using Off = std::function<long long(HostInst&)>;
 
-  void c2c(const std::vector<int>& who, Off src, Off dst, int pred) {
+  // fp64: operands in the FP64 pool (state / work) instead of the T pool
+  void c2c(const std::vector<int>& who, Off src, Off dst, int pred, bool fp64) {
     std::vector<Task> ts;
     for (int i : who) {
       HostInst& h = b.H[i];
@@ -1084,10 +1209,12 @@ struct Builder {
       t.xout = dst(h);
       tiles(ts, t, 0, h.d.m - 1, 0, h.d.n - 1);
     }
-    add(L_C2C, ts).pred = pred;
+    Launch& L = add(L_C2C, ts);
+    L.pred = pred;
+    L.flags = fp64 ? 1 : 0;
   }
 
-  void ortho(const std::vector<int>& who, Off base) {
+  void ortho(const std::vector<int>& who, Off base, bool fp64) {
     std::vector<Task> ts;
     for (int i : who) {
       HostInst& h = b.H[i];
@@ -1095,7 +1222,7 @@ struct Builder {
       t.xout = base(h);
       tiles(ts, t, 1, h.d.m - 1, 1, h.d.n - 1);
     }
-    add(L_ORTHO, ts);
+    add(L_ORTHO, ts).flags = fp64 ? 1 : 0;
   }
 
   // flat element ranges [0, cnt) per instance
@@ -1128,19 +1255,21 @@ struct Builder {
       h.buf[0][1] = h.x0alt;
       h.cur[0] = 0;
     }
-    c2c(who, [k](HostInst& h) { return h.d.k[k]; }, [](HostInst& h) { return h.d.L[0].b; }, 0);
+    c2c(who, [k](HostInst& h) { return h.d.k[k]; }, [](HostInst& h) { return h.d.L[0].b; }, 0,
+        false);
     std::vector<int> st(b.ninst, 0);
     vcycle(who, st, 0, 0, false);
     vcycle(who, st, 0, 0, true);
     for (int i : who)
       if (b.H[i].cur[0] != 0) fprintf(stderr, "engine: fast inversion parity\n");
-    ortho(who, [k](HostInst& h) { return h.d.k[k]; });
+    ortho(who, [k](HostInst& h) { return h.d.k[k]; }, false);
   }
 
   // full solve of the state's psi (core/operators.py:470-481)
   void solve_full(const std::vector<int>& who) {
     if (who.empty()) return;
-    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0);
+    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0,
+        true);
     for (int i : who) {
       HostInst& h = b.H[i];
       h.buf[0][0] = h.d.x0[0];
@@ -1156,7 +1285,7 @@ struct Builder {
       }
       const int sp = spans_of(rs);
       Launch& L = add(L_RR, rs);
-      L.flags = 1 | 4 | 8 | 16;
+      L.flags = 1 | 4 | 8 | 16 | 32;  // FP64 outer residual of the refinement
       L.pred = pred;
       fold(what, sp, pred);
     };
@@ -1209,8 +1338,8 @@ struct Builder {
       HostInst& h = b.H[i];
       const long long tot = (long long)(h.d.m - 2 * NH) * (h.d.n - 2 * NH);
       const int nch = (int)((tot + 8191) / 8192);
-      for (int c = 0; c < nch; c += 256) {
-        Task t = mk_task(i, 0, c, std::min(256, nch - c));
+      for (int c = 0; c < nch; c += 4) {  // one wavefront per chunk
+        Task t = mk_task(i, 0, c, std::min(4, nch - c));
         ns.push_back(t);
       }
     }
@@ -1231,7 +1360,7 @@ struct Builder {
       fold(3, spoff, 0);
     }
     add(L_FINPSI, flat(who, [](HostInst& h) { return (long long)h.d.m * h.d.n; }, 1));
-    ortho(who, [](HostInst& h) { return h.d.state; });
+    ortho(who, [](HostInst& h) { return h.d.state; }, true);
   }
 
   void diag() {
@@ -1286,50 +1415,62 @@ struct Builder {
 };
 
 // ---- execution ------------------------------------------------------------
-static int run_prog(Batch& b, const std::vector<Launch>& prog) {
+template <typename T>
+static int run_prog_t(Batch& b, const std::vector<Launch>& prog) {
   const dim3 B2(64, 4), B1(256);
+  T* Q = (T*)b.Q;
+  double* P = b.P;
   for (const Launch& L : prog) {
-    const Task* T = b.dT + L.toff;
+    const Task* TK = b.dT + L.toff;
     const unsigned g = (unsigned)L.ntask;
     if (L.kind != L_FOLD && g == 0) continue;
     switch (L.kind) {
       case L_RHS:
-        hipLaunchKernelGGL(k_rhs<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B, b.dD, L.stage,
+        hipLaunchKernelGGL(k_rhs<T>, dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
                            L.outk);
         break;
       case L_C2C:
-        hipLaunchKernelGGL(k_c2c<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.dD, L.pred);
+        if (L.flags & 1)
+          hipLaunchKernelGGL(k_c2c<double>, dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.dD, L.pred);
+        else
+          hipLaunchKernelGGL(k_c2c<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.dD, L.pred);
         break;
       case L_SMOOTH:
-        hipLaunchKernelGGL(k_smooth<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B, b.dD,
-                           L.pred);
+        hipLaunchKernelGGL(k_smooth<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
         break;
       case L_RR:
-        hipLaunchKernelGGL(k_resid_restrict<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B,
-                           b.dD, L.pred, L.flags, b.partial);
+        if (L.flags & 32)
+          hipLaunchKernelGGL((k_resid_restrict<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, Q,
+                             b.B, b.dD, L.pred, L.flags, b.partial);
+        else
+          hipLaunchKernelGGL((k_resid_restrict<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.B,
+                             b.dD, L.pred, L.flags, b.partial);
         break;
       case L_COARSE:
-        hipLaunchKernelGGL(k_coarse<double>, dim3(g), dim3(256), L.shmem, b.s, T, b.dI, b.P, b.B,
+        hipLaunchKernelGGL(k_coarse<T>, dim3(g), dim3(256), L.shmem, b.s, TK, b.dI, Q, b.B,
                            b.dD, L.pred, L.mode);
         break;
       case L_ORTHO:
-        hipLaunchKernelGGL(k_ortho<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B);
+        if (L.flags & 1)
+          hipLaunchKernelGGL(k_ortho<double>, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B);
+        else
+          hipLaunchKernelGGL(k_ortho<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B);
         break;
       case L_COMBINE:
-        hipLaunchKernelGGL(k_combine<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.dD);
+        hipLaunchKernelGGL(k_combine<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD);
         break;
       case L_UPDATE:
-        hipLaunchKernelGGL(k_update<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.dD, L.pred);
+        hipLaunchKernelGGL(k_update<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD, L.pred);
         break;
       case L_FINPSI:
-        hipLaunchKernelGGL(k_finalize_psi<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.B, b.dD);
+        hipLaunchKernelGGL(k_finalize_psi, dim3(g), B1, 0, b.s, TK, b.dI, P, b.B, b.dD);
         break;
       case L_NPSUM:
-        hipLaunchKernelGGL(k_npsum<double>, dim3(g), B1, 0, b.s, T, b.dI, b.P, b.partial,
+        hipLaunchKernelGGL(k_npsum, dim3(g), B1, 0, b.s, TK, b.dI, P, b.partial,
                            b.dSlot + L.slot_off);
         break;
       case L_DIAG:
-        hipLaunchKernelGGL(k_diag<double>, dim3(g), B2, 0, b.s, T, b.dI, b.P, b.B, b.partial);
+        hipLaunchKernelGGL(k_diag, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B, b.partial);
         break;
       case L_FOLD:
         hipLaunchKernelGGL(k_fold, dim3(b.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
@@ -1341,11 +1482,16 @@ static int run_prog(Batch& b, const std::vector<Launch>& prog) {
   return F2D_OK;
 }
 
+static int run_prog(Batch& b, const std::vector<Launch>& prog) {
+  return b.prec == 0 ? run_prog_t<double>(b, prog) : run_prog_t<float>(b, prog);
+}
+
 static void batch_free(Batch* b) {
   if (!b) return;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->P) (void)hipFree(b->P);
+  if (b->Q) (void)hipFree(b->Q);
   if (b->B) (void)hipFree(b->B);
   if (b->dI) (void)hipFree(b->dI);
   if (b->dD) (void)hipFree(b->dD);
@@ -1394,7 +1540,7 @@ static int build_programs(Batch& b) {
 
 static size_t coarse_bytes(const InstD& d, int k0) {
   size_t sz = 0;
-  for (int l = k0; l < d.nlev; l++) sz += (size_t)d.L[l].m * d.L[l].n * (9 * sizeof(double));
+  for (int l = k0; l < d.nlev; l++) sz += (size_t)d.L[l].m * d.L[l].n * (8 * sizeof(double));
   for (int l = k0; l < d.nlev; l++) sz += ((size_t)d.L[l].m * d.L[l].n + 15) & ~(size_t)15;
   return sz;
 }
@@ -1412,9 +1558,12 @@ extern "C" {
 const char* f2d_batch_last_error(void) { return e_err.c_str(); }
 
 int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* const* msks,
-                     int coarse_lds_budget, f2d_batch** out) {
+                     int precision, int coarse_lds_budget, f2d_batch** out) {
   if (ninst < 1) return efail(F2D_E_ARG, "ninst must be >= 1");
+  if (precision != 0 && precision != 1) return efail(F2D_E_ARG, "precision must be 0 or 1");
   f2d_batch* b = new f2d_batch();
+  b->prec = precision;
+  const size_t tsz = precision == 0 ? sizeof(double) : sizeof(float);
   auto bail = [&](int rc) {
     batch_free(b);
     return rc;
@@ -1425,14 +1574,21 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
   (void)hipGetDevice(&dev);
   int maxsh = 65536;
   (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  size_t budget = coarse_lds_budget > 0 ? (size_t)coarse_lds_budget : 65536;
-  budget = std::min(budget, (size_t)maxsh);
+  // static LDS of k_coarse (level table) comes on top of the dynamic part
+  const size_t avail = (size_t)maxsh > 4096 ? (size_t)maxsh - 4096 : (size_t)maxsh;
+  size_t budget = coarse_lds_budget > 0 ? (size_t)coarse_lds_budget : avail;
+  budget = std::min(budget, avail);
   const int tscheme0 = params[0].timestepping;
   // layout
-  size_t nP = 0, nB = 0;
+  size_t nP = 0, nQ = 0, nB = 0;
   auto takeP = [&](size_t cnt) {
     const size_t o = nP;
     nP += (cnt + 31) & ~(size_t)31;
+    return (long long)o;
+  };
+  auto takeQ = [&](size_t cnt) {
+    const size_t o = nQ;
+    nQ += (cnt + 63) & ~(size_t)63;
     return (long long)o;
   };
   auto takeB = [&](size_t cnt) {
@@ -1492,23 +1648,25 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
     h.coarse_lds = coarse_bytes(d, k0);
     const size_t F = (size_t)d.m * d.n;
     d.state = takeP(d.nvar * F);
-    for (int k = 0; k < 3; k++) d.k[k] = takeP(d.nvar * F);
     d.work = takeP(F);
-    d.x0[0] = takeP(F);
-    d.x0[1] = takeP(F);
-    h.x0alt = takeP(F);
     d.xr = takeP(F);
     d.yr = takeP(F);
+    d.A0d = takeP(5 * F);
+    for (int k = 0; k < 3; k++) d.k[k] = takeQ(d.nvar * F);
+    d.x0[0] = takeQ(F);
+    d.x0[1] = takeQ(F);
+    h.x0alt = takeQ(F);
     d.msk = takeB(F);
     d.mskp = takeB(F);
+    d.sb0 = takeB(F);
     for (int l = 0; l < nl; l++) {
       LevD& L = d.L[l];
       const size_t S = (size_t)L.m * L.n;
-      L.A = takeP(5 * S);
-      L.b = takeP(S);
+      L.A = takeQ(5 * S);
+      L.b = takeQ(S);
       if (l > 0) {
-        L.x[0] = takeP(S);
-        L.x[1] = takeP(S);
+        L.x[0] = takeQ(S);
+        L.x[1] = takeQ(S);
       }
       L.msk = l == 0 ? d.mskp : takeB(S);
       h.buf[l][0] = L.x[0];
@@ -1521,13 +1679,16 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
     d.area = (double)a;
   }
   b->nP = nP;
+  b->nQ = nQ;
   b->nB = nB;
   if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess)
     return bail(efail(F2D_E_HIP, "stream"));
   if (hipMalloc(&b->P, nP * sizeof(double)) != hipSuccess ||
+      hipMalloc(&b->Q, std::max<size_t>(nQ, 1) * tsz) != hipSuccess ||
       hipMalloc(&b->B, std::max<size_t>(nB, 1)) != hipSuccess)
-    return bail(efail(F2D_E_NOMEM, "pools (%zu doubles, %zu bytes)", nP, nB));
+    return bail(efail(F2D_E_NOMEM, "pools (%zu doubles, %zu reals, %zu bytes)", nP, nQ, nB));
   if (hipMemset(b->P, 0, nP * sizeof(double)) != hipSuccess ||
+      hipMemset(b->Q, 0, std::max<size_t>(nQ, 1) * tsz) != hipSuccess ||
       hipMemset(b->B, 0, std::max<size_t>(nB, 1)) != hipSuccess)
     return bail(efail(F2D_E_HIP, "memset pools"));
   // masks, operators: level hierarchy built by the multigrid set-up kernels
@@ -1563,9 +1724,47 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
         f2d_gmg_free_raw(g);
         return bail(efail(F2D_E_ARG, "level shape mismatch"));
       }
-      if (hipMemcpy(b->P + d.L[l].A, gA, 5 * S * sizeof(double), hipMemcpyDeviceToDevice) !=
-              hipSuccess ||
-          (l > 0 && hipMemcpy(b->B + d.L[l].msk, gm, S, hipMemcpyDeviceToDevice) != hipSuccess)) {
+      bool ok = true;
+      if (b->prec == 0) {
+        ok = hipMemcpy((double*)b->Q + d.L[l].A, gA, 5 * S * sizeof(double),
+                       hipMemcpyDeviceToDevice) == hipSuccess;
+      } else {  // operators rounded to FP32 for the FP32 V-cycles
+        std::vector<double> hA(5 * S);
+        std::vector<float> fA(5 * S);
+        ok = hipMemcpy(hA.data(), gA, 5 * S * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+        for (size_t q = 0; q < 5 * S; q++) fA[q] = (float)hA[q];
+        ok = ok && hipMemcpy((float*)b->Q + d.L[l].A, fA.data(), 5 * S * sizeof(float),
+                             hipMemcpyHostToDevice) == hipSuccess;
+      }
+      if (l == 0) {
+        ok = ok && hipMemcpy(b->P + d.A0d, gA, 5 * S * sizeof(double),
+                             hipMemcpyDeviceToDevice) == hipSuccess;
+        // stencil bits of the finest operator, verified value by value
+        std::vector<double> hA(5 * S);
+        ok = ok && hipMemcpy(hA.data(), gA, 5 * S * sizeof(double), hipMemcpyDeviceToHost) ==
+                       hipSuccess;
+        std::vector<uint8_t> bits(S, 0);
+        bool fly = ok;
+        for (int k = 0; k < 5 && fly; k++) {
+          double wk = 0.;
+          for (size_t q = 0; q < S; q++) {
+            const double a = hA[k * S + q];
+            if (a == 0.) continue;
+            if (wk == 0.) wk = a;
+            if (a != wk) {
+              fly = false;
+              break;
+            }
+            bits[q] |= (uint8_t)(1u << k);
+          }
+          d.w0[k] = wk;
+        }
+        d.a0fly = fly ? 1 : 0;
+        if (fly)
+          ok = ok && hipMemcpy(b->B + d.sb0, bits.data(), S, hipMemcpyHostToDevice) == hipSuccess;
+      }
+      if (l > 0) ok = ok && hipMemcpy(b->B + d.L[l].msk, gm, S, hipMemcpyDeviceToDevice) == hipSuccess;
+      if (!ok) {
         f2d_gmg_free_raw(g);
         return bail(efail(F2D_E_HIP, "level copy"));
       }
@@ -1595,10 +1794,12 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
   memset(b->hD, 0, ninst * sizeof(DynD));
   size_t maxlds = 0;
   for (auto& h : b->H) maxlds = std::max(maxlds, h.coarse_lds);
-  if (maxlds > 65536 &&
-      hipFuncSetAttribute((const void*)k_coarse<double>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds) != hipSuccess)
-    return bail(efail(F2D_E_HIP, "coarse LDS %zu bytes", maxlds));
+  if (maxlds > 65536) {
+    const void* fn = b->prec == 0 ? (const void*)k_coarse<double> : (const void*)k_coarse<float>;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds) !=
+        hipSuccess)
+      return bail(efail(F2D_E_HIP, "coarse LDS %zu bytes", maxlds));
+  }
   int rc = build_programs(*b);
   if (rc) return bail(rc);
   *out = b;

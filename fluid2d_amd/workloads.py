@@ -78,6 +78,22 @@ CONFIGS = {
 }
 
 
+def ensemble_specs(n=256, seed=4, sizes=(64, 128, 256, 512), geometries=("perio", "closed")):
+    """Batched ensemble: n independent Euler runs, grid size and geometry
+    drawn (seeded) from ``sizes`` and ``geometries``; initial field k^-3
+    turbulence with seed 1000 + i."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = int(rng.choice(sizes))
+        geo = str(rng.choice(geometries))
+        kw = dict(modelname="euler", nx=s, ny=s, Lx=1.0, Ly=1.0, geometry=geo,
+                  adaptable_dt=False, order=5, flux_splitting_method="parabolic",
+                  timestepping="RK3_SSP")
+        out.append((kw, 1000 + i))
+    return out
+
+
 def make_param(name, **over):
     kw, _, _ = CONFIGS[name]
     kw = dict(kw)

@@ -216,11 +216,15 @@ int f2d_model_last_solve(f2d_model *M, int *nite, double *res);
  */
 typedef struct f2d_batch f2d_batch;
 const char *f2d_batch_last_error(void);
-/* coarse_lds_budget: bytes of LDS for the single-workgroup coarse-level
+/* precision: 0 FP64 throughout (bit-identical to the reference algorithm);
+ *   1 mixed: FP64 state, RK combination, full-solve rhs, outer residual and
+ *   correction (iterative refinement) and coarse-level solve; FP32
+ *   advection, tendencies and fine-level V-cycles.
+ * coarse_lds_budget: bytes of LDS for the single-workgroup coarse-level
  * kernel (0 = 64 KiB); levels are handed to it from the coarsest up while
  * they fit.  msks[i]: host cell mask of instance i. */
 int f2d_batch_create(int ninst, const f2d_model_params *params, const int8_t *const *msks,
-                     int coarse_lds_budget, f2d_batch **out);
+                     int precision, int coarse_lds_budget, f2d_batch **out);
 int f2d_batch_destroy(f2d_batch *b);
 int f2d_batch_info(f2d_batch *b, int inst, int *nlev, int *nfine, long long *launches_per_step);
 int f2d_batch_set_field(f2d_batch *b, int inst, int field, const double *host);

@@ -124,3 +124,28 @@ def test_graph_equals_eager():
         b.step(2)  # replays after the first capture
         out.append(b.state(0))
     assert (out[0] == out[1]).all()
+
+
+@pytest.mark.parametrize("case", [
+    ("euler", 128, 128, 1.0, 1.0, "perio", 50),
+    ("euler", 256, 256, 1.0, 1.0, "closed", 50),
+    ("boussinesq", 256, 128, 2.0, 1.0, "xchannel", 50),
+])
+def test_mixed_precision_error(case):
+    """Mixed mode (FP32 advection/V-cycles, FP64 state and refinement) stays
+    within rel-L2 1e-6 of the FP64 path (itself bit-identical to the oracle)."""
+    kind, nx, ny, Lx, Ly, geom, nsteps = case
+    kw = dict(diffusion=True, Kdiff=1e-5) if kind == "boussinesq" else {}
+    out = {}
+    for prec in ("fp64", "mixed"):
+        b = Batch([spec(kind, nx, ny, Lx, Ly, geom, **kw)], precision=prec)
+        init_fields(b, 0, 3)
+        b.set_psi_from_vorticity()
+        if prec == "fp64":
+            dt = dt_for(b, 0, b.diagnostics(0)["maxspeed"])
+        b.set_dt(dt)
+        b.step(nsteps)
+        out[prec] = b.state(0)
+    for k in range(out["fp64"].shape[0]):
+        r = rel(out["mixed"][k], out["fp64"][k])
+        assert r <= 1e-6, "field %d rel-L2 %.3e" % (k, r)
