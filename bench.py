@@ -1,14 +1,29 @@
 """Benchmark: cell-updates/s of the Fluid2d timestep path on one MI355X.
 
-Workload (default): Euler (vorticity-streamfunction), 1024x1024 closed
-free-slip box, FP64, UP5 parabolic advection, RK3_SSP, fixed dt at CFL 0.8
-of the initial maxspeed, multigrid Poisson (2 V-cycles per inverted stage +
-the end-of-step F-cycle solve, maxite 4, tol 1e-11), k^-3 turbulence init.
+Headline (``value``): the plain FP64 path on the Euler 1024x1024 closed
+free-slip box (UP5 parabolic advection, RK3_SSP, fixed dt at CFL 0.8 of the
+initial maxspeed, multigrid Poisson: 2 V-cycles per inverted stage + the
+end-of-step F-cycle solve with maxite 4 / tol 1e-11, k^-3 turbulence init).
 A "step" is one RK3_SSP step of that grid including every multigrid cycle
-and the per-step diagnostics; inputs are resident in HBM before timing.
+and the per-step diagnostics, replayed from a hipGraph; inputs are resident
+in HBM before timing.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this
-build runs on one GPU.
+Further legs, reported as extra keys of the same JSON line:
+
+* ``roofline``  the kernel with the largest share of the plain step, its
+  algorithmic bytes per launch / mean launch time (HIP events on the
+  engine's stream) against 8 TB/s;
+* ``batched``   256-instance ensemble (sizes 64..512, periodic/closed) in
+  mixed precision, one graph per step for all instances;
+* ``precision`` rel-L2 of mixed precision against FP64 (the FP64 engine is
+  bit-identical to the oracle) on the 1024^2 box, the 2048x1024 Boussinesq
+  Rayleigh-Taylor run and the ensemble;
+* ``serving``   Poisson arrivals (50/s) of Euler/Boussinesq runs of uneven
+  size, continuous batching, p50/p99 time-to-result;
+* ``cpu_baseline`` the C oracle (FP64, 1 thread) on a bounded sample of the
+  headline workload.
+
+--gpus N > 1 is refused: this build runs on one GPU.
 """
 import argparse
 import json
@@ -22,29 +37,105 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+HEADLINE = "euler-closed-1024"
+BOUSSINESQ = "boussinesq-rt-2048x1024"
+KERNEL_NAMES = {"smooth": "k_smooth", "resid_restrict": "k_resid_restrict", "rhs": "k_rhs",
+                "coarse": "k_coarse"}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="euler-closed-1024")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=0,
-                    help="oracle steps for the CPU baseline (0: auto, ~15 s)")
+    ap.add_argument("--legs", default="plain,roofline,batched,precision,serving,cpu",
+                    help="comma list of legs to run")
+    ap.add_argument("--ensemble", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--bous-steps", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     return ap.parse_args()
 
 
-def words_per_cell_update(model, n_f):
-    """Algorithmic FP64 words per interior cell per step (each fused pass
-    reads its inputs and writes its outputs once; multigrid operator
-    matrix-free; masks and halos ignored): Euler RK3_SSP 109 + 28 n_F,
-    Boussinesq RK3_SSP 162."""
-    return 109 + 28 * n_f if model == "euler" else 162
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(name, steps_hint):
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def max_rel(sa, sb):
+    return max(rel(sa[k], sb[k]) for k in range(sa.shape[0]))
+
+
+def config_batch(name, precision, dt=None):
+    from fluid2d_amd import workloads
+    from fluid2d_amd.batch import Batch
+    from fluid2d_amd.grid import Grid
+
+    p = workloads.make_param(name)
+    g = Grid(p)
+    b = Batch([(p, g)], precision=precision)
+    for k, v in workloads.initial_fields(name, g).items():
+        b.set_field(0, k, v)
+    b.set_psi_from_vorticity()
+    if dt is None:
+        dt = workloads.time_step(name, g, b.diagnostics(0)["maxspeed"])
+    b.set_dt(dt)
+    return b, g, dt
+
+
+def ensemble_batch(n, precision, dts=None):
+    from fluid2d_amd import workloads
+    from fluid2d_amd.batch import Batch
+    from fluid2d_amd.grid import Grid
+    from fluid2d_amd.param import Param
+
+    specs = workloads.ensemble_specs(n)
+    pg = []
+    for kw, _ in specs:
+        p = Param(**kw)
+        pg.append((p, Grid(p)))
+    b = Batch(pg, precision=precision)
+    for i, (_, seed) in enumerate(specs):
+        b.set_field(i, "vorticity", workloads.turbulence(b.grids[i], seed))
+    b.set_psi_from_vorticity()
+    if dts is None:
+        dts = [0.8 * min(g.dx, g.dy) / b.diagnostics(i)["maxspeed"] for i, g in enumerate(b.grids)]
+    for i, dt in enumerate(dts):
+        b.set_dt(dt, i)
+    cells = sum(g.nx * g.ny for g in b.grids)
+    return b, dts, cells
+
+
+def timed(b, steps):
+    import fluid2d_amd as F
+    b.synchronize()
+    F.load().f2d_synchronize()
+    t0 = time.perf_counter()
+    b.step_async(steps)
+    b.synchronize()
+    F.load().f2d_synchronize()
+    return time.perf_counter() - t0
+
+
+def roofline(b, nsteps=2):
+    """Dominant kernel of one step: the kind with the largest event-timed
+    share; its finest-level launches give the roofline figure."""
+    shares = {}
+    for kind in ("smooth", "resid_restrict", "rhs", "coarse"):
+        pr = b.profile(kind, -1, nsteps)
+        shares[kind] = pr["avg_ms"] * pr["launches"] / nsteps
+    dom = max(shares, key=shares.get)
+    pr = b.profile(dom, 0, nsteps)
+    achieved = pr["alg_bytes"] / (pr["avg_ms"] * 1e-3) / 1e9 if pr["avg_ms"] > 0 else 0.0
+    return dom, shares, pr, achieved
+
+
+def cpu_baseline(name, seconds):
     from fluid2d_amd import workloads
     from oracle import oracle as orc
 
@@ -58,82 +149,153 @@ def cpu_baseline(name, steps_hint):
     o.set_psi_from_vorticity()
     d = o.diagnostics()
     dt = workloads.time_step(name, g, d["maxspeed"])
-    steps = steps_hint
-    if steps <= 0:
-        t0 = time.perf_counter()
-        o.run(1, dt)
-        one = time.perf_counter() - t0
-        steps = max(1, min(200, int(15.0 / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.run(1, dt)
+    one = time.perf_counter() - t0
+    steps = max(1, min(200, int(seconds / max(one, 1e-6))))
     t0 = time.perf_counter()
     o.run(steps, dt)
     el = time.perf_counter() - t0
     return {"value": g.nx * g.ny * steps / el, "unit": "cell-updates/s", "cores": 1,
             "kind": "port",
-            "sample": "%s, %d RK3_SSP steps of the C oracle (FP64, 1 thread, -O2, "
-                      "no FMA) after the initial solve, %.1f s" % (name, steps, el)}
+            "sample": "%s: %d RK3_SSP steps (after 1 untimed) of the C oracle, FP64, 1 thread, "
+                      "gcc -O2 -ffp-contract=off, %.1f s" % (name, steps, el)}
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(REPO, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("bytes_per_launch"), os.path.relpath(path, REPO)
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % a.gpus,
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(instances shard across GPUs with no exchange; not built here)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     import fluid2d_amd as F
-    from fluid2d_amd import workloads
 
     if F.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    name = a.workload
-    kw, _, _ = workloads.CONFIGS[name]
-    p = workloads.make_param(name)
-    g = F.Grid(p)
-    f = F.Fluid2d(p, g)
-    for k, v in workloads.initial_fields(name, g).items():
-        f.model.var.get(k)[...] = v
-    f.model.set_psi_from_vorticity()
-    d = f.model.diagnostics(f.model.var, 0.0)
-    dt = workloads.time_step(name, g, d["maxspeed"])
-    F.load().f2d_synchronize()
-    if a.warmup:
-        f.model.run(a.warmup, dt)
-    F.load().f2d_synchronize()
-    t0 = time.perf_counter()
-    nlog = f.model.run(a.steps, dt)
-    F.load().f2d_synchronize()
-    el = time.perf_counter() - t0
-    cells = g.nx * g.ny
-    value = cells * a.steps / el
-    n_f = float(np.mean(nlog)) if kw["modelname"] == "euler" else 0.0
-    wpc = words_per_cell_update(kw["modelname"], n_f)
-    step_bytes = wpc * 8.0 * cells
-    achieved = step_bytes / (el / a.steps) / 1e9
+    legs = set(a.legs.split(","))
+    name = HEADLINE
     out = {
         "metric": "cell-updates/s incl. MG Poisson, batched vs plain @ rel-L2<=1e-6; p99 latency",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": el / a.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded k^-3 turbulence init)",
-        "config": {"workload": name, "path": "plain-fp64", "nx": g.nx, "ny": g.ny,
-                   "model": kw["modelname"], "geometry": g.geometry, "scheme": "RK3_SSP",
-                   "advection": "UP5 parabolic", "mg": "2 V-cycles/stage + F-cycle solve",
-                   "mean_F_cycles": n_f},
-        "roofline": {"bound": "hbm", "kernel": "whole step (algorithmic words %d/cell)" % wpc,
-                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
-        "cpu_baseline": None,
+        "value": None, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded k^-3 turbulence, vortex pair, Rayleigh-Taylor inits)",
+        "config": {"workload": name, "path": "plain FP64, one instance, graph-replayed step",
+                   "nx": 1024, "ny": 1024, "model": "euler", "geometry": "closed",
+                   "scheme": "RK3_SSP", "advection": "UP5 parabolic",
+                   "mg": "2 V-cycles per inverted stage + F-cycle solve (maxite 4, tol 1e-11)"},
+        "roofline": None, "cpu_baseline": None,
     }
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(name, a.cpu_steps)
-    print(json.dumps(out))
+    plain_state = None
+    b = None
+    if legs & {"plain", "roofline", "precision"}:
+        log("plain FP64 %s: warmup %d, %d steps" % (name, a.warmup, a.steps))
+        b, g, dt = config_batch(name, "fp64")
+        b.step(a.warmup)
+        el = timed(b, a.steps)
+        cells = g.nx * g.ny
+        out["value"] = cells * a.steps / el
+        out["ms_per_step"] = el / a.steps * 1e3
+        nite = b.solve_info(0)[0]
+        out["config"]["F_cycles_last_step"] = nite
+        nf = float(nite)
+        wpc = 109 + 28 * nf
+        out["step_roofline"] = {
+            "words_per_cell_update": wpc,
+            "achieved": out["value"] * wpc * 8 / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": out["value"] * wpc * 8 / 1e9 / PEAK_HBM_GBS}
+        plain_state = b.state(0)
+        log("plain: %.3f ms/step, %.3e cell-updates/s" % (out["ms_per_step"], out["value"]))
+    if "roofline" in legs and b is not None:
+        dom, shares, pr, achieved = roofline(b)
+        kname = KERNEL_NAMES[dom] + "<double>"
+        traffic, src = load_traffic(KERNEL_NAMES[dom], name)
+        out["roofline"] = {
+            "bound": "hbm", "kernel": kname + " finest-level launches",
+            "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+            "alg_bytes_per_launch": pr["alg_bytes"], "avg_launch_ms": pr["avg_ms"],
+            "launches_timed": pr["launches"],
+            "step_share_ms": {k: round(v, 4) for k, v in shares.items()}}
+        if src:
+            out["roofline"]["traffic_source"] = src
+        log("roofline: %s %.1f GB/s (%.1f%%)" % (kname, achieved, 100 * achieved / PEAK_HBM_GBS))
+    del b
+    if "batched" in legs or "precision" in legs:
+        log("ensemble %d instances mixed, %d steps" % (a.ensemble, a.ensemble_steps))
+        eb, dts, ecells = ensemble_batch(a.ensemble, "mixed")
+        eb.step(2)
+        el = timed(eb, a.ensemble_steps - 2)
+        bval = ecells * (a.ensemble_steps - 2) / el
+        out["batched"] = {"value": bval, "unit": "cell-updates/s", "dtype": "f32+f64",
+                          "workload": "ensemble-%d-euler-64..512" % a.ensemble,
+                          "instances": a.ensemble, "cells": ecells,
+                          "steps": a.ensemble_steps, "ms_per_step": el / (a.ensemble_steps - 2) * 1e3,
+                          "speedup_vs_plain_fp64": (bval / out["value"]) if out["value"] else None}
+        log("batched mixed: %.3e cell-updates/s" % bval)
+        if "precision" in legs:
+            mixed_states = [eb.state(i) for i in range(eb.n)]
+            del eb
+            log("ensemble FP64, %d steps" % a.ensemble_steps)
+            fb, _, _ = ensemble_batch(a.ensemble, "fp64", dts)
+            fb.step(2)
+            el64 = timed(fb, a.ensemble_steps - 2)
+            out["batched"]["fp64_value"] = ecells * (a.ensemble_steps - 2) / el64
+            err = max(max_rel(mixed_states[i], fb.state(i)) for i in range(fb.n))
+            del fb
+            out.setdefault("precision", {})["ensemble_max_rel_l2"] = err
+            out["precision"]["ensemble_steps"] = a.ensemble_steps
+        else:
+            del eb
+    if "precision" in legs and plain_state is not None:
+        log("mixed %s, %d steps" % (name, a.warmup + a.steps))
+        _, _, dt = config_batch(name, "fp64")
+        mb, g, _ = config_batch(name, "mixed", dt)
+        el = timed(mb, a.warmup + a.steps)
+        out.setdefault("precision", {})
+        out["precision"]["euler_closed_1024_rel_l2"] = max_rel(mb.state(0), plain_state)
+        out["precision"]["euler_closed_1024_steps"] = a.warmup + a.steps
+        out["precision"]["euler_closed_1024_mixed_value"] = g.nx * g.ny * (a.warmup + a.steps) / el
+        del mb
+        log("Boussinesq %s fp64 + mixed, %d steps" % (BOUSSINESQ, a.bous_steps))
+        bb, g, dt = config_batch(BOUSSINESQ, "fp64")
+        el64 = timed(bb, a.bous_steps)
+        s64 = bb.state(0)
+        del bb
+        bm, _, _ = config_batch(BOUSSINESQ, "mixed", dt)
+        elm = timed(bm, a.bous_steps)
+        out["precision"]["boussinesq_rt_rel_l2"] = max_rel(bm.state(0), s64)
+        out["precision"]["boussinesq_rt_steps"] = a.bous_steps
+        out["precision"]["boussinesq_rt_fp64_value"] = g.nx * g.ny * a.bous_steps / el64
+        out["precision"]["boussinesq_rt_mixed_value"] = g.nx * g.ny * a.bous_steps / elm
+        out["precision"]["reference"] = "FP64 engine (bit-identical to the oracle)"
+        del bm
+    if "serving" in legs:
+        from fluid2d_amd import serving
+        log("serving %d requests" % a.serve_requests)
+        reqs = serving.stream_requests(a.serve_requests)
+        srv = serving.Server(hbm_budget=64 << 30, precision="mixed", chunk=20)
+        summary, _ = srv.run(reqs)
+        summary["arrival_rate_per_s"] = 50.0
+        out["serving"] = summary
+        log("serving: p50 %.3f s p99 %.3f s" % (summary["p50_s"], summary["p99_s"]))
+    if "cpu" in legs:
+        log("cpu baseline (~%.0f s)" % a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(name, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

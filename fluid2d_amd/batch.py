@@ -26,8 +26,10 @@ _d = C.c_double
 
 _SIG = {
     "f2d_batch_last_error": ([], C.c_char_p),
-    "f2d_batch_create": ([_i, C.POINTER(_lib.f2d_model_params), C.POINTER(C.c_void_p), _i, _i,
+    "f2d_batch_create": ([_i, C.POINTER(_lib.f2d_model_params), C.POINTER(C.c_void_p), _i, _p,
                           C.POINTER(C.c_void_p)], _i),
+    "f2d_batch_retune": ([_p, _p], _i),
+    "f2d_batch_tuning_info": ([_p, _p], _i),
     "f2d_batch_destroy": ([_p], _i),
     "f2d_batch_info": ([_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_longlong)], _i),
     "f2d_batch_set_field": ([_p, _i, _i, _p], _i),
@@ -35,11 +37,16 @@ _SIG = {
     "f2d_batch_state_ptr": ([_p, _i, C.POINTER(C.c_void_p)], _i),
     "f2d_batch_set_dt": ([_p, _i, _d], _i),
     "f2d_batch_set_adaptive": ([_p, _i, _d, _d], _i),
+    "f2d_batch_set_steps": ([_p, _i, _i], _i),
+    "f2d_batch_progress": ([_p, _i, C.POINTER(_i), C.POINTER(C.c_longlong)], _i),
     "f2d_batch_use_graph": ([_p, _i], _i),
     "f2d_batch_set_psi_from_vorticity": ([_p], _i),
     "f2d_batch_step": ([_p, _i], _i),
     "f2d_batch_step_async": ([_p, _i], _i),
     "f2d_batch_synchronize": ([_p], _i),
+    "f2d_batch_query": ([_p, C.POINTER(_i)], _i),
+    "f2d_batch_profile": ([_p, _i, _i, _i, C.POINTER(_d), C.POINTER(C.c_longlong), C.POINTER(_d),
+                           C.POINTER(_d)], _i),
     "f2d_batch_diagnostics": ([_p, _i, C.POINTER(_d)], _i),
     "f2d_batch_solve_info": ([_p, _i, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i)], _i),
 }
@@ -67,6 +74,24 @@ def _call(name, *args):
         raise _lib.F2DError(rc, msg)
 
 
+class Tuning(C.Structure):
+    """f2d_batch_tuning: coarse-kernel LDS budget (bytes, 0 = device
+    maximum) and the minimum workgroup count of a fine phase (0 = always
+    16-row tiles)."""
+    _fields_ = [("coarse_lds_budget", C.c_int), ("min_tiles", C.c_int)]
+
+    def as_dict(self):
+        return {"coarse_lds_budget": self.coarse_lds_budget, "min_tiles": self.min_tiles}
+
+
+def _tuning(t):
+    if t is None:
+        return None
+    if isinstance(t, dict):
+        t = Tuning(int(t.get("coarse_lds_budget", 0)), int(t.get("min_tiles", 0)))
+    return C.byref(t)
+
+
 def params_struct(param, grid):
     p = _lib.f2d_model_params()
     p.model = {"euler": 0, "boussinesq": 1}[param.modelname]
@@ -88,7 +113,7 @@ def params_struct(param, grid):
 
 
 class Batch:
-    def __init__(self, specs, precision="fp64", coarse_lds_budget=0, use_graph=True):
+    def __init__(self, specs, precision="fp64", tuning=None, use_graph=True):
         self.params, self.grids = [], []
         for s in specs:
             if isinstance(s, tuple):
@@ -106,7 +131,7 @@ class Batch:
         h = C.c_void_p()
         self.precision = precision
         prec = {"fp64": 0, "mixed": 1}[precision]
-        _call("f2d_batch_create", n, arr, mp, prec, int(coarse_lds_budget), C.byref(h))
+        _call("f2d_batch_create", n, arr, mp, prec, _tuning(tuning), C.byref(h))
         self._h = h.value
         self.n = n
         _call("f2d_batch_use_graph", self._h, 1 if use_graph else 0)
@@ -122,6 +147,15 @@ class Batch:
 
     def __len__(self):
         return self.n
+
+    def retune(self, tuning):
+        """Rebuild the launch program for a new tuning (dict or Tuning)."""
+        _call("f2d_batch_retune", self._h, _tuning(tuning))
+
+    def tuning(self):
+        t = Tuning()
+        _call("f2d_batch_tuning_info", self._h, C.byref(t))
+        return t.as_dict()
 
     def info(self, i):
         nl, nf, nlaunch = C.c_int(), C.c_int(), C.c_longlong()
@@ -145,6 +179,15 @@ class Batch:
     def set_dt(self, dt, i=-1):
         _call("f2d_batch_set_dt", self._h, int(i), float(dt))
 
+    def set_steps(self, nsteps, i=-1):
+        """Step budget of instance i (all if i < 0; nsteps < 0: unlimited)."""
+        _call("f2d_batch_set_steps", self._h, int(i), int(nsteps))
+
+    def progress(self, i):
+        left, done = C.c_int(), C.c_longlong()
+        _call("f2d_batch_progress", self._h, i, C.byref(left), C.byref(done))
+        return left.value, done.value
+
     def set_adaptive(self, on=True, cfl=0.5, dtmax=5.0):
         _call("f2d_batch_set_adaptive", self._h, 1 if on else 0, float(cfl), float(dtmax))
 
@@ -159,6 +202,23 @@ class Batch:
 
     def synchronize(self):
         _call("f2d_batch_synchronize", self._h)
+
+    def done(self):
+        """True when all enqueued work on this batch's stream has finished."""
+        d = C.c_int()
+        _call("f2d_batch_query", self._h, C.byref(d))
+        return bool(d.value)
+
+    KINDS = {"smooth": 0, "resid_restrict": 1, "rhs": 2, "coarse": 3}
+
+    def profile(self, kind, level=-1, nsteps=2):
+        """Time every launch of ``kind`` at ``level`` over ``nsteps`` eager
+        steps with HIP events; returns mean ms and bytes per launch."""
+        ms, nl, ab, ib = C.c_double(), C.c_longlong(), C.c_double(), C.c_double()
+        _call("f2d_batch_profile", self._h, self.KINDS[kind], int(level), int(nsteps),
+              C.byref(ms), C.byref(nl), C.byref(ab), C.byref(ib))
+        return {"avg_ms": ms.value, "launches": nl.value, "alg_bytes": ab.value,
+                "impl_bytes": ib.value}
 
     def raw_diagnostics(self, i):
         out = (C.c_double * 8)()

@@ -56,7 +56,9 @@ struct DynD {
   int nite, ndiv, active, diverged;
   double psim;
   double diag[NDIAG];
-  double pad[2];
+  int run;         // this step runs for the instance (set by the step gate)
+  int steps_left;  // remaining steps (< 0: unlimited)
+  long long steps_done;
 };
 
 // one tile of a 2D phase

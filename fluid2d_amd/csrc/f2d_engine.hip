@@ -40,7 +40,19 @@ namespace eng {
 // device helpers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool skip(const DynD* dyn, int inst, int pred) {
-  return pred && !dyn[inst].active;
+  return !dyn[inst].run || (pred && !dyn[inst].active);
+}
+__device__ __forceinline__ bool idle(const DynD* dyn, int inst) { return !dyn[inst].run; }
+
+// Step gate (first launch of a step): an instance runs this step while its
+// step budget lasts; every kernel of the step skips instances that do not.
+__global__ void k_gate(DynD* dyn, int ninst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  DynD& d = dyn[i];
+  d.run = d.steps_left != 0;
+  if (d.steps_left > 0) d.steps_left -= 1;
+  if (d.run) d.steps_done += 1;
 }
 
 // stage state (core/timescheme.py:167-180, :152-164), NumPy order
@@ -82,6 +94,7 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   __shared__ T vs[(TY + 1) * TX];
   __shared__ int8_t ms[QH * QW];
   const Task tk = tasks[blockIdx.x];
+  if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
   const size_t F = (size_t)m * n;
@@ -218,13 +231,13 @@ __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
 // MODE 0: x = xin; 1: x = 0; 2: x = xin + I(xc); 3: x = I(xc).
 // FINAL: each stored copy multiplied by mskp (psi * mskp).
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int TYV>
 __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
                                                 const InstD* __restrict__ I, T* Q,
                                                 const int8_t* __restrict__ B,
                                                 const DynD* __restrict__ dyn, int pred) {
-  __shared__ T xs[TY + 4][TX + 4];
-  __shared__ T j1[TY + 2][TX + 2];
+  __shared__ T xs[TYV + 4][TX + 4];
+  __shared__ T j1[TYV + 2][TX + 2];
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
   const int MODE = tk.mode;
@@ -240,7 +253,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
   const int y0 = tk.y0, x0 = tk.x0;
   // x on rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1], clipped to [1, m-2]
-  for (int q = tid; q < (TY + 4) * (TX + 4); q += nt) {
+  for (int q = tid; q < (TYV + 4) * (TX + 4); q += nt) {
     const int rr = q / (TX + 4), cc = q % (TX + 4);
     const int j = y0 - 2 + rr, i = x0 - 2 + cc;
     T v = T(0);
@@ -263,7 +276,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   const uint8_t* sb = (const uint8_t*)(B + in.sb0);
   auto sweeps = [&](auto Aat) {
     // first sweep on rows [y0-1, y0+TY] x cols [x0-1, x0+TX], clipped to [.., m-NH]
-    for (int q = tid; q < (TY + 2) * (TX + 2); q += nt) {
+    for (int q = tid; q < (TYV + 2) * (TX + 2); q += nt) {
       const int rr = q / (TX + 2), cc = q % (TX + 2);
       const int j = y0 - 1 + rr, i = x0 - 1 + cc;
       T v = T(0);
@@ -278,7 +291,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       j1[rr][cc] = v;
     }
     __syncthreads();
-    for (int rr = threadIdx.y; rr < TY; rr += blockDim.y) {
+    for (int rr = threadIdx.y; rr < TYV; rr += blockDim.y) {
       const int J = y0 + rr, Ic = x0 + threadIdx.x;
       if (J >= m - NH || Ic >= n - NH) continue;
       const int cc = threadIdx.x;
@@ -305,14 +318,14 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
 //        2 restrict b itself instead of the residual (F-cycle descent),
 //        4 norm partial of r, 8 norm partial of b (owned interior, msk != 0)
 // ---------------------------------------------------------------------------
-template <typename T, typename R>
+template <typename T, typename R, int TY2V>
 __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__ tasks,
                                                         const InstD* __restrict__ I,
                                                         const R* RP, T* Q,
                                                         const int8_t* __restrict__ B,
                                                         const DynD* __restrict__ dyn, int pred,
                                                         int flags, double* partial) {
-  constexpr int RY = 2 * TY2 + 1, RX = 2 * TX2 + 1;
+  constexpr int RY = 2 * TY2V + 1, RX = 2 * TX2 + 1;
   __shared__ R rs[RY][RX];
   __shared__ double red[2][8];
   const Task tk = tasks[blockIdx.x];
@@ -356,7 +369,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       v = residual_x<R>(msk1[c], A, PL, c, n1, b[c], [&](int dd) { return x[c + dd]; });
     rs[rr][cc] = v;
     // owned fine cells: local rows/cols [0, 2*T-1] that are interior
-    if (rr < 2 * TY2 && cc < 2 * TX2 && jf == j && if_ == i && jf >= NH && if_ >= NH &&
+    if (rr < 2 * TY2V && cc < 2 * TX2 && jf == j && if_ == i && jf >= NH && if_ >= NH &&
         jf < m1 - NH && if_ < n1 - NH && (tk.y0 + rr / 2) < m2 - NH && (tk.x0 + cc / 2) < n2 - NH) {
       if (flags & 1) store_img_mask<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
       if (msk1[c] != 0) {
@@ -387,7 +400,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     partial[2 * blockIdx.x + 1] = c;
   }
   T* b2 = Q + Lc.b;
-  for (int q = tid; q < TY2 * TX2; q += nt) {
+  for (int q = tid; q < TY2V * TX2; q += nt) {
     const int ry = q / TX2, rx = q % TX2;
     const int J2 = tk.y0 + ry, I2 = tk.x0 + rx;
     if (J2 >= m2 - NH || I2 >= n2 - NH) continue;
@@ -616,9 +629,11 @@ __global__ void __launch_bounds__(256) k_coarse(const Task* __restrict__ tasks,
 template <typename T>
 __global__ void __launch_bounds__(256) k_ortho(const Task* __restrict__ tasks,
                                                const InstD* __restrict__ I, T* P,
-                                               const int8_t* __restrict__ B) {
+                                               const int8_t* __restrict__ B,
+                                               const DynD* __restrict__ dyn) {
   // P: the pool holding the state-shaped array (FP64 state or T tendency)
   const Task tk = tasks[blockIdx.x];
+  if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
   const size_t F = (size_t)m * n;
@@ -646,6 +661,7 @@ __global__ void __launch_bounds__(256) k_combine(const Task* __restrict__ tasks,
                                                  const InstD* __restrict__ I, double* P, T* Q,
                                                  const DynD* __restrict__ dyn) {
   const Task tk = tasks[blockIdx.x];
+  if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const size_t F = (size_t)in.m * in.n;
   const double dt = dyn[tk.inst].dt;
@@ -684,6 +700,7 @@ __global__ void __launch_bounds__(256) k_finalize_psi(const Task* __restrict__ t
                                                       const int8_t* __restrict__ B,
                                                       const DynD* __restrict__ dyn) {
   const Task tk = tasks[blockIdx.x];
+  if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const size_t F = (size_t)in.m * in.n;
   double* psi = P + in.state + F;
@@ -764,8 +781,10 @@ __device__ double np_pairwise_flat(const T* x, int n, int N, size_t start, size_
 // lengths run NumPy's recursion on lane 0.
 __global__ void __launch_bounds__(256) k_npsum(const Task* __restrict__ tasks,
                                                const InstD* __restrict__ I, double* P,
-                                               double* partial, const int* __restrict__ slot) {
+                                               double* partial, const int* __restrict__ slot,
+                                               const DynD* __restrict__ dyn) {
   const Task tk = tasks[blockIdx.x];
+  if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   if (!in.perio) return;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -803,10 +822,12 @@ __global__ void __launch_bounds__(256) k_npsum(const Task* __restrict__ tasks,
 // diagnostics partials: {maxu, ke, sum w, sum w^2, d4, d5, d6, d7}
 __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
                                               const InstD* __restrict__ I, const double* P,
-                                              const int8_t* __restrict__ B, double* partial) {
+                                              const int8_t* __restrict__ B, double* partial,
+                                              const DynD* __restrict__ dyn) {
   typedef double T;
   __shared__ double red[8][4];
   const Task tk = tasks[blockIdx.x];
+  if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
   const size_t F = (size_t)m * n;
@@ -880,7 +901,7 @@ __global__ void __launch_bounds__(256) k_fold(const FoldSpan* __restrict__ spans
   const int inst = blockIdx.x, tid = threadIdx.x;
   DynD& d = dyn[inst];
   const FoldSpan sp = spans[inst];
-  if (sp.count == 0) return;
+  if (sp.count == 0 || !d.run) return;
   if (what == 3) {
     if (tid != 0 || !I[inst].perio) return;
     double acc = 0.;
@@ -999,7 +1020,8 @@ enum Kind {
   L_FINPSI,
   L_NPSUM,
   L_DIAG,
-  L_FOLD
+  L_FOLD,
+  L_GATE
 };
 
 struct Launch {
@@ -1011,6 +1033,7 @@ struct Launch {
   int span_off = 0;          // fold spans (ninst entries)
   int slot_off = 0;          // npsum slots
   size_t shmem = 0;          // k_coarse dynamic LDS
+  int ty = TY;               // fine tile rows of k_smooth (k_resid_restrict: ty / 2 coarse rows)
 };
 
 struct HostInst {
@@ -1051,6 +1074,8 @@ struct Batch {
   int maxite = 4;
   double tol = 1e-11;
   size_t max_shmem = 0;
+  int min_tiles = 0;           // fine-phase tile choice (Builder::pick_ty)
+  size_t coarse_budget = 0;    // LDS bytes for k_coarse (0: device maximum)
 };
 
 // ---- task helpers ----------------------------------------------------------
@@ -1064,9 +1089,10 @@ static Task mk_task(int inst, int lev, int y0, int x0) {
   return t;
 }
 
-// tiles of TY x TX over rows [r0, r1) x cols [c0, c1)
-static void tiles(std::vector<Task>& out, const Task& proto, int r0, int r1, int c0, int c1) {
-  for (int y = r0; y < r1; y += TY)
+// tiles of ty x TX over rows [r0, r1) x cols [c0, c1)
+static void tiles(std::vector<Task>& out, const Task& proto, int r0, int r1, int c0, int c1,
+                  int ty = TY) {
+  for (int y = r0; y < r1; y += ty)
     for (int x = c0; x < c1; x += TX) {
       Task t = proto;
       t.y0 = y;
@@ -1112,8 +1138,28 @@ struct Builder {
 
   const LevD& lev(int i, int l) { return b.H[i].d.L[l]; }
 
+  // Fine-phase tile height: the tallest of 16, 8, 4 rows that still gives
+  // at least min_tiles workgroups over the cells of the phase (all
+  // instances at bottom-aligned level dd); min_tiles <= 0: always 16.
+  int pick_ty_cells(double cells) const {
+    if (b.min_tiles <= 0) return TY;
+    for (int ty = TY; ty > 4; ty /= 2)
+      if (cells / ((double)TX * ty) >= (double)b.min_tiles) return ty;
+    return 4;
+  }
+  int pick_ty(const std::vector<int>& who, const std::vector<int>& start, int dd) const {
+    double cells = 0.;
+    for (int i : who) {
+      const int D = b.H[i].d.nfine - 1 - start[i];
+      if (dd > D) continue;
+      const LevD& L = b.H[i].d.L[b.H[i].d.nfine - 1 - dd];
+      cells += (double)(L.m - 2 * NH) * (L.n - 2 * NH);
+    }
+    return pick_ty_cells(cells);
+  }
+
   // smoother on level l of instance i: input mode, output the other buffer
-  Task smooth_task(int i, int l, int mode, int final_mask, std::vector<Task>& out) {
+  Task smooth_task(int i, int l, int mode, int final_mask, std::vector<Task>& out, int ty) {
     HostInst& h = b.H[i];
     Task t = mk_task(i, l, 0, 0);
     t.mode = mode;
@@ -1122,19 +1168,19 @@ struct Builder {
     t.xout = h.buf[l][h.cur[l] ^ 1];
     if (mode == 2 || mode == 3) t.xc = h.buf[l + 1][h.cur[l + 1]];
     const LevD& L = lev(i, l);
-    tiles(out, t, NH, L.m - NH, NH, L.n - NH);
+    tiles(out, t, NH, L.m - NH, NH, L.n - NH, ty);
     h.cur[l] ^= 1;
     return t;
   }
   // residual (or b) of level l restricted onto l+1
   void rr_tasks(int i, int l, long long xin, long long rhs, long long rstore,
-                std::vector<Task>& out) {
+                std::vector<Task>& out, int ty2 = TY2) {
     const LevD& C = lev(i, l + 1);
     Task t = mk_task(i, l, 0, 0);
     t.xin = xin;
     t.xout = rhs;
     t.xc = rstore;
-    for (int y = NH; y < C.m - NH; y += TY2)
+    for (int y = NH; y < C.m - NH; y += ty2)
       for (int x = NH; x < C.n - NH; x += TX2) {
         Task u = t;
         u.y0 = y;
@@ -1152,30 +1198,40 @@ struct Builder {
     int Dmax = -1;
     for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1 - start[i]);
     for (int dd = Dmax; dd >= 0; dd--) {
+      const int ty = pick_ty(who, start, dd);
       std::vector<Task> ts, rs;
       for (int i : who) {
         const int D = b.H[i].d.nfine - 1 - start[i];
         if (dd > D) continue;
         const int l = b.H[i].d.nfine - 1 - dd;
-        smooth_task(i, l, dd == D ? m0 : 1, 0, ts);
+        smooth_task(i, l, dd == D ? m0 : 1, 0, ts, ty);
         HostInst& h = b.H[i];
-        rr_tasks(i, l, h.buf[l][h.cur[l]], -1, -1, rs);
+        rr_tasks(i, l, h.buf[l][h.cur[l]], -1, -1, rs, ty / 2);
       }
       if (!ts.empty()) {
-        add(L_SMOOTH, ts).pred = pred;
-        add(L_RR, rs).pred = pred;
+        Launch& Ls = add(L_SMOOTH, ts);
+        Ls.pred = pred;
+        Ls.ty = ty;
+        Launch& Lr = add(L_RR, rs);
+        Lr.pred = pred;
+        Lr.ty = ty;
       }
     }
     coarse(who, 0, pred);
     for (int dd = 0; dd <= Dmax; dd++) {
+      const int ty = pick_ty(who, start, dd);
       std::vector<Task> ts;
       for (int i : who) {
         const int D = b.H[i].d.nfine - 1 - start[i];
         if (dd > D) continue;
         const int l = b.H[i].d.nfine - 1 - dd;
-        smooth_task(i, l, 2, (final_top && dd == D) ? 1 : 0, ts);
+        smooth_task(i, l, 2, (final_top && dd == D) ? 1 : 0, ts, ty);
       }
-      if (!ts.empty()) add(L_SMOOTH, ts).pred = pred;
+      if (!ts.empty()) {
+        Launch& Ls = add(L_SMOOTH, ts);
+        Ls.pred = pred;
+        Ls.ty = ty;
+      }
     }
   }
 
@@ -1296,16 +1352,23 @@ struct Builder {
       int Dmax = 0;
       for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1);
       for (int dd = Dmax - 1; dd >= 0; dd--) {
+        double cells = 0.;
+        for (int i : who) {
+          const int l = b.H[i].d.nfine - 1 - dd;
+          if (l >= 1) cells += (double)(b.H[i].d.L[l].m - 2 * NH) * (b.H[i].d.L[l].n - 2 * NH);
+        }
+        const int ty = pick_ty_cells(cells);
         std::vector<Task> rs;
         for (int i : who) {
           const int l = b.H[i].d.nfine - 1 - dd;
           if (l < 1) continue;
-          rr_tasks(i, l, 0, -1, -1, rs);
+          rr_tasks(i, l, 0, -1, -1, rs, ty / 2);
         }
         if (!rs.empty()) {
           Launch& L = add(L_RR, rs);
           L.flags = 2;
           L.pred = 1;
+          L.ty = ty;
         }
       }
       coarse(who, 1, 1);
@@ -1415,15 +1478,50 @@ struct Builder {
 };
 
 // ---- execution ------------------------------------------------------------
+// Optional timing of one launch kind (eager runs only): events recorded
+// on the batch stream around every matching launch.
+struct ProfSel {
+  int kind = -1, level = -1;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::vector<const Launch*> which;
+};
+
+template <typename T, typename R>
+static void launch_rr(int ty, unsigned g, hipStream_t s, const Task* TK, const InstD* dI,
+                      const R* RP, T* Q, const int8_t* B, const DynD* dD, int pred, int flags,
+                      double* partial) {
+  const dim3 B2(64, 4);
+  if (ty == 4)
+    hipLaunchKernelGGL((k_resid_restrict<T, R, 2>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD, pred,
+                       flags, partial);
+  else if (ty == 8)
+    hipLaunchKernelGGL((k_resid_restrict<T, R, 4>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD, pred,
+                       flags, partial);
+  else
+    hipLaunchKernelGGL((k_resid_restrict<T, R, TY2>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD,
+                       pred, flags, partial);
+}
+
 template <typename T>
-static int run_prog_t(Batch& b, const std::vector<Launch>& prog) {
+static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof = nullptr) {
   const dim3 B2(64, 4), B1(256);
   T* Q = (T*)b.Q;
   double* P = b.P;
   for (const Launch& L : prog) {
     const Task* TK = b.dT + L.toff;
     const unsigned g = (unsigned)L.ntask;
-    if (L.kind != L_FOLD && g == 0) continue;
+    if (L.kind != L_FOLD && L.kind != L_GATE && g == 0) continue;
+    bool timed = false;
+    if (prof && L.kind == prof->kind &&
+        (prof->level < 0 || (L.ntask > 0 && b.tasks[L.toff].lev == prof->level))) {
+      hipEvent_t e0, e1;
+      ECHK(hipEventCreate(&e0));
+      ECHK(hipEventCreate(&e1));
+      ECHK(hipEventRecord(e0, b.s));
+      prof->ev.push_back({e0, e1});
+      prof->which.push_back(&L);
+      timed = true;
+    }
     switch (L.kind) {
       case L_RHS:
         hipLaunchKernelGGL(k_rhs<T>, dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
@@ -1436,15 +1534,20 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog) {
           hipLaunchKernelGGL(k_c2c<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.dD, L.pred);
         break;
       case L_SMOOTH:
-        hipLaunchKernelGGL(k_smooth<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        if (L.ty == 4)
+          hipLaunchKernelGGL((k_smooth<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        else if (L.ty == 8)
+          hipLaunchKernelGGL((k_smooth<T, 8>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        else
+          hipLaunchKernelGGL((k_smooth<T, TY>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                             L.pred);
         break;
       case L_RR:
         if (L.flags & 32)
-          hipLaunchKernelGGL((k_resid_restrict<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, Q,
-                             b.B, b.dD, L.pred, L.flags, b.partial);
+          launch_rr<T, double>(L.ty, g, b.s, TK, b.dI, P, Q, b.B, b.dD, L.pred, L.flags, b.partial);
         else
-          hipLaunchKernelGGL((k_resid_restrict<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.B,
-                             b.dD, L.pred, L.flags, b.partial);
+          launch_rr<T, T>(L.ty, g, b.s, TK, b.dI, (const T*)Q, Q, b.B, b.dD, L.pred, L.flags,
+                          b.partial);
         break;
       case L_COARSE:
         hipLaunchKernelGGL(k_coarse<T>, dim3(g), dim3(256), L.shmem, b.s, TK, b.dI, Q, b.B,
@@ -1452,9 +1555,9 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog) {
         break;
       case L_ORTHO:
         if (L.flags & 1)
-          hipLaunchKernelGGL(k_ortho<double>, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B);
+          hipLaunchKernelGGL(k_ortho<double>, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B, b.dD);
         else
-          hipLaunchKernelGGL(k_ortho<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B);
+          hipLaunchKernelGGL(k_ortho<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD);
         break;
       case L_COMBINE:
         hipLaunchKernelGGL(k_combine<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD);
@@ -1467,24 +1570,68 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog) {
         break;
       case L_NPSUM:
         hipLaunchKernelGGL(k_npsum, dim3(g), B1, 0, b.s, TK, b.dI, P, b.partial,
-                           b.dSlot + L.slot_off);
+                           b.dSlot + L.slot_off, b.dD);
         break;
       case L_DIAG:
-        hipLaunchKernelGGL(k_diag, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B, b.partial);
+        hipLaunchKernelGGL(k_diag, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B, b.partial, b.dD);
+        break;
+      case L_GATE:
+        hipLaunchKernelGGL(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
       case L_FOLD:
         hipLaunchKernelGGL(k_fold, dim3(b.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
                            b.partial, L.mode, b.maxite, b.tol, L.pred, b.adapt, b.cfl, b.dtmax);
         break;
     }
+    if (timed) ECHK(hipEventRecord(prof->ev.back().second, b.s));
   }
   ECHK(hipGetLastError());
   return F2D_OK;
 }
 
-static int run_prog(Batch& b, const std::vector<Launch>& prog) {
-  return b.prec == 0 ? run_prog_t<double>(b, prog) : run_prog_t<float>(b, prog);
+static int run_prog(Batch& b, const std::vector<Launch>& prog, ProfSel* prof = nullptr) {
+  return b.prec == 0 ? run_prog_t<double>(b, prog, prof) : run_prog_t<float>(b, prog, prof);
 }
+
+// Bytes a launch moves: algorithmic (each operand read once, results
+// written once, operator matrix-free, masks ignored) and as implemented
+// (operator planes or stencil bytes and masks included).
+static void launch_bytes(const Batch& b, const Launch& L, double* alg, double* impl) {
+  const double tz = b.prec == 0 ? 8. : 4.;
+  double a = 0., im = 0.;
+  for (int k = 0; k < L.ntask; k++) {
+    const Task& t = b.tasks[L.toff + k];
+    const InstD& d = b.H[t.inst].d;
+    const LevD& lv = d.L[t.lev];
+    if (L.kind == L_SMOOTH) {
+      const int cj = std::min(L.ty, lv.m - NH - t.y0), ci = std::min(TX, lv.n - NH - t.x0);
+      const double cells = (double)std::max(cj, 0) * std::max(ci, 0);
+      const bool rd_x = t.mode == 0 || t.mode == 2;
+      const double wrd = (rd_x ? 1. : 0.) + 1.;  // x (if read), b
+      a += cells * (wrd + 1.) * tz;               // + x written
+      const bool fly = t.lev == 0 && d.a0fly;
+      im += cells * ((wrd + 1.) * tz + 1. + (fly ? 1. : 5. * tz) + (t.mode >= 2 ? 0.25 * tz : 0.));
+    } else if (L.kind == L_RR) {
+      const LevD& lc = d.L[t.lev + 1];
+      const int cj = std::min(L.ty / 2, lc.m - NH - t.y0), ci = std::min(TX2, lc.n - NH - t.x0);
+      const double cc = (double)std::max(cj, 0) * std::max(ci, 0);
+      const double fine = 4. * cc;
+      const double rz = (L.flags & 32) ? 8. : tz;
+      const double rd = (L.flags & 2) ? 1. : 2.;  // b (and x)
+      a += fine * rd * rz + cc * tz + ((L.flags & 1) ? fine * tz : 0.);
+      const bool fly = t.lev == 0 && d.a0fly;
+      im += fine * (rd * rz + 1. + ((L.flags & 2) ? 0. : (fly ? 1. : 5. * rz))) + cc * (tz + 1.) +
+            ((L.flags & 1) ? fine * tz : 0.);
+    }
+  }
+  *alg = a;
+  *impl = im;
+}
+
+}  // namespace eng
+}  // namespace f2d
+namespace f2d {
+namespace eng {
 
 static void batch_free(Batch* b) {
   if (!b) return;
@@ -1517,6 +1664,11 @@ static int build_programs(Batch& b) {
   bd.solve_full(bd.all());
   bd.diag();
   bd.prog = &b.prog_step;
+  {
+    Launch g;
+    g.kind = L_GATE;
+    b.prog_step.push_back(g);
+  }
   bd.rk_step();
   size_t np = 0;
   for (auto* pr : {&b.prog_init, &b.prog_step})
@@ -1545,6 +1697,56 @@ static size_t coarse_bytes(const InstD& d, int k0) {
   return sz;
 }
 
+// LDS bytes the coarse kernel may use: the requested budget (0: all), at
+// most the device's per-workgroup maximum less the static level table
+static size_t lds_budget(int requested) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int maxsh = 65536;
+  (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  const size_t avail = (size_t)maxsh > 4096 ? (size_t)maxsh - 4096 : (size_t)maxsh;
+  return requested > 0 ? std::min((size_t)requested, avail) : avail;
+}
+
+// first level of the coarse kernel: the largest set of coarse levels whose
+// operands fit the budget (at least the coarsest, at most all but level 0)
+static int first_coarse_level(const InstD& d, size_t budget, int* k0) {
+  int k = d.nlev - 1;
+  while (k > 1 && coarse_bytes(d, k - 1) <= budget) k--;
+  if (coarse_bytes(d, k) > budget)
+    return efail(F2D_E_ARG, "coarse levels need %zu bytes of LDS, budget %zu",
+                 coarse_bytes(d, k), budget);
+  *k0 = k;
+  return F2D_OK;
+}
+
+// apply a tuning: coarse split per instance, tile policy, programs rebuilt
+static int apply_tuning(Batch& b, const f2d_batch_tuning* t) {
+  const size_t budget = lds_budget(t ? t->coarse_lds_budget : 0);
+  std::vector<int> k0(b.ninst);
+  for (int i = 0; i < b.ninst; i++) ETRY(first_coarse_level(b.H[i].d, budget, &k0[i]));
+  size_t maxlds = 0;
+  for (int i = 0; i < b.ninst; i++) {
+    b.H[i].d.nfine = k0[i];
+    b.H[i].coarse_lds = coarse_bytes(b.H[i].d, k0[i]);
+    maxlds = std::max(maxlds, b.H[i].coarse_lds);
+  }
+  b.min_tiles = t ? std::max(0, t->min_tiles) : 0;
+  b.coarse_budget = budget;
+  if (maxlds > 65536) {
+    const void* fn = b.prec == 0 ? (const void*)k_coarse<double> : (const void*)k_coarse<float>;
+    ECHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
+  }
+  std::vector<InstD> di(b.ninst);
+  for (int i = 0; i < b.ninst; i++) di[i] = b.H[i].d;
+  ECHK(hipMemcpy(b.dI, di.data(), b.ninst * sizeof(InstD), hipMemcpyHostToDevice));
+  if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.gexec = nullptr;
+  b.graph = nullptr;
+  return build_programs(b);
+}
+
 }  // namespace eng
 }  // namespace f2d
 
@@ -1558,7 +1760,7 @@ extern "C" {
 const char* f2d_batch_last_error(void) { return e_err.c_str(); }
 
 int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* const* msks,
-                     int precision, int coarse_lds_budget, f2d_batch** out) {
+                     int precision, const f2d_batch_tuning* tuning, f2d_batch** out) {
   if (ninst < 1) return efail(F2D_E_ARG, "ninst must be >= 1");
   if (precision != 0 && precision != 1) return efail(F2D_E_ARG, "precision must be 0 or 1");
   f2d_batch* b = new f2d_batch();
@@ -1570,14 +1772,8 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
   };
   b->ninst = ninst;
   b->H.resize(ninst);
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  int maxsh = 65536;
-  (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   // static LDS of k_coarse (level table) comes on top of the dynamic part
-  const size_t avail = (size_t)maxsh > 4096 ? (size_t)maxsh - 4096 : (size_t)maxsh;
-  size_t budget = coarse_lds_budget > 0 ? (size_t)coarse_lds_budget : avail;
-  budget = std::min(budget, avail);
+  const size_t budget = lds_budget(tuning ? tuning->coarse_lds_budget : 0);
   const int tscheme0 = params[0].timestepping;
   // layout
   size_t nP = 0, nQ = 0, nB = 0;
@@ -1640,10 +1836,8 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
     }
     if (nl < 2) return bail(efail(F2D_E_ARG, "grid must have at least two levels"));
     d.nlev = nl;
-    int k0 = nl - 1;
-    while (k0 > 1 && coarse_bytes(d, k0 - 1) <= budget) k0--;
-    if (coarse_bytes(d, k0) > budget)
-      return bail(efail(F2D_E_ARG, "coarse levels exceed the LDS budget"));
+    int k0 = 0;
+    if (int rc = first_coarse_level(d, budget, &k0)) return bail(rc);
     d.nfine = k0;
     h.coarse_lds = coarse_bytes(d, k0);
     const size_t F = (size_t)d.m * d.n;
@@ -1792,17 +1986,26 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
   if (hipHostMalloc((void**)&b->hD, ninst * sizeof(DynD), hipHostMallocDefault) != hipSuccess)
     return bail(efail(F2D_E_NOMEM, "pinned dyn"));
   memset(b->hD, 0, ninst * sizeof(DynD));
-  size_t maxlds = 0;
-  for (auto& h : b->H) maxlds = std::max(maxlds, h.coarse_lds);
-  if (maxlds > 65536) {
-    const void* fn = b->prec == 0 ? (const void*)k_coarse<double> : (const void*)k_coarse<float>;
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds) !=
-        hipSuccess)
-      return bail(efail(F2D_E_HIP, "coarse LDS %zu bytes", maxlds));
+  for (int i = 0; i < ninst; i++) {
+    b->hD[i].run = 1;
+    b->hD[i].steps_left = -1;
   }
-  int rc = build_programs(*b);
+  if (hipMemcpy(b->dD, b->hD, ninst * sizeof(DynD), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(efail(F2D_E_HIP, "dyn init"));
+  int rc = apply_tuning(*b, tuning);
   if (rc) return bail(rc);
   *out = b;
+  return F2D_OK;
+}
+
+int f2d_batch_retune(f2d_batch* b, const f2d_batch_tuning* tuning) {
+  ECHK(hipStreamSynchronize(b->s));
+  return apply_tuning(*b, tuning);
+}
+
+int f2d_batch_tuning_info(f2d_batch* b, f2d_batch_tuning* out) {
+  out->coarse_lds_budget = (int)b->coarse_budget;
+  out->min_tiles = b->min_tiles;
   return F2D_OK;
 }
 
@@ -1864,6 +2067,24 @@ int f2d_batch_set_dt(f2d_batch* b, int inst, double dt) {
   return sync_dyn_to_device(b);
 }
 
+int f2d_batch_set_steps(f2d_batch* b, int inst, int nsteps) {
+  ETRY(sync_dyn_to_host(b));
+  for (int i = 0; i < b->ninst; i++)
+    if (inst < 0 || i == inst) {
+      b->hD[i].steps_left = nsteps;
+      b->hD[i].run = 1;
+    }
+  return sync_dyn_to_device(b);
+}
+
+int f2d_batch_progress(f2d_batch* b, int inst, int* steps_left, long long* steps_done) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  ETRY(sync_dyn_to_host(b));
+  *steps_left = b->hD[inst].steps_left;
+  *steps_done = b->hD[inst].steps_done;
+  return F2D_OK;
+}
+
 int f2d_batch_set_adaptive(f2d_batch* b, int on, double cfl, double dtmax) {
   b->adapt = on;
   b->cfl = cfl;
@@ -1888,37 +2109,86 @@ int f2d_batch_set_psi_from_vorticity(f2d_batch* b) {
   return F2D_OK;
 }
 
-int f2d_batch_step(f2d_batch* b, int nsteps) {
-  if (b->use_graph) {
-    if (!b->gexec) {
-      ECHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-      int rc = run_prog(*b, b->prog_step);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(b->s, &g);
-      if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return efail(F2D_E_HIP, "capture: %s", hipGetErrorString(e));
-      b->graph = g;
-      ECHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-    }
-    for (int k = 0; k < nsteps; k++) ECHK(hipGraphLaunch(b->gexec, b->s));
-  } else {
-    for (int k = 0; k < nsteps; k++) ETRY(run_prog(*b, b->prog_step));
+static int ensure_graph(f2d_batch* b) {
+  if (b->gexec) return F2D_OK;
+  ECHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+  int rc = run_prog(*b, b->prog_step);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
   }
-  ECHK(hipStreamSynchronize(b->s));
+  if (e != hipSuccess) return efail(F2D_E_HIP, "capture: %s", hipGetErrorString(e));
+  b->graph = g;
+  ECHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
   return F2D_OK;
 }
 
 int f2d_batch_step_async(f2d_batch* b, int nsteps) {
-  if (!b->use_graph || !b->gexec) return f2d_batch_step(b, nsteps);
-  for (int k = 0; k < nsteps; k++) ECHK(hipGraphLaunch(b->gexec, b->s));
+  if (b->use_graph) {
+    ETRY(ensure_graph(b));
+    for (int k = 0; k < nsteps; k++) ECHK(hipGraphLaunch(b->gexec, b->s));
+  } else {
+    for (int k = 0; k < nsteps; k++) ETRY(run_prog(*b, b->prog_step));
+  }
+  return F2D_OK;
+}
+
+int f2d_batch_step(f2d_batch* b, int nsteps) {
+  ETRY(f2d_batch_step_async(b, nsteps));
+  ECHK(hipStreamSynchronize(b->s));
   return F2D_OK;
 }
 
 int f2d_batch_synchronize(f2d_batch* b) {
   ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+
+int f2d_batch_query(f2d_batch* b, int* done) {
+  const hipError_t e = hipStreamQuery(b->s);
+  if (e == hipSuccess) {
+    *done = 1;
+    return F2D_OK;
+  }
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return F2D_OK;
+  }
+  return efail(F2D_E_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
+int f2d_batch_profile(f2d_batch* b, int kind, int level, int nsteps, double* avg_ms,
+                      long long* nlaunch, double* alg_bytes, double* impl_bytes) {
+  const int kinds[4] = {L_SMOOTH, L_RR, L_RHS, L_COARSE};
+  if (kind < 0 || kind > 3) return efail(F2D_E_ARG, "kind 0..3");
+  ProfSel ps;
+  ps.kind = kinds[kind];
+  ps.level = level;
+  ECHK(hipStreamSynchronize(b->s));
+  for (int k = 0; k < nsteps; k++) {
+    int rc = run_prog(*b, b->prog_step, &ps);
+    if (rc) return rc;
+  }
+  ECHK(hipStreamSynchronize(b->s));
+  double tot = 0., ab = 0., ib = 0.;
+  for (size_t q = 0; q < ps.ev.size(); q++) {
+    float ms = 0.f;
+    ECHK(hipEventElapsedTime(&ms, ps.ev[q].first, ps.ev[q].second));
+    tot += ms;
+    double a, im;
+    launch_bytes(*b, *ps.which[q], &a, &im);
+    ab += a;
+    ib += im;
+    (void)hipEventDestroy(ps.ev[q].first);
+    (void)hipEventDestroy(ps.ev[q].second);
+  }
+  const double cnt = ps.ev.empty() ? 1. : (double)ps.ev.size();
+  *nlaunch = (long long)ps.ev.size();
+  *avg_ms = tot / cnt;
+  *alg_bytes = ab / cnt;
+  *impl_bytes = ib / cnt;
   return F2D_OK;
 }
 

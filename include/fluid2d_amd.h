@@ -216,15 +216,28 @@ int f2d_model_last_solve(f2d_model *M, int *nite, double *res);
  */
 typedef struct f2d_batch f2d_batch;
 const char *f2d_batch_last_error(void);
+/* Launch configuration of a batch, chosen by the autotuner
+ * (fluid2d_amd/autotune.py) or left at the defaults (NULL / zeros).
+ * coarse_lds_budget: bytes of LDS for the single-workgroup coarse-level
+ *   kernel (0 = the device maximum); levels are handed to it from the
+ *   coarsest up while they fit.
+ * min_tiles: a fine-level phase uses the tallest tile (16, 8 or 4 rows of
+ *   64 cells) that still gives at least this many workgroups (0 = always
+ *   16 rows).  No reference counterpart (the reference has no launches). */
+typedef struct {
+  int coarse_lds_budget;
+  int min_tiles;
+} f2d_batch_tuning;
 /* precision: 0 FP64 throughout (bit-identical to the reference algorithm);
  *   1 mixed: FP64 state, RK combination, full-solve rhs, outer residual and
  *   correction (iterative refinement) and coarse-level solve; FP32
  *   advection, tendencies and fine-level V-cycles.
- * coarse_lds_budget: bytes of LDS for the single-workgroup coarse-level
- * kernel (0 = 64 KiB); levels are handed to it from the coarsest up while
- * they fit.  msks[i]: host cell mask of instance i. */
+ * msks[i]: host cell mask of instance i; tuning may be NULL. */
 int f2d_batch_create(int ninst, const f2d_model_params *params, const int8_t *const *msks,
-                     int precision, int coarse_lds_budget, f2d_batch **out);
+                     int precision, const f2d_batch_tuning *tuning, f2d_batch **out);
+/* new launch configuration for an existing batch (between steps) */
+int f2d_batch_retune(f2d_batch *b, const f2d_batch_tuning *tuning);
+int f2d_batch_tuning_info(f2d_batch *b, f2d_batch_tuning *out);
 int f2d_batch_destroy(f2d_batch *b);
 int f2d_batch_info(f2d_batch *b, int inst, int *nlev, int *nfine, long long *launches_per_step);
 int f2d_batch_set_field(f2d_batch *b, int inst, int field, const double *host);
@@ -232,6 +245,12 @@ int f2d_batch_get_state(f2d_batch *b, int inst, double *host);
 int f2d_batch_state_ptr(f2d_batch *b, int inst, double **dev);
 /* fixed dt of instance inst (inst < 0: all) */
 int f2d_batch_set_dt(f2d_batch *b, int inst, double dt);
+/* Step budget of instance inst (inst < 0: all; nsteps < 0: unlimited).  A
+ * gate at the top of every step lets an instance run while its budget
+ * lasts; exhausted instances are skipped by every kernel, so instances of
+ * one batch can stop at different step counts. */
+int f2d_batch_set_steps(f2d_batch *b, int inst, int nsteps);
+int f2d_batch_progress(f2d_batch *b, int inst, int *steps_left, long long *steps_done);
 /* adaptive dt = cfl*min(dx,dy)/maxspeed capped at dtmax (core/fluid2d.py:376-397) */
 int f2d_batch_set_adaptive(f2d_batch *b, int on, double cfl, double dtmax);
 int f2d_batch_use_graph(f2d_batch *b, int on);
@@ -241,6 +260,14 @@ int f2d_batch_set_psi_from_vorticity(f2d_batch *b);
 int f2d_batch_step(f2d_batch *b, int nsteps);
 int f2d_batch_step_async(f2d_batch *b, int nsteps);
 int f2d_batch_synchronize(f2d_batch *b);
+/* done = 1 when all work enqueued on the batch's stream has completed */
+int f2d_batch_query(f2d_batch *b, int *done);
+/* Runs nsteps steps eagerly with HIP events around every launch of kind
+ * (0 smoother, 1 residual/restriction, 2 advection, 3 coarse levels) at
+ * top-down level `level` (-1: any).  avg_ms: mean launch duration;
+ * alg_bytes / impl_bytes: mean algorithmic / implemented bytes per launch. */
+int f2d_batch_profile(f2d_batch *b, int kind, int level, int nsteps, double *avg_ms,
+                      long long *nlaunch, double *alg_bytes, double *impl_bytes);
 /* out8 as f2d_model_diagnostics (raw sums; maxspeed and ke scaled) */
 int f2d_batch_diagnostics(f2d_batch *b, int inst, double *out8);
 int f2d_batch_solve_info(f2d_batch *b, int inst, int *nite, double *res, int *diverged);
