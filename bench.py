@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--bous-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the step kernel by kernel instead of replaying a hipGraph "
+                         "(for kernel-trace profilers)")
     return ap.parse_args()
 
 
@@ -71,6 +74,9 @@ def max_rel(sa, sb):
     return max(rel(sa[k], sb[k]) for k in range(sa.shape[0]))
 
 
+USE_GRAPH = True
+
+
 def config_batch(name, precision, dt=None):
     from fluid2d_amd import workloads
     from fluid2d_amd.batch import Batch
@@ -78,7 +84,7 @@ def config_batch(name, precision, dt=None):
 
     p = workloads.make_param(name)
     g = Grid(p)
-    b = Batch([(p, g)], precision=precision)
+    b = Batch([(p, g)], precision=precision, use_graph=USE_GRAPH)
     for k, v in workloads.initial_fields(name, g).items():
         b.set_field(0, k, v)
     b.set_psi_from_vorticity()
@@ -99,7 +105,7 @@ def ensemble_batch(n, precision, dts=None):
     for kw, _ in specs:
         p = Param(**kw)
         pg.append((p, Grid(p)))
-    b = Batch(pg, precision=precision)
+    b = Batch(pg, precision=precision, use_graph=USE_GRAPH)
     for i, (_, seed) in enumerate(specs):
         b.set_field(i, "vorticity", workloads.turbulence(b.grids[i], seed))
     b.set_psi_from_vorticity()
@@ -186,6 +192,8 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
     legs = set(a.legs.split(","))
+    global USE_GRAPH
+    USE_GRAPH = not a.eager
     name = HEADLINE
     out = {
         "metric": "cell-updates/s incl. MG Poisson, batched vs plain @ rel-L2<=1e-6; p99 latency",

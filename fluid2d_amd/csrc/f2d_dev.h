@@ -151,6 +151,21 @@ struct ABits {
   }
 };
 
+// the same weights from a tile of stencil bits: `cell` points at the cell's
+// byte, `pitch` is the tile's row length, `n` the array's (flat offsets d
+// of the operator are row-major in the array: d = dj * n + di)
+template <typename T>
+struct ABitsTile {
+  const uint8_t* cell;
+  int pitch, n;
+  const T* w;
+  __device__ __forceinline__ T operator()(int k, ptrdiff_t d) const {
+    const int dj = d > 1 ? 1 : (d < -1 ? -1 : 0);
+    const int di = (int)(d - (ptrdiff_t)dj * n);
+    return ((cell[dj * pitch + di] >> k) & 1) ? w[k] : T(0);
+  }
+};
+
 template <typename T, typename X>
 __device__ __forceinline__ T offdiag_x(const T* A, size_t P, size_t c, int n, X x) {
   return offdiag_f<T>(n, APlanes<T>{A, P, c}, x);

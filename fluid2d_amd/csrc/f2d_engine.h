@@ -39,6 +39,7 @@ struct InstD {
   // the level-0 correction pair x0 (T is double or float)
   long long state, work, xr, yr, A0d;
   long long k[3], x0[2];
+  long long kp[3];                     // mixed mode: FP64 psi tendencies (P pool); else -1
   long long msk, mskp;                 // byte pool offsets
   // finest-level operator from stencil bits: A_k(cell) = w0[k] where bit k
   // of sb0[cell] is set, else 0 (verified against the explicit operator at

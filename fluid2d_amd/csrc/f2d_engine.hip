@@ -78,9 +78,19 @@ struct Stage {
 
 // ---------------------------------------------------------------------------
 // k_rhs: tendencies of the tracers for one RK stage.  The stage state of the
-// tracer(s) and of (u, v) is formed once per cell into LDS tiles (3-cell
-// halo for the tracer, 1 for the velocities); fluxes, torque and diffusion
-// then read the tiles.
+// tracer(s) and of (u, v) is formed once per cell (FP64) into LDS tiles
+// (3-cell halo for the tracer, 1 for the velocities); fluxes, torque and
+// diffusion then read the tiles.
+//
+// FP64 (T = double): the reference arithmetic, operation for operation.
+// Mixed (T = float): the flux arithmetic runs in FP32 on differences
+// q_k - q_c taken in FP64 against the cell's own value q_c, and the part of
+// the tendency carried by q_c -- q_c times the flux divergence of a unit
+// field, D_c -- is added in FP64.  The flux is linear in q, so the sum is the
+// same tendency; rounding then scales with the local variation of q instead
+// of with q itself (a plain FP32 evaluation loses about L/dx digits to the
+// cancellation in f_e - f_w).  Torque and diffusion likewise difference in
+// FP64 and multiply in FP32.
 // ---------------------------------------------------------------------------
 constexpr int QW = TX + 6, QH = TY + 6;
 
@@ -89,9 +99,9 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
                                              const InstD* __restrict__ I, double* P, T* Q,
                                              const int8_t* __restrict__ B,
                                              const DynD* __restrict__ dyn, int stage, int out_k) {
-  __shared__ T qs[2][QH * QW];
-  __shared__ T us[TY * (TX + 1)];
-  __shared__ T vs[(TY + 1) * TX];
+  __shared__ double qs[2][QH * QW];
+  __shared__ double us[TY * (TX + 1)];
+  __shared__ double vs[(TY + 1) * TX];
   __shared__ int8_t ms[QH * QW];
   const Task tk = tasks[blockIdx.x];
   if (idle(dyn, tk.inst)) return;
@@ -125,61 +135,97 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
     const bool ok = j < m && i < n;
     const size_t e = ok ? at(j, i, n) : 0;
     ms[q] = ok ? msk[e] : 0;
-    for (int t = 0; t < ntr; t++) qs[t][q] = ok ? T(S((t == 0 ? 0 : 4 * F) + e)) : T(0);
+    for (int t = 0; t < ntr; t++) qs[t][q] = ok ? S((t == 0 ? 0 : 4 * F) + e) : 0.;
   }
   for (int q = tid; q < TY * (TX + 1); q += nt) {  // u on rows y0.., cols x0-1..
     const int r = q / (TX + 1), c = q % (TX + 1);
     const int j = y0 + r, i = x0 - 1 + c;
-    us[q] = (j < m && i < n) ? T(S(2 * F + at(j, i, n))) : T(0);
+    us[q] = (j < m && i < n) ? S(2 * F + at(j, i, n)) : 0.;
   }
   for (int q = tid; q < (TY + 1) * TX; q += nt) {  // v on rows y0-1.., cols x0..
     const int r = q / TX, c = q % TX;
     const int j = y0 - 1 + r, i = x0 + c;
-    vs[q] = (j < m && i < n) ? T(S(3 * F + at(j, i, n))) : T(0);
+    vs[q] = (j < m && i < n) ? S(3 * F + at(j, i, n)) : 0.;
   }
   __syncthreads();
   // advection parameters (core/fortran_advection.f90:46-60), cst[3] = umax
-  UpParams<T> p;
+  UpParams<double> p;
   const double umax = d.umax, logcosh = 0.05;
-  p.zdx = T(1. / in.dx);
-  p.zdy = T(1. / in.dy);
+  p.zdx = 1. / in.dx;
+  p.zdy = 1. / in.dy;
   {
     const double a = logcosh / umax;
-    p.a = T(a);
-    p.KK = T((umax - log(cosh(umax * a)) / a) * 0.5);
+    p.a = a;
+    p.KK = (umax - log(cosh(umax * a)) / a) * 0.5;
     const double u1 = in.aparab * umax;
-    p.u1 = T(u1);
-    p.aa = T(1. / (2. * u1));
-    p.bb = T(u1 * 0.5);
+    p.u1 = u1;
+    p.aa = 1. / (2. * u1);
+    p.bb = u1 * 0.5;
   }
   p.method = in.method;
   p.order = in.order;
+  UpParams<T> pt;
+  pt.zdx = T(p.zdx);
+  pt.zdy = T(p.zdy);
+  pt.a = T(p.a);
+  pt.KK = T(p.KK);
+  pt.u1 = T(p.u1);
+  pt.aa = T(p.aa);
+  pt.bb = T(p.bb);
+  pt.method = p.method;
+  pt.order = p.order;
   const bool upwind = in.order % 2 == 1;
   const bool kforce = stage == (in.tscheme == 0 ? 2 : 0);
   T* out = Q + in.k[out_k];
+  constexpr bool MIXED = sizeof(T) == 4;
   for (int r = threadIdx.y; r < TY; r += blockDim.y) {
     const int J = y0 + r, Ic = x0 + threadIdx.x;
     if (J >= m - NH || Ic >= n - NH) continue;
     const int lq = (r + 3) * QW + threadIdx.x + 3;  // cell in the tracer/mask tile
-    const T ue = us[r * (TX + 1) + threadIdx.x + 1], uw = us[r * (TX + 1) + threadIdx.x];
-    const T vn = vs[(r + 1) * TX + threadIdx.x], vsn = vs[r * TX + threadIdx.x];
+    const double ue = us[r * (TX + 1) + threadIdx.x + 1], uw = us[r * (TX + 1) + threadIdx.x];
+    const double vn = vs[(r + 1) * TX + threadIdx.x], vsn = vs[r * TX + threadIdx.x];
+    // flux divergence of the unit field (mixed mode only)
+    double Dc = 0.;
+    if (MIXED) {
+      auto one = [](int) { return 1.; };
+      double ce, cw, cn, cs;
+      if (upwind) {
+        ce = up_edge_flux_q(ms + lq, 1, ue, p, one);
+        cw = up_edge_flux_q(ms + lq - 1, 1, uw, p, one);
+        cn = up_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, p, one);
+        cs = up_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, p, one);
+      } else {
+        ce = ce_edge_flux_q(ms + lq, 1, ue, in.order, true, one);
+        cw = ce_edge_flux_q(ms + lq - 1, 1, uw, in.order, true, one);
+        cn = ce_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, in.order, true, one);
+        cs = ce_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, in.order, true, one);
+      }
+      Dc = -p.zdx * (ce - cw) - p.zdy * (cn - cs);
+    }
     for (int t = 0; t < ntr; t++) {
-      const T* qt = qs[t];
-      auto qx = [&](int base) { return [qt, base](int k) { return qt[base + k]; }; };
-      auto qy = [&](int base) { return [qt, base](int k) { return qt[base + k * QW]; }; };
+      const double* qt = qs[t];
+      const double qc = MIXED ? qt[lq] : 0.;
+      // tracer at offset k (T-rounded; relative to qc in mixed mode)
+      auto qx = [&](int base) {
+        return [qt, base, qc](int k) { return T(qt[base + k] - qc); };
+      };
+      auto qy = [&](int base) {
+        return [qt, base, qc](int k) { return T(qt[base + k * QW] - qc); };
+      };
       T fe, fw, fn, fs;
       if (upwind) {
-        fe = up_edge_flux_q(ms + lq, 1, ue, p, qx(lq));
-        fw = up_edge_flux_q(ms + lq - 1, 1, uw, p, qx(lq - 1));
-        fn = up_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, p, qy(lq));
-        fs = up_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, p, qy(lq - QW));
+        fe = up_edge_flux_q(ms + lq, 1, T(ue), pt, qx(lq));
+        fw = up_edge_flux_q(ms + lq - 1, 1, T(uw), pt, qx(lq - 1));
+        fn = up_edge_flux_q(ms + lq, (ptrdiff_t)QW, T(vn), pt, qy(lq));
+        fs = up_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, T(vsn), pt, qy(lq - QW));
       } else {
-        fe = ce_edge_flux_q(ms + lq, 1, ue, in.order, true, qx(lq));
-        fw = ce_edge_flux_q(ms + lq - 1, 1, uw, in.order, true, qx(lq - 1));
-        fn = ce_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, in.order, true, qy(lq));
-        fs = ce_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, in.order, true, qy(lq - QW));
+        fe = ce_edge_flux_q(ms + lq, 1, T(ue), in.order, true, qx(lq));
+        fw = ce_edge_flux_q(ms + lq - 1, 1, T(uw), in.order, true, qx(lq - 1));
+        fn = ce_edge_flux_q(ms + lq, (ptrdiff_t)QW, T(vn), in.order, true, qy(lq));
+        fs = ce_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, T(vsn), in.order, true, qy(lq - QW));
       }
-      T val = -p.zdx * (fe - fw) - p.zdy * (fn - fs);
+      T val = -pt.zdx * (fe - fw) - pt.zdy * (fn - fs);
+      if (MIXED) val = T((double)val + qc * Dc);
       const int8_t mc = ms[lq];
       if (in.model == 1 && t == 0) {  // y *= msk; add_torque
         val = val * T(mc);
@@ -187,17 +233,17 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
         ml = ml < 1 ? 1 : ml;
         mr = mr < 1 ? 1 : mr;
         if (ml + mr == 4)
-          val = val + (qs[1][lq + 1] - qs[1][lq - 1]) * T(0.5 * in.gravity / in.dx) * T(mc);
+          val = val + T(qs[1][lq + 1] - qs[1][lq - 1]) * T(0.5 * in.gravity / in.dx) * T(mc);
       }
       if (in.diffusion && kforce && mc == 1) {  // add_diffusion
         const double coefd =
             (in.model == 0 ? 1. : (in.tscheme == 0 ? 1.5 : 1.)) * (t == 0 ? in.kd0 : in.kd1);
         const T coef = T(coefd / (in.dx * in.dx));
-        const T tc = qt[lq];
-        T sm = T(ms[lq - 1]) * (qt[lq - 1] - tc);
-        sm = sm + T(ms[lq + 1]) * (qt[lq + 1] - tc);
-        sm = sm + T(ms[lq - QW]) * (qt[lq - QW] - tc);
-        sm = sm + T(ms[lq + QW]) * (qt[lq + QW] - tc);
+        const double tc = qt[lq];
+        T sm = T(ms[lq - 1]) * T(qt[lq - 1] - tc);
+        sm = sm + T(ms[lq + 1]) * T(qt[lq + 1] - tc);
+        sm = sm + T(ms[lq - QW]) * T(qt[lq - QW] - tc);
+        sm = sm + T(ms[lq + QW]) * T(qt[lq + QW] - tc);
         val = val + coef * sm;
       }
       store_img_mask<T>(out + (t == 0 ? 0 : 4 * F), nullptr, J, Ic, val, m, n);
@@ -208,21 +254,22 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
 // ---------------------------------------------------------------------------
 // k_c2c: dst = celltocorner(src) on [0, m-2] x [0, n-2]
 // ---------------------------------------------------------------------------
-template <typename T>
+// T: source type, D: destination type (an FP32 source summed in FP64 is exact)
+template <typename T, typename D>
 __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
-                                             const InstD* __restrict__ I, const T* SP, T* DP,
+                                             const InstD* __restrict__ I, const T* SP, D* DP,
                                              const DynD* __restrict__ dyn, int pred) {
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
   const T* src = SP + tk.xin;
-  T* dst = DP + tk.xout;
+  D* dst = DP + tk.xout;
   for (int r = threadIdx.y; r < TY; r += blockDim.y) {
     const int j = tk.y0 + r, i = tk.x0 + threadIdx.x;
     if (j >= m - 1 || i >= n - 1) continue;
     const size_t c = at(j, i, n);
-    dst[c] = T(0.25) * (src[c] + src[c + 1] + src[c + n] + src[c + n + 1]);
+    dst[c] = D(0.25) * (D(src[c]) + D(src[c + 1]) + D(src[c + n]) + D(src[c + n + 1]));
   }
 }
 
@@ -238,6 +285,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
                                                 const DynD* __restrict__ dyn, int pred) {
   __shared__ T xs[TYV + 4][TX + 4];
   __shared__ T j1[TYV + 2][TX + 2];
+  __shared__ uint8_t sbs[TYV + 4][TX + 4];  // stencil bits (finest level)
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
   const int MODE = tk.mode;
@@ -252,11 +300,15 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   const T* xin = Q + tk.xin;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
   const int y0 = tk.y0, x0 = tk.x0;
-  // x on rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1], clipped to [1, m-2]
+  const bool fly = tk.lev == 0 && in.a0fly;
+  const uint8_t* sb = (const uint8_t*)(B + in.sb0);
+  // x (and the stencil bits) on rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1],
+  // clipped to [1, m-2]
   for (int q = tid; q < (TYV + 4) * (TX + 4); q += nt) {
     const int rr = q / (TX + 4), cc = q % (TX + 4);
     const int j = y0 - 2 + rr, i = x0 - 2 + cc;
     T v = T(0);
+    uint8_t bits = 0;
     if (j <= m - 2 && i <= n - 2) {
       if (MODE == 0 || MODE == 2) v = xin[at(j, i, n)];
       if (MODE == 2 || MODE == 3) {
@@ -264,8 +316,10 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
         const T r = interp_at<T>(msk, n, B + C.msk, Q + tk.xc, C.n, j, i);
         v = MODE == 2 ? v + r : r;
       }
+      if (fly) bits = sb[at(j, i, n)];
     }
     xs[rr][cc] = v;
+    sbs[rr][cc] = bits;
   }
   __syncthreads();
   const T c1 = T(in.omega), c2 = T(1. - in.omega);
@@ -273,8 +327,9 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   const int8_t* mp = final_mask ? B + in.mskp : nullptr;
   T w[5];
   for (int k = 0; k < 5; k++) w[k] = T(in.w0[k]);
-  const uint8_t* sb = (const uint8_t*)(B + in.sb0);
-  auto sweeps = [&](auto Aat) {
+  // Aat(c, r, q): operator accessor of the cell at global c, tile (r, q) of
+  // the x/bits tiles; Mk(c, r, q): its mask
+  auto sweeps = [&](auto Aat, auto Mk) {
     // first sweep on rows [y0-1, y0+TY] x cols [x0-1, x0+TX], clipped to [.., m-NH]
     for (int q = tid; q < (TYV + 2) * (TX + 2); q += nt) {
       const int rr = q / (TX + 2), cc = q % (TX + 2);
@@ -282,11 +337,12 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       T v = T(0);
       if (j <= m - NH && i <= n - NH) {
         const size_t c = at(j, i, n);
-        v = jacobi_f<T>(msk[c], n, b[c], c1, c2, Aat(c), [&](int dd) {
-          const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-          const int di = dd - dj * n;
-          return xs[rr + 1 + dj][cc + 1 + di];
-        });
+        v = jacobi_f<T>(Mk(c, rr + 1, cc + 1), n, b[c], c1, c2, Aat(c, rr + 1, cc + 1),
+                        [&](int dd) {
+                          const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+                          const int di = dd - dj * n;
+                          return xs[rr + 1 + dj][cc + 1 + di];
+                        });
       }
       j1[rr][cc] = v;
     }
@@ -296,18 +352,24 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       if (J >= m - NH || Ic >= n - NH) continue;
       const int cc = threadIdx.x;
       const size_t c = at(J, Ic, n);
-      const T v = jacobi_f<T>(msk[c], n, b[c], c1, c2, Aat(c), [&](int dd) {
-        const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-        const int di = dd - dj * n;
-        return j1[rr + 1 + dj][cc + 1 + di];
-      });
+      const T v = jacobi_f<T>(Mk(c, rr + 2, cc + 2), n, b[c], c1, c2, Aat(c, rr + 2, cc + 2),
+                              [&](int dd) {
+                                const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
+                                const int di = dd - dj * n;
+                                return j1[rr + 1 + dj][cc + 1 + di];
+                              });
       store_img_mask<T>(xout, mp, J, Ic, v, m, n);
     }
   };
-  if (tk.lev == 0 && in.a0fly)
-    sweeps([&](size_t c) { return ABits<T>{sb, w, c}; });
+  if (fly)
+    sweeps(
+        [&](size_t, int r, int q) {
+          return ABitsTile<T>{&sbs[r][q], TX + 4, n, w};
+        },
+        [&](size_t, int r, int q) { return (int8_t)((sbs[r][q] >> 4) & 1); });
   else
-    sweeps([&](size_t c) { return APlanes<T>{A, PL, c}; });
+    sweeps([&](size_t c, int, int) { return APlanes<T>{A, PL, c}; },
+           [&](size_t c, int, int) { return msk[c]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -549,8 +611,10 @@ __device__ void c_vcycle(CL<T>* Ls, int lev1, int deep, bool zero_first, T c1, T
 
 // Operands come from the T pool and are solved in FP64 in LDS (the
 // coarsest-grid solve stays FP64 in mixed mode).
+constexpr int COARSE_THREADS = 1024;  // one workgroup per instance holds the CU's LDS anyway
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_coarse(const Task* __restrict__ tasks,
+__global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restrict__ tasks,
                                                 const InstD* __restrict__ I, T* Q,
                                                 const int8_t* __restrict__ B,
                                                 const DynD* __restrict__ dyn, int pred, int mode) {
@@ -626,27 +690,29 @@ __global__ void __launch_bounds__(256) k_coarse(const Task* __restrict__ tasks,
 // ---------------------------------------------------------------------------
 // k_ortho: (u,v) from psi on [1, m-2] (computeorthogradient)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, typename S>
 __global__ void __launch_bounds__(256) k_ortho(const Task* __restrict__ tasks,
-                                               const InstD* __restrict__ I, T* P,
+                                               const InstD* __restrict__ I, T* P, const S* SP,
                                                const int8_t* __restrict__ B,
                                                const DynD* __restrict__ dyn) {
-  // P: the pool holding the state-shaped array (FP64 state or T tendency)
+  // P: the pool holding the state-shaped array (FP64 state or T tendency),
+  // u and v written at +2F, +3F of tk.xout; psi read at SP + tk.xin (its
+  // own plane, FP64 for the refined tendencies of the mixed mode)
   const Task tk = tasks[blockIdx.x];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
   const size_t F = (size_t)m * n;
   const int8_t* msk = B + in.msk;
-  T* base = P + tk.xout;  // state-shaped (psi at +F, u at +2F, v at +3F)
-  const T* psi = base + F;
-  const T zdx = T(1. / in.dx), zdy = T(1. / in.dy);
+  T* base = P + tk.xout;
+  const S* psi = SP + tk.xin;
+  const S zdx = S(1. / in.dx), zdy = S(1. / in.dy);
   for (int r = threadIdx.y; r < TY; r += blockDim.y) {
     const int j = tk.y0 + r, i = tk.x0 + threadIdx.x;
     if (j < 1 || i < 1 || j >= m - 1 || i >= n - 1) continue;
     const size_t c = at(j, i, n);
-    base[2 * F + c] = (msk[c] + msk[c + 1] == 2) ? zdy * (psi[c - n] - psi[c]) : T(0);
-    base[3 * F + c] = (msk[c] + msk[c + n] == 2) ? zdx * (psi[c] - psi[c - 1]) : T(0);
+    base[2 * F + c] = (msk[c] + msk[c + 1] == 2) ? T(zdy * (psi[c - n] - psi[c])) : T(0);
+    base[3 * F + c] = (msk[c] + msk[c + n] == 2) ? T(zdx * (psi[c] - psi[c - 1])) : T(0);
   }
 }
 
@@ -669,13 +735,18 @@ __global__ void __launch_bounds__(256) k_combine(const Task* __restrict__ tasks,
   const T *k0 = Q + in.k[0], *k1 = Q + in.k[1], *k2 = Q + in.k[2];
   const int field = tk.lev;
   const size_t base = (size_t)field * F;
+  // psi tendencies of the mixed mode are FP64 planes of their own
+  const bool p64 = field == 1 && in.kp[0] >= 0;
+  const double *p0 = P + in.kp[0], *p1 = P + in.kp[1], *p2 = P + in.kp[2];
   for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
     const size_t e = base + q;
+    const double a0 = p64 ? p0[q] : (double)k0[e], a1 = p64 ? p1[q] : (double)k1[e],
+                 a2 = p64 ? p2[q] : (double)k2[e];
     if (in.tscheme == 0) {
       const double c6 = dt / 6.;
-      x[e] = x[e] + c6 * (((double)k0[e] + (double)k1[e]) + 4. * (double)k2[e]);
+      x[e] = x[e] + c6 * ((a0 + a1) + 4. * a2);
     } else {
-      x[e] = x[e] + dt * (double)k2[e];
+      x[e] = x[e] + dt * a2;
     }
     if (in.model == 1 && field == 4) x[5 * F + q] = x[e] - 0.;  // banom = b - bref(=0)
   }
@@ -1255,8 +1326,8 @@ struct Builder {
 
   using Off = std::function<long long(HostInst&)>;
 
-  // fp64: operands in the FP64 pool (state / work) instead of the T pool
-  void c2c(const std::vector<int>& who, Off src, Off dst, int pred, bool fp64) {
+  // pools: 0 T -> T, 1 FP64 -> FP64 (state / work), 2 T -> FP64
+  void c2c(const std::vector<int>& who, Off src, Off dst, int pred, int pools) {
     std::vector<Task> ts;
     for (int i : who) {
       HostInst& h = b.H[i];
@@ -1267,18 +1338,21 @@ struct Builder {
     }
     Launch& L = add(L_C2C, ts);
     L.pred = pred;
-    L.flags = fp64 ? 1 : 0;
+    L.flags = pools;
   }
 
-  void ortho(const std::vector<int>& who, Off base, bool fp64) {
+  // (u, v) at +2F, +3F of base from psi; pools: 0 T (psi at base + F),
+  // 1 FP64 (psi at base + F), 2 T output from the FP64 psi plane `psi`
+  void ortho(const std::vector<int>& who, Off base, int pools, Off psi = nullptr) {
     std::vector<Task> ts;
     for (int i : who) {
       HostInst& h = b.H[i];
       Task t = mk_task(i, 0, 0, 0);
       t.xout = base(h);
+      t.xin = psi ? psi(h) : base(h) + (long long)h.d.m * h.d.n;
       tiles(ts, t, 1, h.d.m - 1, 1, h.d.n - 1);
     }
-    add(L_ORTHO, ts).flags = fp64 ? 1 : 0;
+    add(L_ORTHO, ts).flags = pools;
   }
 
   // flat element ranges [0, cnt) per instance
@@ -1304,6 +1378,7 @@ struct Builder {
   // fast inversion of tendency slot k (core/operators.py:421-498, 'fast')
   void invert_fast(const std::vector<int>& who, int k) {
     if (who.empty()) return;
+    if (b.prec != 0) return invert_fast_refined(who, k);
     for (int i : who) {
       HostInst& h = b.H[i];
       const size_t F = (size_t)h.d.m * h.d.n;
@@ -1318,14 +1393,48 @@ struct Builder {
     vcycle(who, st, 0, 0, true);
     for (int i : who)
       if (b.H[i].cur[0] != 0) fprintf(stderr, "engine: fast inversion parity\n");
-    ortho(who, [k](HostInst& h) { return h.d.k[k]; }, false);
+    ortho(who, [k](HostInst& h) { return h.d.k[k]; }, 0);
+  }
+
+  // Mixed mode: the same two V-cycles (hierarchy.py:207-218) in correction
+  // form.  V-cycles are affine, V(x0, b) = x0 + V(0, b - A x0), so the FP64
+  // guess x0 (the previous psi tendency, an FP64 plane) is kept, the FP64
+  // defect r0 = b - A x0 is the FP32 V-cycles' right-hand side, and the
+  // correction is added in FP64.  u, v tendencies come from the FP64 psi
+  // tendency, so their differences do not cancel in FP32.
+  void invert_fast_refined(const std::vector<int>& who, int k) {
+    c2c(who, [k](HostInst& h) { return h.d.k[k]; }, [](HostInst& h) { return h.d.work; }, 0, 2);
+    std::vector<Task> rs;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      h.buf[0][0] = h.d.x0[0];
+      h.buf[0][1] = h.d.x0[1];
+      h.cur[0] = 0;
+      rr_tasks(i, 0, h.d.kp[k], h.d.work, h.d.L[0].b, rs);
+    }
+    add(L_RR, rs).flags = 1 | 16 | 32;  // FP64 defect, stored (T) as the level-0 rhs
+    std::vector<int> st(b.ninst, 0);
+    vcycle(who, st, 1, 0, false);
+    vcycle(who, st, 0, 0, true);
+    std::vector<Task> up;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      const size_t F = (size_t)h.d.m * h.d.n;
+      for (size_t q = 0; q < F; q += FLAT) {
+        Task t = mk_task(i, 0, (int)q, (int)std::min<size_t>(FLAT, F - q));
+        t.xout = h.d.kp[k];
+        t.xin = h.buf[0][h.cur[0]];
+        up.push_back(t);
+      }
+    }
+    add(L_UPDATE, up);
+    ortho(who, [k](HostInst& h) { return h.d.k[k]; }, 2, [k](HostInst& h) { return h.d.kp[k]; });
   }
 
   // full solve of the state's psi (core/operators.py:470-481)
   void solve_full(const std::vector<int>& who) {
     if (who.empty()) return;
-    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0,
-        true);
+    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0, 1);
     for (int i : who) {
       HostInst& h = b.H[i];
       h.buf[0][0] = h.d.x0[0];
@@ -1423,7 +1532,7 @@ struct Builder {
       fold(3, spoff, 0);
     }
     add(L_FINPSI, flat(who, [](HostInst& h) { return (long long)h.d.m * h.d.n; }, 1));
-    ortho(who, [](HostInst& h) { return h.d.state; }, true);
+    ortho(who, [](HostInst& h) { return h.d.state; }, 1);
   }
 
   void diag() {
@@ -1528,10 +1637,14 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
                            L.outk);
         break;
       case L_C2C:
-        if (L.flags & 1)
-          hipLaunchKernelGGL(k_c2c<double>, dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.dD, L.pred);
+        if (L.flags == 1)
+          hipLaunchKernelGGL((k_c2c<double, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.dD,
+                             L.pred);
+        else if (L.flags == 2)
+          hipLaunchKernelGGL((k_c2c<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.dD,
+                             L.pred);
         else
-          hipLaunchKernelGGL(k_c2c<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.dD, L.pred);
+          hipLaunchKernelGGL((k_c2c<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.dD, L.pred);
         break;
       case L_SMOOTH:
         if (L.ty == 4)
@@ -1550,14 +1663,18 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
                           b.partial);
         break;
       case L_COARSE:
-        hipLaunchKernelGGL(k_coarse<T>, dim3(g), dim3(256), L.shmem, b.s, TK, b.dI, Q, b.B,
+        hipLaunchKernelGGL(k_coarse<T>, dim3(g), dim3(COARSE_THREADS), L.shmem, b.s, TK, b.dI, Q, b.B,
                            b.dD, L.pred, L.mode);
         break;
       case L_ORTHO:
-        if (L.flags & 1)
-          hipLaunchKernelGGL(k_ortho<double>, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B, b.dD);
+        if (L.flags == 1)
+          hipLaunchKernelGGL((k_ortho<double, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.B,
+                             b.dD);
+        else if (L.flags == 2)
+          hipLaunchKernelGGL((k_ortho<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B,
+                             b.dD);
         else
-          hipLaunchKernelGGL(k_ortho<T>, dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD);
+          hipLaunchKernelGGL((k_ortho<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.B, b.dD);
         break;
       case L_COMBINE:
         hipLaunchKernelGGL(k_combine<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD);
@@ -1847,6 +1964,7 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
     d.yr = takeP(F);
     d.A0d = takeP(5 * F);
     for (int k = 0; k < 3; k++) d.k[k] = takeQ(d.nvar * F);
+    for (int k = 0; k < 3; k++) d.kp[k] = precision != 0 ? takeP(F) : -1;
     d.x0[0] = takeQ(F);
     d.x0[1] = takeQ(F);
     h.x0alt = takeQ(F);
@@ -1953,6 +2071,9 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
           }
           d.w0[k] = wk;
         }
+        // the smoother takes the cell mask from the centre bit
+        for (size_t q = 0; q < S && fly; q++)
+          if ((mp[q] != 0) != ((bits[q] >> 4) & 1)) fly = false;
         d.a0fly = fly ? 1 : 0;
         if (fly)
           ok = ok && hipMemcpy(b->B + d.sb0, bits.data(), S, hipMemcpyHostToDevice) == hipSuccess;

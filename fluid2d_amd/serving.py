@@ -49,7 +49,8 @@ class Request:
         cells = (self.n + 6) ** 2
         nvar = 5 if self.modelname == "euler" else 6
         tz = 8 if precision == "fp64" else 4
-        return int(1.15 * cells * (8 * (nvar + 8) + tz * (3 * nvar + 3 + 11) + 5))
+        pw = nvar + 8 + (0 if precision == "fp64" else 3)  # FP64 pool words per cell
+        return int(1.15 * cells * (8 * pw + tz * (3 * nvar + 3 + 11) + 5))
 
 
 def stream_requests(count, rate=50.0, seed=5, sizes_log2=(7, 11), steps=(50, 2000)):

@@ -149,3 +149,49 @@ def test_mixed_precision_error(case):
     for k in range(out["fp64"].shape[0]):
         r = rel(out["mixed"][k], out["fp64"][k])
         assert r <= 1e-6, "field %d rel-L2 %.3e" % (k, r)
+
+
+@pytest.mark.parametrize("tuning", [
+    {"coarse_lds_budget": 24576, "min_tiles": 0},
+    {"coarse_lds_budget": 0, "min_tiles": 1024},
+    {"coarse_lds_budget": 49152, "min_tiles": 256},
+])
+def test_tuning_does_not_change_results(tuning):
+    """Tile heights and the coarse-kernel split change only how the work is
+    launched: FP64 results stay bit-identical (and so equal to the oracle)."""
+    specs = [spec("euler", 256, 256, 1.0, 1.0, "closed"),
+             spec("euler", 64, 64, 1.0, 1.0, "perio"),
+             spec("boussinesq", 128, 64, 2.0, 1.0, "xchannel", diffusion=True, Kdiff=1e-5)]
+    out = []
+    for t in (None, tuning):
+        b = Batch(specs, tuning=t)
+        for i in range(len(specs)):
+            init_fields(b, i, 20 + i)
+        b.set_psi_from_vorticity()
+        for i in range(len(specs)):
+            b.set_dt(dt_for(b, i, b.diagnostics(i)["maxspeed"]), i)
+        b.step(6)
+        out.append([b.state(i) for i in range(len(specs))])
+        if t is not None:
+            got = b.tuning()
+            assert got["min_tiles"] == t["min_tiles"]
+    for i in range(len(specs)):
+        assert (out[0][i] == out[1][i]).all(), "instance %d" % i
+
+
+def test_retune_between_steps():
+    s = [spec("euler", 128, 128, 1.0, 1.0, "closed")]
+    ref = Batch(s)
+    init_fields(ref, 0, 5)
+    ref.set_psi_from_vorticity()
+    dt = dt_for(ref, 0, ref.diagnostics(0)["maxspeed"])
+    ref.set_dt(dt)
+    ref.step(6)
+    b = Batch(s)
+    init_fields(b, 0, 5)
+    b.set_psi_from_vorticity()
+    b.set_dt(dt)
+    b.step(3)
+    b.retune({"coarse_lds_budget": 24576, "min_tiles": 512})
+    b.step(3)
+    assert (b.state(0) == ref.state(0)).all()
