@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--bous-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch configuration instead of the autotuned one")
+    ap.add_argument("--tuning", default=None,
+                    help="JSON launch configuration to use instead of running the autotuner")
     ap.add_argument("--eager", action="store_true",
                     help="launch the step kernel by kernel instead of replaying a hipGraph "
                          "(for kernel-trace profilers)")
@@ -77,14 +81,30 @@ def max_rel(sa, sb):
 USE_GRAPH = True
 
 
-def config_batch(name, precision, dt=None):
+FIXED_TUNING = None
+
+
+def tuned(pg, precision, enabled):
+    """Launch configuration from the autotuner (timed on a scratch batch,
+    outside every timed region), or the defaults."""
+    if not enabled:
+        return None
+    if FIXED_TUNING is not None:
+        return FIXED_TUNING
+    from fluid2d_amd import autotune
+    best, _ = autotune.tune(pg, precision, use_cache=False)
+    return best
+
+
+def config_batch(name, precision, dt=None, tune=False):
     from fluid2d_amd import workloads
     from fluid2d_amd.batch import Batch
     from fluid2d_amd.grid import Grid
 
     p = workloads.make_param(name)
     g = Grid(p)
-    b = Batch([(p, g)], precision=precision, use_graph=USE_GRAPH)
+    b = Batch([(p, g)], precision=precision, use_graph=USE_GRAPH,
+              tuning=tuned([(p, g)], precision, tune))
     for k, v in workloads.initial_fields(name, g).items():
         b.set_field(0, k, v)
     b.set_psi_from_vorticity()
@@ -94,7 +114,7 @@ def config_batch(name, precision, dt=None):
     return b, g, dt
 
 
-def ensemble_batch(n, precision, dts=None):
+def ensemble_batch(n, precision, dts=None, tune=False):
     from fluid2d_amd import workloads
     from fluid2d_amd.batch import Batch
     from fluid2d_amd.grid import Grid
@@ -105,7 +125,7 @@ def ensemble_batch(n, precision, dts=None):
     for kw, _ in specs:
         p = Param(**kw)
         pg.append((p, Grid(p)))
-    b = Batch(pg, precision=precision, use_graph=USE_GRAPH)
+    b = Batch(pg, precision=precision, use_graph=USE_GRAPH, tuning=tuned(pg, precision, tune))
     for i, (_, seed) in enumerate(specs):
         b.set_field(i, "vorticity", workloads.turbulence(b.grids[i], seed))
     b.set_psi_from_vorticity()
@@ -136,7 +156,9 @@ def roofline(b, nsteps=2):
         pr = b.profile(kind, -1, nsteps)
         shares[kind] = pr["avg_ms"] * pr["launches"] / nsteps
     dom = max(shares, key=shares.get)
-    pr = b.profile(dom, 0, nsteps)
+    # the finest level of the dominant kind (the coarse kernel runs once
+    # per solve at its own first level)
+    pr = b.profile(dom, -1 if dom == "coarse" else 0, nsteps)
     achieved = pr["alg_bytes"] / (pr["avg_ms"] * 1e-3) / 1e9 if pr["avg_ms"] > 0 else 0.0
     return dom, shares, pr, achieved
 
@@ -192,8 +214,22 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
     legs = set(a.legs.split(","))
-    global USE_GRAPH
+    if "copy" in legs:
+        # calibration for the HBM byte counters: a plain device copy of a
+        # known size (the runtime's copy kernel), timed with the same clock
+        from fluid2d_amd.device import DeviceArray
+        nbytes = 256 << 20
+        src = DeviceArray((nbytes // 8,), np.float64)
+        dst = DeviceArray((nbytes // 8,), np.float64)
+        for _ in range(5):
+            dst.copy_from(src)
+        F.load().f2d_synchronize()
+        log("copy: 5 x %d bytes" % nbytes)
+        del src, dst
+    global USE_GRAPH, FIXED_TUNING
     USE_GRAPH = not a.eager
+    if a.tuning:
+        FIXED_TUNING = json.loads(a.tuning)
     name = HEADLINE
     out = {
         "metric": "cell-updates/s incl. MG Poisson, batched vs plain @ rel-L2<=1e-6; p99 latency",
@@ -211,7 +247,8 @@ def main():
     b = None
     if legs & {"plain", "roofline", "precision"}:
         log("plain FP64 %s: warmup %d, %d steps" % (name, a.warmup, a.steps))
-        b, g, dt = config_batch(name, "fp64")
+        b, g, dt = config_batch(name, "fp64", tune=not a.no_tune)
+        out["config"]["tuning"] = b.tuning()
         b.step(a.warmup)
         el = timed(b, a.steps)
         cells = g.nx * g.ny
@@ -232,7 +269,8 @@ def main():
         kname = KERNEL_NAMES[dom] + "<double>"
         traffic, src = load_traffic(KERNEL_NAMES[dom], name)
         out["roofline"] = {
-            "bound": "hbm", "kernel": kname + " finest-level launches",
+            "bound": "hbm",
+            "kernel": kname + (" (all launches)" if dom == "coarse" else " finest-level launches"),
             "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
             "alg_bytes_per_launch": pr["alg_bytes"], "avg_launch_ms": pr["avg_ms"],
@@ -244,7 +282,7 @@ def main():
     del b
     if "batched" in legs or "precision" in legs:
         log("ensemble %d instances mixed, %d steps" % (a.ensemble, a.ensemble_steps))
-        eb, dts, ecells = ensemble_batch(a.ensemble, "mixed")
+        eb, dts, ecells = ensemble_batch(a.ensemble, "mixed", tune=not a.no_tune)
         eb.step(2)
         el = timed(eb, a.ensemble_steps - 2)
         bval = ecells * (a.ensemble_steps - 2) / el
@@ -252,13 +290,14 @@ def main():
                           "workload": "ensemble-%d-euler-64..512" % a.ensemble,
                           "instances": a.ensemble, "cells": ecells,
                           "steps": a.ensemble_steps, "ms_per_step": el / (a.ensemble_steps - 2) * 1e3,
-                          "speedup_vs_plain_fp64": (bval / out["value"]) if out["value"] else None}
+                          "speedup_vs_plain_fp64": (bval / out["value"]) if out["value"] else None,
+                          "tuning": eb.tuning()}
         log("batched mixed: %.3e cell-updates/s" % bval)
         if "precision" in legs:
             mixed_states = [eb.state(i) for i in range(eb.n)]
             del eb
             log("ensemble FP64, %d steps" % a.ensemble_steps)
-            fb, _, _ = ensemble_batch(a.ensemble, "fp64", dts)
+            fb, _, _ = ensemble_batch(a.ensemble, "fp64", dts, tune=not a.no_tune)
             fb.step(2)
             el64 = timed(fb, a.ensemble_steps - 2)
             out["batched"]["fp64_value"] = ecells * (a.ensemble_steps - 2) / el64
@@ -271,7 +310,7 @@ def main():
     if "precision" in legs and plain_state is not None:
         log("mixed %s, %d steps" % (name, a.warmup + a.steps))
         _, _, dt = config_batch(name, "fp64")
-        mb, g, _ = config_batch(name, "mixed", dt)
+        mb, g, _ = config_batch(name, "mixed", dt, tune=not a.no_tune)
         el = timed(mb, a.warmup + a.steps)
         out.setdefault("precision", {})
         out["precision"]["euler_closed_1024_rel_l2"] = max_rel(mb.state(0), plain_state)
@@ -279,11 +318,11 @@ def main():
         out["precision"]["euler_closed_1024_mixed_value"] = g.nx * g.ny * (a.warmup + a.steps) / el
         del mb
         log("Boussinesq %s fp64 + mixed, %d steps" % (BOUSSINESQ, a.bous_steps))
-        bb, g, dt = config_batch(BOUSSINESQ, "fp64")
+        bb, g, dt = config_batch(BOUSSINESQ, "fp64", tune=not a.no_tune)
         el64 = timed(bb, a.bous_steps)
         s64 = bb.state(0)
         del bb
-        bm, _, _ = config_batch(BOUSSINESQ, "mixed", dt)
+        bm, _, _ = config_batch(BOUSSINESQ, "mixed", dt, tune=not a.no_tune)
         elm = timed(bm, a.bous_steps)
         out["precision"]["boussinesq_rt_rel_l2"] = max_rel(bm.state(0), s64)
         out["precision"]["boussinesq_rt_steps"] = a.bous_steps

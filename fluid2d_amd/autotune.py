@@ -8,6 +8,9 @@ Two knobs shape the launches of a batch (``f2d_batch_tuning``):
 * ``min_tiles``: the fine phases use 64 x 16, 64 x 8 or 64 x 4 tiles, the
   tallest that still gives this many workgroups, so that small levels
   spread over more compute units.
+* ``coarse_wave_cells``: inside the coarse kernel, levels up to this size
+  run on a single wavefront (no workgroup barriers) instead of the whole
+  workgroup.
 
 :func:`tune` builds a scratch batch with the same instance specs (so the
 caller's state is never touched), starts it from seeded turbulence,
@@ -25,7 +28,8 @@ from .grid import Grid
 from .param import Param
 
 BUDGETS = (0, 98304, 49152, 24576)
-MIN_TILES = (0, 256, 1024)
+MIN_TILES = (0, 1024, 4096)
+WAVE_CELLS = (256, 0)
 
 
 def _pg(specs):
@@ -86,7 +90,8 @@ def _init(b, seed=11):
 
 
 def candidates():
-    return [{"coarse_lds_budget": cb, "min_tiles": mt} for cb in BUDGETS for mt in MIN_TILES]
+    return [{"coarse_lds_budget": cb, "min_tiles": mt, "coarse_wave_cells": wc}
+            for cb in BUDGETS for mt in MIN_TILES for wc in WAVE_CELLS]
 
 
 def tune(specs, precision="fp64", steps=3, cands=None, use_cache=True, verbose=False):

@@ -76,19 +76,23 @@ def _call(name, *args):
 
 class Tuning(C.Structure):
     """f2d_batch_tuning: coarse-kernel LDS budget (bytes, 0 = device
-    maximum) and the minimum workgroup count of a fine phase (0 = always
-    16-row tiles)."""
-    _fields_ = [("coarse_lds_budget", C.c_int), ("min_tiles", C.c_int)]
+    maximum), the minimum workgroup count of a fine phase (0 = always
+    16-row tiles) and the largest coarse level (cells) run on one wavefront
+    (-1 = default 256)."""
+    _fields_ = [("coarse_lds_budget", C.c_int), ("min_tiles", C.c_int),
+                ("coarse_wave_cells", C.c_int)]
 
     def as_dict(self):
-        return {"coarse_lds_budget": self.coarse_lds_budget, "min_tiles": self.min_tiles}
+        return {"coarse_lds_budget": self.coarse_lds_budget, "min_tiles": self.min_tiles,
+                "coarse_wave_cells": self.coarse_wave_cells}
 
 
 def _tuning(t):
     if t is None:
         return None
     if isinstance(t, dict):
-        t = Tuning(int(t.get("coarse_lds_budget", 0)), int(t.get("min_tiles", 0)))
+        t = Tuning(int(t.get("coarse_lds_budget", 0)), int(t.get("min_tiles", 0)),
+                   int(t.get("coarse_wave_cells", -1)))
     return C.byref(t)
 
 

@@ -90,46 +90,84 @@ __device__ __forceinline__ T ce_edge_flux_q(const int8_t* mk, ptrdiff_t s, T vel
   return vel * qp;
 }
 
-// Multigrid operator.  AF(k, d) returns coefficient plane k (A_1..A_5 of
-// the reference) at flat offset d from the cell: A_k couples the cell to
-// its lower neighbour k (SW, S, SE, W, centre); the upper couplings are
-// read from the neighbours by symmetry (core/gmg/fortran_multigrid.f90:
-// 68-82, 342-350).  X(d) returns x at flat offset d.
+// Multigrid operator in two-dimensional form.  A2(k, dj, di) returns
+// coefficient plane k (A_1..A_5 of the reference) at the cell offset
+// (dj, di): A_k couples the cell to its lower neighbour k (SW, S, SE, W,
+// centre); the upper couplings are read from the neighbours by symmetry
+// (core/gmg/fortran_multigrid.f90:68-82, 342-350).  X2(dj, di) returns x
+// at that offset.  Offsets are literals, so tile accessors compile to
+// immediate offsets.
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T offdiag_2d(AF A, X x) {
+  T s = A(0, 0, 0) * x(-1, -1);
+  s = s + A(1, 0, 0) * x(-1, 0);
+  s = s + A(2, 0, 0) * x(-1, 1);
+  s = s + A(3, 0, 0) * x(0, -1);
+  s = s + A(3, 0, 1) * x(0, 1);
+  s = s + A(2, 1, -1) * x(1, -1);
+  s = s + A(1, 1, 0) * x(1, 0);
+  s = s + A(0, 1, 1) * x(1, 1);
+  return s;
+}
+
+// one damped-Jacobi update (core/gmg/fortran_multigrid.f90:66-85); 0 where
+// the mask is 0.  Evaluated without a branch (a masked cell's value is
+// computed and discarded), so every operand load is issued at once.
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T jacobi_2d(int8_t mk, T bc, T c1, T c2, AF A, X x) {
+  const T c3 = c1 / fabs(A(4, 0, 0));
+  const T v = x(0, 0) * c2 + c3 * (offdiag_2d<T>(A, x) - bc);
+  return mk == 0 ? T(0) : v;
+}
+
+// jacobi_2d for a cell known to be active, with its factor c3 = c1 / |A_5|
+// computed once by the caller (the same value: the division is exact
+// arithmetic on the same operands)
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T jacobi_c3_2d(T bc, T c3, T c2, AF A, X x) {
+  return x(0, 0) * c2 + c3 * (offdiag_2d<T>(A, x) - bc);
+}
+
+// r = b - A x (core/gmg/fortran_multigrid.f90:340-357); 0 where the mask
+// is 0 (evaluated without a branch, as jacobi_2d)
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T residual_2d(int8_t mk, T bc, AF A, X x) {
+  T s = bc - A(0, 0, 0) * x(-1, -1);
+  s = s - A(1, 0, 0) * x(-1, 0);
+  s = s - A(2, 0, 0) * x(-1, 1);
+  s = s - A(3, 0, 0) * x(0, -1);
+  s = s - A(4, 0, 0) * x(0, 0);
+  s = s - A(3, 0, 1) * x(0, 1);
+  s = s - A(2, 1, -1) * x(1, -1);
+  s = s - A(1, 1, 0) * x(1, 0);
+  s = s - A(0, 1, 1) * x(1, 1);
+  return mk == 0 ? T(0) : s;
+}
+
+// The same in flat form: AF(k, d) and X(d) at flat offset d = dj * n + di
+// of a row-major array with row length n.
 template <typename T, typename AF, typename X>
 __device__ __forceinline__ T offdiag_f(int n, AF A, X x) {
-  T s = A(0, 0) * x(-n - 1);
-  s = s + A(1, 0) * x(-n);
-  s = s + A(2, 0) * x(-n + 1);
-  s = s + A(3, 0) * x(-1);
-  s = s + A(3, 1) * x(1);
-  s = s + A(2, n - 1) * x(n - 1);
-  s = s + A(1, n) * x(n);
-  s = s + A(0, n + 1) * x(n + 1);
-  return s;
+  return offdiag_2d<T>([&](int k, int dj, int di) { return A(k, (ptrdiff_t)dj * n + di); },
+                       [&](int dj, int di) { return x(dj * n + di); });
 }
-
-// one damped-Jacobi update (core/gmg/fortran_multigrid.f90:66-85)
 template <typename T, typename AF, typename X>
 __device__ __forceinline__ T jacobi_f(int8_t mk, int n, T bc, T c1, T c2, AF A, X x) {
-  if (mk == 0) return T(0);
-  const T c3 = c1 / fabs(A(4, 0));
-  return x(0) * c2 + c3 * (offdiag_f<T>(n, A, x) - bc);
+  return jacobi_2d<T>(mk, bc, c1, c2,
+                      [&](int k, int dj, int di) { return A(k, (ptrdiff_t)dj * n + di); },
+                      [&](int dj, int di) { return x(dj * n + di); });
 }
-
-// r = b - A x (core/gmg/fortran_multigrid.f90:340-357)
+template <typename T, typename AF, typename X>
+__device__ __forceinline__ T jacobi_c3(int n, T bc, T c3, T c2, AF A, X x) {
+  return jacobi_c3_2d<T>(bc, c3, c2,
+                         [&](int k, int dj, int di) { return A(k, (ptrdiff_t)dj * n + di); },
+                         [&](int dj, int di) { return x(dj * n + di); });
+}
 template <typename T, typename AF, typename X>
 __device__ __forceinline__ T residual_f(int8_t mk, int n, T bc, AF A, X x) {
-  if (mk == 0) return T(0);
-  T s = bc - A(0, 0) * x(-n - 1);
-  s = s - A(1, 0) * x(-n);
-  s = s - A(2, 0) * x(-n + 1);
-  s = s - A(3, 0) * x(-1);
-  s = s - A(4, 0) * x(0);
-  s = s - A(3, 1) * x(1);
-  s = s - A(2, n - 1) * x(n - 1);
-  s = s - A(1, n) * x(n);
-  s = s - A(0, n + 1) * x(n + 1);
-  return s;
+  return residual_2d<T>(mk, bc,
+                        [&](int k, int dj, int di) { return A(k, (ptrdiff_t)dj * n + di); },
+                        [&](int dj, int di) { return x(dj * n + di); });
 }
 
 // explicit planes of stride P at cell c
@@ -151,18 +189,14 @@ struct ABits {
   }
 };
 
-// the same weights from a tile of stencil bits: `cell` points at the cell's
-// byte, `pitch` is the tile's row length, `n` the array's (flat offsets d
-// of the operator are row-major in the array: d = dj * n + di)
-template <typename T>
-struct ABitsTile {
+// the same weights from a tile of stencil bits (two-dimensional form):
+// `cell` points at the cell's byte in a tile of row length PITCH
+template <typename T, int PITCH>
+struct ABitsTile2 {
   const uint8_t* cell;
-  int pitch, n;
   const T* w;
-  __device__ __forceinline__ T operator()(int k, ptrdiff_t d) const {
-    const int dj = d > 1 ? 1 : (d < -1 ? -1 : 0);
-    const int di = (int)(d - (ptrdiff_t)dj * n);
-    return ((cell[dj * pitch + di] >> k) & 1) ? w[k] : T(0);
+  __device__ __forceinline__ T operator()(int k, int dj, int di) const {
+    return ((cell[dj * PITCH + di] >> k) & 1) ? w[k] : T(0);
   }
 };
 
@@ -193,9 +227,8 @@ __device__ __forceinline__ T restrict_r(R r) {
 // nh = 3): coarse (J2,I2) = ((j+2)>>1, (i+2)>>1); row/column parity picks
 // the injected, edge (mask-weighted pair) or centre (mask-weighted quad)
 // form.  The 1/3 weight is the float32 literal of line 439 widened.
-template <typename T>
-__device__ __forceinline__ T interp_at(const int8_t* msk1, int n1, const int8_t* msk2,
-                                       const T* x2, int n2, int j, int i) {
+template <typename T, typename PM1, typename PM2, typename PX>
+__device__ __forceinline__ T interp_at(PM1 msk1, int n1, PM2 msk2, PX x2, int n2, int j, int i) {
   if (msk1[(size_t)j * n1 + i] <= 0) return T(0);
   const int J2 = (j + 2) >> 1, I2 = (i + 2) >> 1;
   const size_t k = (size_t)J2 * n2 + I2;
@@ -214,6 +247,55 @@ __device__ __forceinline__ T interp_at(const int8_t* msk1, int n1, const int8_t*
   return w * (a + x2[k + 1] + x2[k + n2] + x2[k + n2 + 1]);
 }
 
+// interp_at in two steps: IpOps holds every operand of one fine point,
+// loaded up front (no load waits on the masks; all addresses in bounds for
+// fine rows/columns in [1, m1-2] x [1, n1-2]); ip_eval applies the same
+// expressions as interp_at.
+template <typename T>
+struct IpOps {
+  T a, b, c, d;
+  int8_t m1, sa, sb, sc, sd;
+  int par;  // bit 0: column parity, bit 1: row parity
+};
+template <typename T>
+__device__ __forceinline__ IpOps<T> ip_load(const int8_t* msk1, int n1, const int8_t* msk2,
+                                            const T* x2, int n2, int j, int i) {
+  IpOps<T> o;
+  const int J2 = (j + 2) >> 1, I2 = (i + 2) >> 1;
+  const size_t k = (size_t)J2 * n2 + I2;
+  o.m1 = msk1[(size_t)j * n1 + i];
+  o.a = x2[k];
+  o.b = x2[k + 1];
+  o.c = x2[k + n2];
+  o.d = x2[k + n2 + 1];
+  o.sa = msk2[k];
+  o.sb = msk2[k + 1];
+  o.sc = msk2[k + n2];
+  o.sd = msk2[k + n2 + 1];
+  o.par = (i & 1) | ((j & 1) << 1);
+  return o;
+}
+template <typename T>
+__device__ __forceinline__ T ip_eval(const IpOps<T>& o) {
+  auto c2 = [](int s) { return s == 2 ? T(0.5) : (s == 1 ? T(1) : T(0)); };
+  const int pj = o.par >> 1, pi = o.par & 1;
+  T r;
+  if (!pj && !pi) {
+    r = o.a;
+  } else if (!pj) {
+    r = (o.a + o.b) * c2(o.sa + o.sb);
+  } else if (!pi) {
+    r = (o.a + o.c) * c2(o.sa + o.sc);
+  } else {
+    const int s4 = o.sa + o.sb + o.sc + o.sd;
+    const T w = s4 == 4 ? T(0.25)
+                        : (s4 == 3 ? T((double)0.3333333333333333333333333333f)
+                                   : (s4 == 2 ? T(0.5) : (s4 == 1 ? T(1) : T(0))));
+    r = w * (o.a + o.b + o.c + o.d);
+  }
+  return o.m1 <= 0 ? T(0) : r;
+}
+
 // periodic image index of a halo row/column
 __device__ __forceinline__ int img(int j, int m) {
   const int M = m - 2 * NH;
@@ -222,23 +304,30 @@ __device__ __forceinline__ int img(int j, int m) {
 
 // Store at interior (J,I) and its halo images, multiplying each stored copy
 // by the mask at its own location when mk != nullptr (psi * mskp of
-// core/operators.py:481 applied to a filled array).
-template <typename T>
-__device__ __forceinline__ void store_img_mask(T* a, const int8_t* mk, int J, int I, T v, int m,
+// core/operators.py:481 applied to a filled array).  A row has an image
+// above (J + M) when J < 2 NH and one below (J - M) when J >= m - 2 NH, both
+// for interiors narrower than 2 NH; likewise for columns.  Straight-line
+// predicated stores.
+template <typename T, typename PA>
+__device__ __forceinline__ void store_img_mask(PA a, const int8_t* mk, int J, int I, T v, int m,
                                                int n) {
+  auto put = [&](int j, int i) {
+    const size_t c = at(j, i, n);
+    a[c] = mk ? v * T(mk[c]) : v;
+  };
+  put(J, I);
+  if (J >= 2 * NH && J < m - 2 * NH && I >= 2 * NH && I < n - 2 * NH) return;  // no image
   const int M = m - 2 * NH, N = n - 2 * NH;
-  int jc[3], ic[3], nj = 1, ni = 1;
-  jc[0] = J;
-  ic[0] = I;
-  if (J + M < m && J + M >= m - NH) jc[nj++] = J + M;
-  if (J - M >= 0 && J - M < NH) jc[nj++] = J - M;
-  if (I + N < n && I + N >= n - NH) ic[ni++] = I + N;
-  if (I - N >= 0 && I - N < NH) ic[ni++] = I - N;
-  for (int p = 0; p < nj; p++)
-    for (int q = 0; q < ni; q++) {
-      const size_t c = at(jc[p], ic[q], n);
-      a[c] = mk ? v * T(mk[c]) : v;
-    }
+  const bool jp = J + M < m && J + M >= m - NH, jm = J - M >= 0 && J - M < NH;
+  const bool ip = I + N < n && I + N >= n - NH, im = I - N >= 0 && I - N < NH;
+  if (jp) put(J + M, I);
+  if (jm) put(J - M, I);
+  if (ip) put(J, I + N);
+  if (im) put(J, I - N);
+  if (jp && ip) put(J + M, I + N);
+  if (jp && im) put(J + M, I - N);
+  if (jm && ip) put(J - M, I + N);
+  if (jm && im) put(J - M, I - N);
 }
 
 }  // namespace f2d

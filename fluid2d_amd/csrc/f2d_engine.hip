@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <functional>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fluid2d_amd.h"
@@ -74,6 +75,15 @@ struct Stage {
       default: return x[q] + c * (double)k1[q];
     }
   }
+  // the same with the kind fixed at compile time (no branch between loads)
+  template <int KIND>
+  __device__ __forceinline__ double at(size_t q) const {
+    if constexpr (KIND == 0) return x[q];
+    if constexpr (KIND == 1) return x[q] + c * (double)k0[q];
+    if constexpr (KIND == 2) return x[q] + c * ((double)k0[q] + (double)k1[q]);
+    if constexpr (KIND == 3) return x[q] + c * (double)k1[q];
+    return 0.;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -93,6 +103,24 @@ struct Stage {
 // FP64 and multiply in FP32.
 // ---------------------------------------------------------------------------
 constexpr int QW = TX + 6, QH = TY + 6;
+
+// Load phase of a 256-thread tile: element q = tid + 256 k for k < IT,
+// every load issued before the first store (load(q) sees q clamped to
+// [0, N), store runs for q < N only).
+template <int IT, typename LD, typename ST>
+__device__ __forceinline__ void tile_load(int tid, int N, LD load, ST store) {
+  decltype(load(0)) v[IT];
+#pragma unroll
+  for (int k = 0; k < IT; k++) v[k] = load(min(tid + 256 * k, N - 1));
+#pragma unroll
+  for (int k = 0; k < IT; k++)
+    if (tid + 256 * k < N) store(tid + 256 * k, v[k]);
+}
+
+struct TrVal {
+  double q0, q1;
+  int8_t m;
+};
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
@@ -128,25 +156,59 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   const int ntr = in.model == 0 ? 1 : 2;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
   const int y0 = tk.y0, x0 = tk.x0;
-  // tiles: tracer rows [y0-3, y0+TY+2] x cols [x0-3, x0+TX+2]
-  for (int q = tid; q < QH * QW; q += nt) {
-    const int r = q / QW, c = q % QW;
-    const int j = y0 - 3 + r, i = x0 - 3 + c;
-    const bool ok = j < m && i < n;
-    const size_t e = ok ? at(j, i, n) : 0;
-    ms[q] = ok ? msk[e] : 0;
-    for (int t = 0; t < ntr; t++) qs[t][q] = ok ? S((t == 0 ? 0 : 4 * F) + e) : 0.;
-  }
-  for (int q = tid; q < TY * (TX + 1); q += nt) {  // u on rows y0.., cols x0-1..
-    const int r = q / (TX + 1), c = q % (TX + 1);
-    const int j = y0 + r, i = x0 - 1 + c;
-    us[q] = (j < m && i < n) ? S(2 * F + at(j, i, n)) : 0.;
-  }
-  for (int q = tid; q < (TY + 1) * TX; q += nt) {  // v on rows y0-1.., cols x0..
-    const int r = q / TX, c = q % TX;
-    const int j = y0 - 1 + r, i = x0 + c;
-    vs[q] = (j < m && i < n) ? S(3 * F + at(j, i, n)) : 0.;
-  }
+  // tiles: tracer rows [y0-3, y0+TY+2] x cols [x0-3, x0+TX+2]; each tile's
+  // loads are issued together (compile-time trip counts, clamped in-bounds
+  // addresses), then stored
+  (void)nt;
+  auto load_tiles = [&](auto kind, auto ntrc) {
+    constexpr int K = decltype(kind)::value;
+    constexpr int NTR = decltype(ntrc)::value;
+    tile_load<(QH * QW + 255) / 256>(tid, QH * QW, [&](int q) {
+      const int r = q / QW, c = q - r * QW;
+      const int j = y0 - 3 + r, i = x0 - 3 + c;
+      const bool ok = j < m && i < n;
+      const size_t e = ok ? at(j, i, n) : 0;
+      TrVal v;
+      v.m = msk[e];
+      v.q0 = S.template at<K>(e);
+      v.q1 = NTR > 1 ? S.template at<K>(4 * F + e) : 0.;
+      if (!ok) {
+        v.m = 0;
+        v.q0 = v.q1 = 0.;
+      }
+      return v;
+    }, [&](int q, const TrVal& v) {
+      ms[q] = v.m;
+      qs[0][q] = v.q0;
+      qs[1][q] = v.q1;
+    });
+    tile_load<(TY * (TX + 1) + 255) / 256>(tid, TY * (TX + 1), [&](int q) {
+      const int r = q / (TX + 1), c = q - r * (TX + 1);  // u on rows y0.., cols x0-1..
+      const int j = y0 + r, i = x0 - 1 + c;
+      const bool ok = j < m && i < n;
+      const double v = S.template at<K>(2 * F + (ok ? at(j, i, n) : 0));
+      return ok ? v : 0.;
+    }, [&](int q, double v) { us[q] = v; });
+    tile_load<((TY + 1) * TX + 255) / 256>(tid, (TY + 1) * TX, [&](int q) {
+      const int r = q / TX, c = q - r * TX;  // v on rows y0-1.., cols x0..
+      const int j = y0 - 1 + r, i = x0 + c;
+      const bool ok = j < m && i < n;
+      const double v = S.template at<K>(3 * F + (ok ? at(j, i, n) : 0));
+      return ok ? v : 0.;
+    }, [&](int q, double v) { vs[q] = v; });
+  };
+  auto by_kind = [&](auto ntrc) {
+    switch (S.kind) {
+      case 0: load_tiles(std::integral_constant<int, 0>(), ntrc); break;
+      case 1: load_tiles(std::integral_constant<int, 1>(), ntrc); break;
+      case 2: load_tiles(std::integral_constant<int, 2>(), ntrc); break;
+      default: load_tiles(std::integral_constant<int, 3>(), ntrc); break;
+    }
+  };
+  if (ntr > 1)
+    by_kind(std::integral_constant<int, 2>());
+  else
+    by_kind(std::integral_constant<int, 1>());
   __syncthreads();
   // advection parameters (core/fortran_advection.f90:46-60), cst[3] = umax
   UpParams<double> p;
@@ -283,9 +345,18 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
                                                 const InstD* __restrict__ I, T* Q,
                                                 const int8_t* __restrict__ B,
                                                 const DynD* __restrict__ dyn, int pred) {
-  __shared__ T xs[TYV + 4][TX + 4];
-  __shared__ T j1[TYV + 2][TX + 2];
-  __shared__ uint8_t sbs[TYV + 4][TX + 4];  // stencil bits (finest level)
+  // Block (64, 4).  Each phase first issues the global loads of all its
+  // iterations (compile-time trip counts, clamped in-bounds addresses, no
+  // branch between the loads), then computes: one memory round trip per
+  // phase instead of one per iteration.
+  constexpr int NT = 256;
+  constexpr int XW = TX + 4, XH = TYV + 4, JW = TX + 2, JH = TYV + 2;
+  constexpr int N1 = XH * XW, IT1 = (N1 + NT - 1) / NT;
+  constexpr int N2 = JH * JW, IT2 = (N2 + NT - 1) / NT;
+  constexpr int IT3 = TYV / 4;
+  __shared__ T xs[XH][XW];
+  __shared__ T j1[JH][JW];
+  __shared__ uint8_t sbs[XH][XW];  // stencil bits (finest level)
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
   const int MODE = tk.mode;
@@ -298,78 +369,150 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   const T* A = Q + L.A;
   const T* b = Q + L.b;
   const T* xin = Q + tk.xin;
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int y0 = tk.y0, x0 = tk.x0;
   const bool fly = tk.lev == 0 && in.a0fly;
   const uint8_t* sb = (const uint8_t*)(B + in.sb0);
-  // x (and the stencil bits) on rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1],
-  // clipped to [1, m-2]
-  for (int q = tid; q < (TYV + 4) * (TX + 4); q += nt) {
-    const int rr = q / (TX + 4), cc = q % (TX + 4);
-    const int j = y0 - 2 + rr, i = x0 - 2 + cc;
-    T v = T(0);
-    uint8_t bits = 0;
-    if (j <= m - 2 && i <= n - 2) {
-      if (MODE == 0 || MODE == 2) v = xin[at(j, i, n)];
-      if (MODE == 2 || MODE == 3) {
-        const LevD& C = in.L[tk.lev + 1];
-        const T r = interp_at<T>(msk, n, B + C.msk, Q + tk.xc, C.n, j, i);
-        v = MODE == 2 ? v + r : r;
-      }
-      if (fly) bits = sb[at(j, i, n)];
-    }
-    xs[rr][cc] = v;
-    sbs[rr][cc] = bits;
+  const bool rdx = MODE == 0 || MODE == 2, rdi = MODE == 2 || MODE == 3;
+  const LevD& C = in.L[tk.lev + (rdi ? 1 : 0)];
+  const int8_t* msk2 = B + C.msk;
+  const T* xc = Q + tk.xc;
+  int full_tile = 0;
+  // right-hand sides of both sweeps, loaded with the tile (one round trip)
+  size_t cc2[IT2];
+  bool ok2[IT2];
+  T bv[IT2];
+#pragma unroll
+  for (int it = 0; it < IT2; it++) {
+    const int q = tid + it * NT;
+    const int rr = q / JW, cc = q - rr * JW;
+    const int j = y0 - 1 + rr, i = x0 - 1 + cc;
+    ok2[it] = q < N2 && j <= m - NH && i <= n - NH;
+    cc2[it] = ok2[it] ? at(j, i, n) : 0;
   }
-  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT2; it++) bv[it] = b[cc2[it]];
+  const int Ic = x0 + threadIdx.x;
+  size_t c3[IT3];
+  bool ok3[IT3];
+  T bv3[IT3];
+#pragma unroll
+  for (int k = 0; k < IT3; k++) {
+    const int J = y0 + threadIdx.y + 4 * k;
+    ok3[k] = J < m - NH && Ic < n - NH;
+    c3[k] = ok3[k] ? at(J, Ic, n) : 0;
+    bv3[k] = b[c3[k]];
+  }
+  // phase 1: x (and the stencil bits) on rows [y0-2, y0+TY+1] x cols
+  // [x0-2, x0+TX+1], clipped to [1, m-2]
+  {
+    int jj[IT1], ii[IT1];
+    bool ok[IT1];
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+      const int q = tid + it * NT;
+      const int rr = q / XW, cc = q - rr * XW;
+      const int j = y0 - 2 + rr, i = x0 - 2 + cc;
+      ok[it] = q < N1 && j <= m - 2 && i <= n - 2;
+      jj[it] = ok[it] ? j : 1;
+      ii[it] = ok[it] ? i : 1;
+    }
+    T v[IT1];
+    uint8_t bits[IT1];
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+      v[it] = T(0);
+      bits[it] = 0;
+    }
+    if (rdx) {
+#pragma unroll
+      for (int it = 0; it < IT1; it++) v[it] = xin[at(jj[it], ii[it], n)];
+    }
+    if (fly) {
+#pragma unroll
+      for (int it = 0; it < IT1; it++) bits[it] = sb[at(jj[it], ii[it], n)];
+    }
+    if (rdi) {
+      IpOps<T> op[IT1];
+#pragma unroll
+      for (int it = 0; it < IT1; it++) op[it] = ip_load<T>(msk, n, msk2, xc, C.n, jj[it], ii[it]);
+#pragma unroll
+      for (int it = 0; it < IT1; it++) {
+        const T r = ip_eval<T>(op[it]);
+        v[it] = MODE == 2 ? v[it] + r : r;
+      }
+    }
+    bool full = true;
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+      const int q = tid + it * NT;
+      if (q < N1) {
+        const int rr = q / XW, cc = q - rr * XW;
+        xs[rr][cc] = ok[it] ? v[it] : T(0);
+        sbs[rr][cc] = ok[it] ? bits[it] : (uint8_t)0;
+        full = full && ok[it] && bits[it] == 31;
+      }
+    }
+    // every cell of the tile has the full 9-point stencil: constant weights
+    full_tile = __syncthreads_and(fly && full);
+  }
   const T c1 = T(in.omega), c2 = T(1. - in.omega);
   T* xout = Q + tk.xout;
   const int8_t* mp = final_mask ? B + in.mskp : nullptr;
   T w[5];
   for (int k = 0; k < 5; k++) w[k] = T(in.w0[k]);
-  // Aat(c, r, q): operator accessor of the cell at global c, tile (r, q) of
-  // the x/bits tiles; Mk(c, r, q): its mask
-  auto sweeps = [&](auto Aat, auto Mk) {
-    // first sweep on rows [y0-1, y0+TY] x cols [x0-1, x0+TX], clipped to [.., m-NH]
-    for (int q = tid; q < (TYV + 2) * (TX + 2); q += nt) {
-      const int rr = q / (TX + 2), cc = q % (TX + 2);
-      const int j = y0 - 1 + rr, i = x0 - 1 + cc;
-      T v = T(0);
-      if (j <= m - NH && i <= n - NH) {
-        const size_t c = at(j, i, n);
-        v = jacobi_f<T>(Mk(c, rr + 1, cc + 1), n, b[c], c1, c2, Aat(c, rr + 1, cc + 1),
-                        [&](int dd) {
-                          const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-                          const int di = dd - dj * n;
-                          return xs[rr + 1 + dj][cc + 1 + di];
-                        });
+  // Jac(c, r, q, b, X): damped-Jacobi value of the cell at global c, tile
+  // (r, q) of the x/bits tiles, right-hand side b, neighbour accessor X
+  auto sweeps = [&](auto Jac) {
+    // phase 2: first sweep on rows [y0-1, y0+TY] x cols [x0-1, x0+TX],
+    // clipped to [.., m-NH]
+    {
+#pragma unroll
+      for (int it = 0; it < IT2; it++) {
+        const int q = tid + it * NT;
+        if (q < N2) {
+          const int rr = q / JW, cc = q - rr * JW;
+          T v = T(0);
+          if (ok2[it]) {
+            const size_t c = cc2[it];
+            v = Jac(c, rr + 1, cc + 1, bv[it],
+                    [&](int dj, int di) { return xs[rr + 1 + dj][cc + 1 + di]; });
+          }
+          j1[rr][cc] = v;
+        }
       }
-      j1[rr][cc] = v;
     }
     __syncthreads();
-    for (int rr = threadIdx.y; rr < TYV; rr += blockDim.y) {
-      const int J = y0 + rr, Ic = x0 + threadIdx.x;
-      if (J >= m - NH || Ic >= n - NH) continue;
-      const int cc = threadIdx.x;
-      const size_t c = at(J, Ic, n);
-      const T v = jacobi_f<T>(Mk(c, rr + 2, cc + 2), n, b[c], c1, c2, Aat(c, rr + 2, cc + 2),
-                              [&](int dd) {
-                                const int dj = dd > 1 ? 1 : (dd < -1 ? -1 : 0);
-                                const int di = dd - dj * n;
-                                return j1[rr + 1 + dj][cc + 1 + di];
-                              });
-      store_img_mask<T>(xout, mp, J, Ic, v, m, n);
+    // phase 3: second sweep on the tile interior, stored with its images
+#pragma unroll
+    for (int k = 0; k < IT3; k++) {
+      if (!ok3[k]) continue;
+      const int rr = threadIdx.y + 4 * k, cc = threadIdx.x;
+      const size_t c = c3[k];
+      const T v = Jac(c, rr + 2, cc + 2, bv3[k],
+                      [&](int dj, int di) { return j1[rr + 1 + dj][cc + 1 + di]; });
+      store_img_mask<T>(xout, mp, y0 + rr, Ic, v, m, n);
     }
   };
-  if (fly)
-    sweeps(
-        [&](size_t, int r, int q) {
-          return ABitsTile<T>{&sbs[r][q], TX + 4, n, w};
-        },
-        [&](size_t, int r, int q) { return (int8_t)((sbs[r][q] >> 4) & 1); });
-  else
-    sweeps([&](size_t c, int, int) { return APlanes<T>{A, PL, c}; },
-           [&](size_t c, int, int) { return msk[c]; });
+  if (full_tile) {
+    // A_k = w_k everywhere, the mask is 1, and c1 / |A_5| is one value
+    const T c3 = c1 / fabs(w[4]);
+    sweeps([&](size_t, int, int, T bc, auto X) {
+      return jacobi_c3_2d<T>(bc, c3, c2, [&](int k, int, int) { return w[k]; }, X);
+    });
+  } else if (fly) {
+    sweeps([&](size_t, int r, int q, T bc, auto X) {
+      return jacobi_2d<T>((int8_t)((sbs[r][q] >> 4) & 1), bc, c1, c2,
+                          ABitsTile2<T, XW>{&sbs[r][q], w}, X);
+    });
+  } else {
+    sweeps([&](size_t c, int, int, T bc, auto X) {
+      const APlanes<T> Ap{A, PL, c};
+      return jacobi_2d<T>(msk[c], bc, c1, c2,
+                          [&](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * n + di); },
+                          X);
+    });
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -416,7 +559,88 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   for (int k = 0; k < 5; k++) w[k] = R(in.w0[k]);
   const uint8_t* sb = (const uint8_t*)(B + in.sb0);
   double nr = 0., nb = 0.;
-  for (int q = tid; q < RY * RX; q += nt) {
+  // Tiles whose fine points are all interior (no periodic images involved)
+  // read x (and the stencil bits) from LDS tiles loaded in one round trip;
+  // the others take the general path below.  Both evaluate the same
+  // expressions in the same order.
+  constexpr int NT = 256, XRW = RX + 2, XRH = RY + 2, NX = XRW * XRH;
+  constexpr int ITX = (NX + NT - 1) / NT, ITP = (RY * RX + NT - 1) / NT;
+  __shared__ R xsr[XRH][XRW];
+  __shared__ uint8_t sbr[XRH][XRW];
+  const bool fly = tk.lev == 0 && in.a0fly;
+  const bool interior = fy0 >= NH && fx0 >= NH && fy0 + RY - 1 <= m1 - NH - 1 &&
+                        fx0 + RX - 1 <= n1 - NH - 1;
+  int full = 0;
+  if (interior) {
+    if (!(flags & 2)) {
+      R xv[ITX];
+      uint8_t bv[ITX];
+      bool allfull = true;
+#pragma unroll
+      for (int it = 0; it < ITX; it++) {
+        const int q = min(tid + it * NT, NX - 1);
+        const int rr = q / XRW, cc = q - rr * XRW;
+        const size_t c = at(fy0 - 1 + rr, fx0 - 1 + cc, n1);
+        xv[it] = x[c];
+        bv[it] = fly ? sb[c] : (uint8_t)0;
+      }
+#pragma unroll
+      for (int it = 0; it < ITX; it++) {
+        const int q = tid + it * NT;
+        if (q < NX) {
+          const int rr = q / XRW, cc = q - rr * XRW;
+          xsr[rr][cc] = xv[it];
+          sbr[rr][cc] = bv[it];
+          allfull = allfull && bv[it] == 31;
+        }
+      }
+      // the whole tile has the full stencil: constant weights
+      full = __syncthreads_and(fly && allfull);
+    }
+    R bq[ITP];
+    int8_t mq[ITP];
+#pragma unroll
+    for (int it = 0; it < ITP; it++) {
+      const int q = min(tid + it * NT, RY * RX - 1);
+      const int rr = q / RX, cc = q - rr * RX;
+      const size_t c = at(fy0 + rr, fx0 + cc, n1);
+      bq[it] = b[c];
+      mq[it] = msk1[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITP; it++) {
+      const int q = tid + it * NT;
+      if (q >= RY * RX) continue;
+      const int rr = q / RX, cc = q - rr * RX;
+      const int jf = fy0 + rr, if_ = fx0 + cc;
+      const size_t c = at(jf, if_, n1);
+      auto xl = [&](int dj, int di) { return xsr[rr + 1 + dj][cc + 1 + di]; };
+      R v;
+      if (flags & 2) {
+        v = bq[it];
+      } else if (full) {
+        v = residual_2d<R>(mq[it], bq[it], [&](int k, int, int) { return w[k]; }, xl);
+      } else if (fly) {
+        v = residual_2d<R>(mq[it], bq[it], ABitsTile2<R, XRW>{&sbr[rr + 1][cc + 1], w}, xl);
+      } else {
+        const APlanes<R> Ap{A, PL, c};
+        v = residual_2d<R>(mq[it], bq[it],
+                           [&](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * n1 + di); },
+                           xl);
+      }
+      rs[rr][cc] = v;
+      if (rr < 2 * TY2V && cc < 2 * TX2) {
+        if (flags & 1) store_img_mask<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
+        if (mq[it] != 0) {
+          nr += (double)v * (double)v;
+          const double bb = (double)bq[it];
+          nb += bb * bb;
+        }
+      }
+    }
+  }
+  for (int q = tid; q < RY * RX && !interior; q += nt) {
     const int rr = q / RX, cc = q % RX;
     const int jf = fy0 + rr, if_ = fx0 + cc;
     if (jf > m1 - NH || if_ > n1 - NH) continue;  // beyond the last needed point
@@ -481,11 +705,17 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
 // above); mode 1: F-cycle tail (hierarchy.py:131-151 from nfine).
 // Input b(nfine) from global, output x(nfine) (filled) to tk.xout.
 // ---------------------------------------------------------------------------
-template <typename T>
+// LDS (address space 3) pointers: accesses compile to ds_* instructions
+// even when the pointers themselves are read from the LDS level table
+#define LDSP __attribute__((address_space(3)))
+// T: arithmetic type (FP64); TA: storage type of the operator (the pool's,
+// FP32 in mixed mode, widened exactly on use)
+template <typename T, typename TA = T>
 struct CL {
   int m, n;
-  int8_t* msk;
-  T *A, *x, *b, *t;  // the residual reuses t (never live at the same time)
+  LDSP int8_t* msk;
+  LDSP TA* A;
+  LDSP T *x, *b, *t;  // the residual reuses t (never live at the same time)
 };
 
 // Phases over a level: levels of at most SMALL cells run on wavefront 0
@@ -493,20 +723,19 @@ struct CL {
 // wavefront execute in order); larger levels use the whole workgroup and a
 // workgroup barrier.  `prev_small` (uniform) inserts the workgroup barrier
 // needed when a large phase follows small ones.
-constexpr int SMALL = 256;
-
 struct Sync {
   int tid, nt;
+  int small_cells;  // levels of at most this many cells run on wavefront 0
   bool prev_small;
   __device__ __forceinline__ bool begin(int cells) {
-    const bool small = cells <= SMALL;
+    const bool small = cells <= small_cells;
     if (!small && prev_small) __syncthreads();
     prev_small = small;
     return small ? tid < 64 : true;
   }
-  __device__ __forceinline__ int stride(int cells) const { return cells <= SMALL ? 64 : nt; }
+  __device__ __forceinline__ int stride(int cells) const { return cells <= small_cells ? 64 : nt; }
   __device__ __forceinline__ void end(int cells) {
-    if (cells <= SMALL) {
+    if (cells <= small_cells) {
       if (tid < 64) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -517,18 +746,30 @@ struct Sync {
   }
 };
 
-template <typename T>
-__device__ void c_smooth(const CL<T>& L, T c1, T c2, Sync& sy) {
+// q / w for the small q, w of the coarse levels (q < 2^16, w < 2^10):
+// (q + 0.5) / w is at least 0.5 / w from an integer and the float product
+// is within 1e-4 of it, so the truncation is exact
+__device__ __forceinline__ int qdiv(int q, int w) {
+  return (int)(((float)q + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+}
+
+// zero_in: the smoother's input x is zero (a level entered from above);
+// the zero is read, not stored, with the arithmetic of a stored zero
+template <typename T, typename TA>
+__device__ void c_smooth(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in = false) {
   const int m = L.m, n = L.n, cells = m * n;
   const size_t P = (size_t)m * n;
   const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2;
   if (sy.begin(cells)) {
     const int st = sy.stride(cells);
     for (int q = sy.tid; q < wj * wi; q += st) {
-      const int j = NH - 1 + q / wi, i = NH - 1 + q % wi;
+      const int qj = qdiv(q, wi);
+      const int j = NH - 1 + qj, i = NH - 1 + (q - qj * wi);
       const size_t c = at(j, i, n);
-      L.t[c] = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
-                           [&](int dd) { return L.x[c + dd]; });
+      auto Af = [&](int k, ptrdiff_t d) { return T(L.A[k * P + c + d]); };
+      L.t[c] = zero_in ? jacobi_f<T>(L.msk[c], n, L.b[c], c1, c2, Af, [&](int) { return T(0); })
+                       : jacobi_f<T>(L.msk[c], n, L.b[c], c1, c2, Af,
+                                     [&](int dd) { return L.x[c + dd]; });
     }
   }
   sy.end(cells);
@@ -536,43 +777,41 @@ __device__ void c_smooth(const CL<T>& L, T c1, T c2, Sync& sy) {
   if (sy.begin(cells)) {
     const int st = sy.stride(cells);
     for (int q = sy.tid; q < M * N; q += st) {
-      const int J = NH + q / N, I2 = NH + q % N;
+      const int qj = qdiv(q, N);
+      const int J = NH + qj, I2 = NH + (q - qj * N);
       const size_t c = at(J, I2, n);
-      const T v = jacobi_x<T>(L.msk[c], L.A, P, c, n, L.b[c], c1, c2,
-                              [&](int dd) { return L.t[c + dd]; });
+      auto Af = [&](int k, ptrdiff_t d) { return T(L.A[k * P + c + d]); };
+      const T v = jacobi_f<T>(L.msk[c], n, L.b[c], c1, c2, Af, [&](int dd) { return L.t[c + dd]; });
       store_img_mask<T>(L.x, nullptr, J, I2, v, m, n);
     }
   }
   sy.end(cells);
 }
-template <typename T>
-__device__ void c_zero(const CL<T>& L, Sync& sy) {
-  const int cnt = L.m * L.n;
-  if (sy.begin(cnt))
-    for (int q = sy.tid; q < cnt; q += sy.stride(cnt)) L.x[q] = T(0);
-  sy.end(cnt);
-}
-template <typename T>
-__device__ void c_resid_restrict(const CL<T>& F, const CL<T>& C, bool restrict_b, Sync& sy) {
+template <typename T, typename TA>
+__device__ void c_resid_restrict(const CL<T, TA>& F, const CL<T, TA>& C, bool restrict_b,
+                                 Sync& sy) {
   const int m = F.m, n = F.n, cells = m * n;
   const size_t P = (size_t)m * n;
   const int N = n - 2 * NH, M = m - 2 * NH;
   if (!restrict_b) {
     if (sy.begin(cells))
       for (int q = sy.tid; q < M * N; q += sy.stride(cells)) {
-        const int J = NH + q / N, I2 = NH + q % N;
+        const int qj = qdiv(q, N);
+        const int J = NH + qj, I2 = NH + (q - qj * N);
         const size_t c = at(J, I2, n);
-        const T v = residual_x<T>(F.msk[c], F.A, P, c, n, F.b[c],
+        const T v = residual_f<T>(F.msk[c], n, F.b[c],
+                                  [&](int k, ptrdiff_t d) { return T(F.A[k * P + c + d]); },
                                   [&](int dd) { return F.x[c + dd]; });
         store_img_mask<T>(F.t, nullptr, J, I2, v, m, n);
       }
     sy.end(cells);
   }
-  const T* src = restrict_b ? F.b : F.t;
+  LDSP T* src = restrict_b ? F.b : F.t;
   const int N2 = C.n - 2 * NH, M2 = C.m - 2 * NH, ccells = C.m * C.n;
   if (sy.begin(ccells))
     for (int q = sy.tid; q < M2 * N2; q += sy.stride(ccells)) {
-      const int J2 = NH + q / N2, I2 = NH + q % N2;
+      const int qj = qdiv(q, N2);
+      const int J2 = NH + qj, I2 = NH + (q - qj * N2);
       T v = T(0);
       if (C.msk[at(J2, I2, C.n)] != 0) {
         const size_t c = at(2 * J2 + 1 - NH, 2 * I2 + 1 - NH, n);
@@ -582,100 +821,128 @@ __device__ void c_resid_restrict(const CL<T>& F, const CL<T>& C, bool restrict_b
     }
   sy.end(ccells);
 }
-template <typename T>
-__device__ void c_interp(const CL<T>& F, const CL<T>& C, bool acc, Sync& sy) {
+template <typename T, typename TA>
+__device__ void c_interp(const CL<T, TA>& F, const CL<T, TA>& C, bool acc, Sync& sy) {
   const int cnt = F.m * F.n;
   if (sy.begin(cnt))
     for (int q = sy.tid; q < cnt; q += sy.stride(cnt)) {
-      const int j = q / F.n, i = q % F.n;
+      const int j = qdiv(q, F.n), i = q - j * F.n;
       const T r = interp_at<T>(F.msk, F.n, C.msk, C.x, C.n, j, i);
       F.x[q] = acc ? F.x[q] + r : r;
     }
   sy.end(cnt);
 }
 
-template <typename T>
-__device__ void c_vcycle(CL<T>* Ls, int lev1, int deep, bool zero_first, T c1, T c2, Sync& sy) {
+// V-cycle from level lev1 down to level deep of the coarse part
+// (hierarchy.py:98-126)
+template <typename T, typename TA>
+__device__ void c_vcycle(const CL<T, TA>* Ls, int lev1, int deep, bool zero_first, T c1, T c2,
+                         Sync& sy) {
   for (int l = lev1; l < deep; l++) {
-    if (l > lev1 || zero_first) c_zero(Ls[l], sy);
-    c_smooth(Ls[l], c1, c2, sy);
+    c_smooth(Ls[l], c1, c2, sy, l > lev1 || zero_first);
     c_resid_restrict(Ls[l], Ls[l + 1], false, sy);
   }
-  c_zero(Ls[deep], sy);
-  for (int k = 0; k < 16; k++) c_smooth(Ls[deep], c1, c2, sy);
+  for (int k = 0; k < 16; k++) c_smooth(Ls[deep], c1, c2, sy, k == 0);
   for (int l = deep - 1; l >= lev1; l--) {
     c_interp(Ls[l], Ls[l + 1], true, sy);
     c_smooth(Ls[l], c1, c2, sy);
   }
 }
 
-// Operands come from the T pool and are solved in FP64 in LDS (the
-// coarsest-grid solve stays FP64 in mixed mode).
 constexpr int COARSE_THREADS = 1024;  // one workgroup per instance holds the CU's LDS anyway
+constexpr int MAXCL = 6;              // most levels the coarse kernel takes
 
+// Operands come from the T pool and are solved in FP64 in LDS (the
+// coarsest-grid solve stays FP64 in mixed mode).  One block per instance.
 template <typename T>
 __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restrict__ tasks,
                                                 const InstD* __restrict__ I, T* Q,
                                                 const int8_t* __restrict__ B,
-                                                const DynD* __restrict__ dyn, int pred, int mode) {
+                                                const DynD* __restrict__ dyn, int pred, int mode,
+                                                int small_cells) {
   typedef double C;
-  extern __shared__ __align__(16) unsigned char lds[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  LDSP unsigned char* lds = (LDSP unsigned char*)lds_raw;
   const Task tk = tasks[blockIdx.x];
   if (skip(dyn, tk.inst, pred)) return;
   const InstD& in = I[tk.inst];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int k0 = in.nfine, deep = in.nlev - 1, nl = deep - k0 + 1;
-  __shared__ CL<C> Ls[MAXLEV];
-  // carve LDS: per level A(5), x, b, t (FP64), then masks
+  __shared__ CL<C, T> Ls[MAXLEV];
+  // carve LDS, 16-byte aligned regions: per level A (5 planes, pool type),
+  // x, b, t (FP64); then masks
+  auto r16 = [](unsigned v) { return (v + 15u) & ~15u; };
   if (tid == 0) {
-    size_t off = 0;
+    unsigned off = 0;
     for (int l = 0; l < nl; l++) {
       const LevD& G = in.L[k0 + l];
-      const int sz = G.m * G.n;
+      const unsigned sz = (unsigned)(G.m * G.n);
       Ls[l].m = G.m;
       Ls[l].n = G.n;
-      Ls[l].A = (C*)(lds + off);
-      off += sizeof(C) * 5 * sz;
-      Ls[l].x = (C*)(lds + off);
+      Ls[l].A = (LDSP T*)(lds + off);
+      off += r16(sizeof(T) * 5 * sz);
+      Ls[l].x = (LDSP C*)(lds + off);
       off += sizeof(C) * sz;
-      Ls[l].b = (C*)(lds + off);
+      Ls[l].b = (LDSP C*)(lds + off);
       off += sizeof(C) * sz;
-      Ls[l].t = (C*)(lds + off);
+      Ls[l].t = (LDSP C*)(lds + off);
       off += sizeof(C) * sz;
     }
     for (int l = 0; l < nl; l++) {
       const LevD& G = in.L[k0 + l];
-      Ls[l].msk = (int8_t*)(lds + off);
-      off += (G.m * G.n + 15) & ~15;
+      Ls[l].msk = (LDSP int8_t*)(lds + off);
+      off += r16(G.m * G.n);
     }
   }
   __syncthreads();
-  for (int l = 0; l < nl; l++) {
-    const LevD& G = in.L[k0 + l];
-    const int sz = G.m * G.n;
-    const T* Ag = Q + G.A;
-    for (int q = tid; q < 5 * sz; q += nt) Ls[l].A[q] = (C)Ag[q];
-    const int8_t* mg = B + G.msk;
-    for (int q = tid; q < sz; q += nt) Ls[l].msk[q] = mg[q];
-    for (int q = tid; q < sz; q += nt) Ls[l].t[q] = C(0);
-  }
+  // operators and masks straight to LDS (global_load_lds, 16 bytes per
+  // lane, 1 KB per wavefront; the pools carry slack past their last array
+  // for the rounded-up tail), all issued before one wait
   {
-    const int sz = Ls[0].m * Ls[0].n;
+    const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+    auto dma = [&](LDSP void* dst, const void* src, int bytes) {
+      for (int c = wv; c * 1024 < bytes; c += nw) {
+        const int off = c * 1024 + lane * 16;
+        if (off < bytes)
+          __builtin_amdgcn_global_load_lds((const void*)((const char*)src + off),
+                                           (LDSP void*)((LDSP char*)dst + c * 1024), 16, 0, 0);
+      }
+    };
+    for (int l = 0; l < nl; l++) {
+      const LevD& G = in.L[k0 + l];
+      const int sz = G.m * G.n;
+      dma(Ls[l].A, Q + G.A, (int)(5 * sz * sizeof(T)));
+      dma(Ls[l].msk, B + G.msk, sz);
+    }
+    // the top level's rhs, widened to FP64
+    const int sz0 = Ls[0].m * Ls[0].n;
     const T* bg = Q + in.L[k0].b;
-    for (int q = tid; q < sz; q += nt) Ls[0].b[q] = (C)bg[q];
+    for (int base = tid; base < sz0; base += 8 * nt) {
+      T v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = bg[min(base + k * nt, sz0 - 1)];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (base + k * nt < sz0) Ls[0].b[base + k * nt] = (C)v[k];
+    }
+    for (int l = 0; l < nl; l++) {
+      const int sz = Ls[l].m * Ls[l].n;
+      for (int q = tid; q < sz; q += nt) Ls[l].t[q] = C(0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
   }
   __syncthreads();
   const C c1 = C(in.omega), c2 = C(1. - in.omega);
   Sync sy;
   sy.tid = tid;
   sy.nt = nt;
+  sy.small_cells = small_cells;
   sy.prev_small = false;
   if (mode == 0) {
     c_vcycle(Ls, 0, nl - 1, true, c1, c2, sy);
   } else {
     for (int l = 0; l < nl - 1; l++) c_resid_restrict(Ls[l], Ls[l + 1], true, sy);
-    c_zero(Ls[nl - 1], sy);
-    for (int k = 0; k < 16; k++) c_smooth(Ls[nl - 1], c1, c2, sy);
+    for (int k = 0; k < 16; k++) c_smooth(Ls[nl - 1], c1, c2, sy, k == 0);
     for (int l = nl - 2; l >= 0; l--) {
       c_interp(Ls[l], Ls[l + 1], false, sy);
       c_vcycle(Ls, l, nl - 1, false, c1, c2, sy);
@@ -1146,6 +1413,7 @@ struct Batch {
   double tol = 1e-11;
   size_t max_shmem = 0;
   int min_tiles = 0;           // fine-phase tile choice (Builder::pick_ty)
+  int coarse_wave_cells = 256; // coarse levels of at most this many cells run on one wavefront
   size_t coarse_budget = 0;    // LDS bytes for k_coarse (0: device maximum)
 };
 
@@ -1664,7 +1932,7 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
         break;
       case L_COARSE:
         hipLaunchKernelGGL(k_coarse<T>, dim3(g), dim3(COARSE_THREADS), L.shmem, b.s, TK, b.dI, Q, b.B,
-                           b.dD, L.pred, L.mode);
+                           b.dD, L.pred, L.mode, b.coarse_wave_cells);
         break;
       case L_ORTHO:
         if (L.flags == 1)
@@ -1739,6 +2007,22 @@ static void launch_bytes(const Batch& b, const Launch& L, double* alg, double* i
       const bool fly = t.lev == 0 && d.a0fly;
       im += fine * (rd * rz + 1. + ((L.flags & 2) ? 0. : (fly ? 1. : 5. * rz))) + cc * (tz + 1.) +
             ((L.flags & 1) ? fine * tz : 0.);
+    } else if (L.kind == L_RHS) {
+      // stage state (x + c k: 1, 2 or 3 reads per field) of the tracer(s)
+      // and (u, v), one tendency written per tracer
+      const int cj = std::min(TY, d.m - NH - t.y0), ci = std::min(TX, d.n - NH - t.x0);
+      const double cells = (double)std::max(cj, 0) * std::max(ci, 0);
+      const int ntr = d.model == 0 ? 1 : 2;
+      const int kr = L.stage == 0 ? 0 : ((d.tscheme == 0 && L.stage == 2) ? 2 : 1);
+      a += cells * ((ntr + 2) * (8. + kr * tz) + ntr * tz);
+      im += cells * ((ntr + 2) * (8. + kr * tz) + ntr * tz + 1.);
+    } else if (L.kind == L_COARSE) {
+      // rhs in, solution out at the first coarse level; the rest stays in LDS
+      const double cells = (double)lv.m * lv.n;
+      a += cells * 2. * tz;
+      double opb = 0.;
+      for (int l = t.lev; l < d.nlev; l++) opb += (double)d.L[l].m * d.L[l].n * (5. * tz + 1.);
+      im += cells * 2. * tz + opb;
     }
   }
   *alg = a;
@@ -1807,10 +2091,16 @@ static int build_programs(Batch& b) {
   return F2D_OK;
 }
 
-static size_t coarse_bytes(const InstD& d, int k0) {
+// LDS of the coarse kernel from level k0 down: per level the operator (5
+// planes of the pool type, tsz bytes) and x, b, t in FP64, then the masks,
+// each region rounded to 16 bytes
+static size_t coarse_bytes(const InstD& d, int k0, size_t tsz) {
+  auto r16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
   size_t sz = 0;
-  for (int l = k0; l < d.nlev; l++) sz += (size_t)d.L[l].m * d.L[l].n * (8 * sizeof(double));
-  for (int l = k0; l < d.nlev; l++) sz += ((size_t)d.L[l].m * d.L[l].n + 15) & ~(size_t)15;
+  for (int l = k0; l < d.nlev; l++) {
+    const size_t c = (size_t)d.L[l].m * d.L[l].n;
+    sz += r16(5 * c * tsz) + 3 * c * sizeof(double) + r16(c);
+  }
   return sz;
 }
 
@@ -1827,12 +2117,12 @@ static size_t lds_budget(int requested) {
 
 // first level of the coarse kernel: the largest set of coarse levels whose
 // operands fit the budget (at least the coarsest, at most all but level 0)
-static int first_coarse_level(const InstD& d, size_t budget, int* k0) {
+static int first_coarse_level(const InstD& d, size_t budget, size_t tsz, int* k0) {
   int k = d.nlev - 1;
-  while (k > 1 && coarse_bytes(d, k - 1) <= budget) k--;
-  if (coarse_bytes(d, k) > budget)
+  while (k > 1 && coarse_bytes(d, k - 1, tsz) <= budget && d.nlev - (k - 1) <= MAXCL) k--;
+  if (coarse_bytes(d, k, tsz) > budget)
     return efail(F2D_E_ARG, "coarse levels need %zu bytes of LDS, budget %zu",
-                 coarse_bytes(d, k), budget);
+                 coarse_bytes(d, k, tsz), budget);
   *k0 = k;
   return F2D_OK;
 }
@@ -1841,14 +2131,16 @@ static int first_coarse_level(const InstD& d, size_t budget, int* k0) {
 static int apply_tuning(Batch& b, const f2d_batch_tuning* t) {
   const size_t budget = lds_budget(t ? t->coarse_lds_budget : 0);
   std::vector<int> k0(b.ninst);
-  for (int i = 0; i < b.ninst; i++) ETRY(first_coarse_level(b.H[i].d, budget, &k0[i]));
+  const size_t tsz = b.prec == 0 ? sizeof(double) : sizeof(float);
+  for (int i = 0; i < b.ninst; i++) ETRY(first_coarse_level(b.H[i].d, budget, tsz, &k0[i]));
   size_t maxlds = 0;
   for (int i = 0; i < b.ninst; i++) {
     b.H[i].d.nfine = k0[i];
-    b.H[i].coarse_lds = coarse_bytes(b.H[i].d, k0[i]);
+    b.H[i].coarse_lds = coarse_bytes(b.H[i].d, k0[i], tsz);
     maxlds = std::max(maxlds, b.H[i].coarse_lds);
   }
   b.min_tiles = t ? std::max(0, t->min_tiles) : 0;
+  b.coarse_wave_cells = (t && t->coarse_wave_cells >= 0) ? t->coarse_wave_cells : 256;
   b.coarse_budget = budget;
   if (maxlds > 65536) {
     const void* fn = b.prec == 0 ? (const void*)k_coarse<double> : (const void*)k_coarse<float>;
@@ -1954,9 +2246,9 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
     if (nl < 2) return bail(efail(F2D_E_ARG, "grid must have at least two levels"));
     d.nlev = nl;
     int k0 = 0;
-    if (int rc = first_coarse_level(d, budget, &k0)) return bail(rc);
+    if (int rc = first_coarse_level(d, budget, tsz, &k0)) return bail(rc);
     d.nfine = k0;
-    h.coarse_lds = coarse_bytes(d, k0);
+    h.coarse_lds = coarse_bytes(d, k0, tsz);
     const size_t F = (size_t)d.m * d.n;
     d.state = takeP(d.nvar * F);
     d.work = takeP(F);
@@ -1995,13 +2287,15 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
   b->nB = nB;
   if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess)
     return bail(efail(F2D_E_HIP, "stream"));
+  // 256 bytes of slack after the T and byte pools: the coarse kernel's
+  // 16-byte LDS loads round the last array's tail up
+  nQ += 256 / tsz;
+  nB += 256;
   if (hipMalloc(&b->P, nP * sizeof(double)) != hipSuccess ||
-      hipMalloc(&b->Q, std::max<size_t>(nQ, 1) * tsz) != hipSuccess ||
-      hipMalloc(&b->B, std::max<size_t>(nB, 1)) != hipSuccess)
+      hipMalloc(&b->Q, nQ * tsz) != hipSuccess || hipMalloc(&b->B, nB) != hipSuccess)
     return bail(efail(F2D_E_NOMEM, "pools (%zu doubles, %zu reals, %zu bytes)", nP, nQ, nB));
   if (hipMemset(b->P, 0, nP * sizeof(double)) != hipSuccess ||
-      hipMemset(b->Q, 0, std::max<size_t>(nQ, 1) * tsz) != hipSuccess ||
-      hipMemset(b->B, 0, std::max<size_t>(nB, 1)) != hipSuccess)
+      hipMemset(b->Q, 0, nQ * tsz) != hipSuccess || hipMemset(b->B, 0, nB) != hipSuccess)
     return bail(efail(F2D_E_HIP, "memset pools"));
   // masks, operators: level hierarchy built by the multigrid set-up kernels
   for (int i = 0; i < ninst; i++) {
@@ -2127,6 +2421,7 @@ int f2d_batch_retune(f2d_batch* b, const f2d_batch_tuning* tuning) {
 int f2d_batch_tuning_info(f2d_batch* b, f2d_batch_tuning* out) {
   out->coarse_lds_budget = (int)b->coarse_budget;
   out->min_tiles = b->min_tiles;
+  out->coarse_wave_cells = b->coarse_wave_cells;
   return F2D_OK;
 }
 

@@ -1,0 +1,98 @@
+"""Summaries of rocprofv3 result databases (``--kernel-trace`` and
+``--pmc`` runs written as ``*_results.db``), used to produce the text
+summaries kept with the benchmark and the HBM-traffic figure of the bench's
+roofline.
+
+    python -m fluid2d_amd.profiling kernels DB [PATTERN]
+    python -m fluid2d_amd.profiling pmc DB COUNTER [PATTERN]
+    python -m fluid2d_amd.profiling traffic FETCH_DB WRITE_DB KERNEL GRID_X OUT_JSON
+
+``traffic`` scales the per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the
+kernel launches with grid size GRID_X by what the same counters read for
+the plain copy of known size in the same run (the runtime's copy kernel,
+``bench.py --legs copy,...``: 5 copies of 256 MiB).
+"""
+import json
+import sqlite3
+import sys
+
+
+def _db(path):
+    return sqlite3.connect(path)
+
+
+def kernel_table(path, pattern="%"):
+    """(name, grid_x, grid_y, calls, avg_us, total_ms) per kernel and grid."""
+    cur = _db(path).cursor()
+    q = ("select name, grid_x, grid_y, count(*), avg(end-start)/1e3, sum(end-start)/1e6 "
+         "from kernels where name like ? group by name, grid_x, grid_y "
+         "order by sum(end-start) desc")
+    return [(r[0].split("(")[0], r[1], r[2], r[3], r[4], r[5]) for r in cur.execute(q, (pattern,))]
+
+
+def pmc_table(path, counter, pattern="%"):
+    """(name, grid_size_x, dispatches, mean value per dispatch) of one
+    counter, values summed over the hardware units of each dispatch."""
+    cur = _db(path).cursor()
+    q = ("select kernel_name, grid_size_x, count(*), avg(v) from "
+         "(select dispatch_id, kernel_name, grid_size_x, sum(value) as v from counters_collection "
+         " where counter_name = ? and kernel_name like ? group by dispatch_id) "
+         "group by kernel_name, grid_size_x order by avg(v) * count(*) desc")
+    return [(r[0].split("(")[0], r[1], r[2], r[3]) for r in cur.execute(q, (counter, pattern))]
+
+
+def _copy_kib(path, counter):
+    rows = pmc_table(path, counter, "%copyBuffer%")
+    best = max(rows, key=lambda r: r[3]) if rows else None
+    return best[3] if best else None
+
+
+def traffic(fetch_db, write_db, kernel, grid_x, copy_bytes=256 << 20):
+    """HBM bytes per launch of `kernel` launches with grid size `grid_x`:
+    FETCH_SIZE and WRITE_SIZE scaled so that the copy of `copy_bytes`
+    reads and writes exactly `copy_bytes`."""
+    out = {"kernel": kernel, "grid_x": grid_x}
+    tot = 0.0
+    for name, path in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
+        rows = [r for r in pmc_table(path, name, "%" + kernel + "%") if r[1] == grid_x]
+        if not rows:
+            raise ValueError("no %s rows for %s grid %d" % (name, kernel, grid_x))
+        kib = sum(r[3] * r[2] for r in rows) / sum(r[2] for r in rows)
+        ckib = _copy_kib(path, name)
+        factor = copy_bytes / (ckib * 1024.0) if ckib else 1.0
+        out[name] = {"kib_per_dispatch": kib, "copy_kib": ckib, "scale": factor,
+                     "bytes_per_launch": kib * 1024.0 * factor}
+        tot += kib * 1024.0 * factor
+    out["bytes_per_launch"] = tot
+    return out
+
+
+def main(argv):
+    if len(argv) < 2:
+        print(__doc__)
+        return 2
+    cmd = argv[0]
+    if cmd == "kernels":
+        pat = argv[2] if len(argv) > 2 else "%"
+        print("%-48s %9s %4s %7s %10s %10s" % ("kernel", "grid_x", "gy", "calls", "avg_us",
+                                               "total_ms"))
+        for r in kernel_table(argv[1], pat):
+            print("%-48s %9d %4d %7d %10.2f %10.3f" % (r[0][:48], r[1], r[2], r[3], r[4], r[5]))
+    elif cmd == "pmc":
+        pat = argv[3] if len(argv) > 3 else "%"
+        print("%-48s %9s %6s %14s" % ("kernel", "grid_x", "calls", argv[2]))
+        for r in pmc_table(argv[1], argv[2], pat):
+            print("%-48s %9d %6d %14.1f" % (r[0][:48], r[1], r[2], r[3]))
+    elif cmd == "traffic":
+        t = traffic(argv[1], argv[2], argv[3], int(argv[4]))
+        with open(argv[5], "w") as f:
+            json.dump(t, f, indent=1)
+        print(json.dumps(t))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -223,10 +223,16 @@ const char *f2d_batch_last_error(void);
  *   coarsest up while they fit.
  * min_tiles: a fine-level phase uses the tallest tile (16, 8 or 4 rows of
  *   64 cells) that still gives at least this many workgroups (0 = always
- *   16 rows).  No reference counterpart (the reference has no launches). */
+ *   16 rows).
+ * coarse_wave_cells: inside the coarse kernel, levels of at most this many
+ *   cells (halo included) run on one wavefront without workgroup barriers;
+ *   larger ones use the whole workgroup (0 = none on one wavefront;
+ *   negative = the default, 256).
+ * No reference counterpart (the reference has no launches). */
 typedef struct {
   int coarse_lds_budget;
   int min_tiles;
+  int coarse_wave_cells;
 } f2d_batch_tuning;
 /* precision: 0 FP64 throughout (bit-identical to the reference algorithm);
  *   1 mixed: FP64 state, RK combination, full-solve rhs, outer residual and
