@@ -28,7 +28,7 @@ from .grid import Grid
 from .param import Param
 
 BUDGETS = (0, 98304, 49152, 24576)
-MIN_TILES = (0, 1024, 4096)
+MIN_TILES = (0, 1024, 4096, 1 << 24)
 WAVE_CELLS = (256, 0)
 
 

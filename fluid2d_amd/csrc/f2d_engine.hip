@@ -45,6 +45,18 @@ __device__ __forceinline__ bool skip(const DynD* dyn, int inst, int pred) {
 }
 __device__ __forceinline__ bool idle(const DynD* dyn, int inst) { return !dyn[inst].run; }
 
+// Task of this workgroup.  Workgroups are dealt to the 8 XCDs round-robin
+// (block b runs on XCD b mod 8); mapping each XCD to a contiguous run of
+// the task list (tasks are ordered by instance, level and tile row) keeps
+// neighbouring tiles -- and the same tiles in consecutive launches -- on
+// one XCD and its L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int task_index() {
+  const int b = blockIdx.x, n = gridDim.x;
+  const int q = n / NXCD, r = n % NXCD, x = b % NXCD, i = b / NXCD;
+  return x * q + min(x, r) + i;
+}
+
 // Step gate (first launch of a step): an instance runs this step while its
 // step budget lasts; every kernel of the step skips instances that do not.
 __global__ void k_gate(DynD* dyn, int ninst) {
@@ -131,7 +143,8 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   __shared__ double us[TY * (TX + 1)];
   __shared__ double vs[(TY + 1) * TX];
   __shared__ int8_t ms[QH * QW];
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
@@ -321,7 +334,8 @@ template <typename T, typename D>
 __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
                                              const InstD* __restrict__ I, const T* SP, D* DP,
                                              const DynD* __restrict__ dyn, int pred) {
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (skip(dyn, tk.inst, pred)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
@@ -357,7 +371,8 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   __shared__ T xs[XH][XW];
   __shared__ T j1[JH][JW];
   __shared__ uint8_t sbs[XH][XW];  // stencil bits (finest level)
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (skip(dyn, tk.inst, pred)) return;
   const int MODE = tk.mode;
   const int final_mask = tk.flags & 1;
@@ -533,12 +548,13 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   constexpr int RY = 2 * TY2V + 1, RX = 2 * TX2 + 1;
   __shared__ R rs[RY][RX];
   __shared__ double red[2][8];
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
   if (skip(dyn, tk.inst, pred)) {
     if ((flags & 12) && tid == 0) {
-      partial[2 * blockIdx.x] = 0.;
-      partial[2 * blockIdx.x + 1] = 0.;
+      partial[2 * ti] = 0.;
+      partial[2 * ti + 1] = 0.;
     }
     return;
   }
@@ -553,7 +569,15 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   // operator with an explicit rhs buffer (flag 16) -- the outer residual
   const R* A = RP + ((flags & 32) ? in.A0d : Lf.A);
   const R* x = RP + tk.xin;
-  const R* b = RP + ((flags & 16) ? tk.xout : Lf.b);
+  const R* bp = (flags & 64) ? x : RP + ((flags & 16) ? tk.xout : Lf.b);
+  // flag 64: the rhs is celltocorner of the T plane at tk.xout
+  // (core/fortran_operators.f90:44-64, evaluated as k_c2c does)
+  const T* cq = Q + tk.xout;
+  auto bat = [&](size_t c) -> R {
+    if (flags & 64)
+      return R(0.25) * (R(cq[c]) + R(cq[c + 1]) + R(cq[c + n1]) + R(cq[c + n1 + 1]));
+    return bp[c];
+  };
   const int fy0 = 2 * tk.y0 - 3, fx0 = 2 * tk.x0 - 3;
   R w[5];
   for (int k = 0; k < 5; k++) w[k] = R(in.w0[k]);
@@ -604,7 +628,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       const int q = min(tid + it * NT, RY * RX - 1);
       const int rr = q / RX, cc = q - rr * RX;
       const size_t c = at(fy0 + rr, fx0 + cc, n1);
-      bq[it] = b[c];
+      bq[it] = bat(c);
       mq[it] = msk1[c];
     }
     __syncthreads();
@@ -648,11 +672,11 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     const size_t c = at(j, i, n1);
     R v;
     if (flags & 2)
-      v = b[c];
+      v = bat(c);
     else if (tk.lev == 0 && in.a0fly)
-      v = residual_f<R>(msk1[c], n1, b[c], ABits<R>{sb, w, c}, [&](int dd) { return x[c + dd]; });
+      v = residual_f<R>(msk1[c], n1, bat(c), ABits<R>{sb, w, c}, [&](int dd) { return x[c + dd]; });
     else
-      v = residual_x<R>(msk1[c], A, PL, c, n1, b[c], [&](int dd) { return x[c + dd]; });
+      v = residual_x<R>(msk1[c], A, PL, c, n1, bat(c), [&](int dd) { return x[c + dd]; });
     rs[rr][cc] = v;
     // owned fine cells: local rows/cols [0, 2*T-1] that are interior
     if (rr < 2 * TY2V && cc < 2 * TX2 && jf == j && if_ == i && jf >= NH && if_ >= NH &&
@@ -660,7 +684,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       if (flags & 1) store_img_mask<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
       if (msk1[c] != 0) {
         nr += (double)v * (double)v;
-        const double bb = (double)b[c];
+        const double bb = (double)bat(c);
         nb += bb * bb;
       }
     }
@@ -682,8 +706,8 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       a += red[0][w];
       c += red[1][w];
     }
-    partial[2 * blockIdx.x] = a;
-    partial[2 * blockIdx.x + 1] = c;
+    partial[2 * ti] = a;
+    partial[2 * ti + 1] = c;
   }
   T* b2 = Q + Lc.b;
   for (int q = tid; q < TY2V * TX2; q += nt) {
@@ -863,7 +887,8 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
   typedef double C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   LDSP unsigned char* lds = (LDSP unsigned char*)lds_raw;
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (skip(dyn, tk.inst, pred)) return;
   const InstD& in = I[tk.inst];
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -965,7 +990,8 @@ __global__ void __launch_bounds__(256) k_ortho(const Task* __restrict__ tasks,
   // P: the pool holding the state-shaped array (FP64 state or T tendency),
   // u and v written at +2F, +3F of tk.xout; psi read at SP + tk.xin (its
   // own plane, FP64 for the refined tendencies of the mixed mode)
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
@@ -993,7 +1019,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_combine(const Task* __restrict__ tasks,
                                                  const InstD* __restrict__ I, double* P, T* Q,
                                                  const DynD* __restrict__ dyn) {
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const size_t F = (size_t)in.m * in.n;
@@ -1024,7 +1051,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_update(const Task* __restrict__ tasks,
                                                 const InstD* __restrict__ I, double* P, T* Q,
                                                 const DynD* __restrict__ dyn, int pred) {
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (skip(dyn, tk.inst, pred)) return;
   double* x = P + tk.xout;
   const T* y = Q + tk.xin;
@@ -1037,7 +1065,8 @@ __global__ void __launch_bounds__(256) k_finalize_psi(const Task* __restrict__ t
                                                       const InstD* __restrict__ I, double* P,
                                                       const int8_t* __restrict__ B,
                                                       const DynD* __restrict__ dyn) {
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const size_t F = (size_t)in.m * in.n;
@@ -1121,7 +1150,8 @@ __global__ void __launch_bounds__(256) k_npsum(const Task* __restrict__ tasks,
                                                const InstD* __restrict__ I, double* P,
                                                double* partial, const int* __restrict__ slot,
                                                const DynD* __restrict__ dyn) {
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   if (!in.perio) return;
@@ -1153,7 +1183,7 @@ __global__ void __launch_bounds__(256) k_npsum(const Task* __restrict__ tasks,
         if ((lane % (2 * w)) == 0) r = r + o;
       }
     }
-    if (lane == 0) partial[slot[blockIdx.x] + ch] = r;
+    if (lane == 0) partial[slot[ti] + ch] = r;
   }
 }
 
@@ -1164,7 +1194,8 @@ __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
                                               const DynD* __restrict__ dyn) {
   typedef double T;
   __shared__ double red[8][4];
-  const Task tk = tasks[blockIdx.x];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const int m = in.m, n = in.n;
@@ -1217,7 +1248,7 @@ __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
     double x = red[tid][0];
     for (int q = 1; q < (int)(blockDim.x * blockDim.y) / 64; q++)
       x = tid == 0 ? fmax(x, red[tid][q]) : x + red[tid][q];
-    partial[(size_t)blockIdx.x * 8 + tid] = x;
+    partial[(size_t)ti * 8 + tid] = x;
   }
 }
 
@@ -1671,16 +1702,17 @@ struct Builder {
   // correction is added in FP64.  u, v tendencies come from the FP64 psi
   // tendency, so their differences do not cancel in FP32.
   void invert_fast_refined(const std::vector<int>& who, int k) {
-    c2c(who, [k](HostInst& h) { return h.d.k[k]; }, [](HostInst& h) { return h.d.work; }, 0, 2);
     std::vector<Task> rs;
     for (int i : who) {
       HostInst& h = b.H[i];
       h.buf[0][0] = h.d.x0[0];
       h.buf[0][1] = h.d.x0[1];
       h.cur[0] = 0;
-      rr_tasks(i, 0, h.d.kp[k], h.d.work, h.d.L[0].b, rs);
+      rr_tasks(i, 0, h.d.kp[k], h.d.k[k], h.d.L[0].b, rs);
     }
-    add(L_RR, rs).flags = 1 | 16 | 32;  // FP64 defect, stored (T) as the level-0 rhs
+    // FP64 defect b - A x0 with b = celltocorner(omega tendency) formed on
+    // the fly, stored (T) as the level-0 rhs and restricted
+    add(L_RR, rs).flags = 1 | 32 | 64;
     std::vector<int> st(b.ninst, 0);
     vcycle(who, st, 1, 0, false);
     vcycle(who, st, 0, 0, true);
