@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--legs", default="plain,roofline,batched,precision,serving,cpu",
+    ap.add_argument("--legs", default="plain,roofline,batched,seqplain,precision,serving,cpu",
                     help="comma list of legs to run")
     ap.add_argument("--ensemble", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=500)
@@ -190,15 +190,51 @@ def cpu_baseline(name, seconds):
                       "gcc -O2 -ffp-contract=off, %.1f s" % (name, steps, el)}
 
 
-def load_traffic(kernel, workload):
+def load_traffic(kernel, workload, tuning):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), when
+    it was measured with the same launch configuration."""
     path = os.path.join(REPO, "profiles", "traffic_%s_%s.json" % (kernel, workload))
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         t = json.load(f)
+    if t.get("tuning") and tuning and t["tuning"].get("min_tiles") != tuning.get("min_tiles"):
+        return None, os.path.relpath(path, REPO) + " (other tile configuration)"
     return t.get("bytes_per_launch"), os.path.relpath(path, REPO)
+
+
+def sequential_plain(n, tune, steps=20):
+    """The ensemble run the plain way: one FP64 instance at a time.  Each
+    (size, geometry) class is timed once (graph-replayed steps) and the
+    ensemble's time is the sum over its instances."""
+    from fluid2d_amd import workloads
+    from fluid2d_amd.batch import Batch
+    from fluid2d_amd.grid import Grid
+    from fluid2d_amd.param import Param
+
+    specs = workloads.ensemble_specs(n)
+    classes = {}
+    for kw, seed in specs:
+        classes.setdefault((kw["nx"], kw["geometry"]), []).append((kw, seed))
+    total_t, total_cells = 0.0, 0
+    per = {}
+    for key, members in sorted(classes.items()):
+        kw, seed = members[0]
+        p = Param(**kw)
+        g = Grid(p)
+        b = Batch([(p, g)], precision="fp64", use_graph=USE_GRAPH,
+                  tuning=tuned([(p, g)], "fp64", tune))
+        b.set_field(0, "vorticity", workloads.turbulence(g, seed))
+        b.set_psi_from_vorticity()
+        b.set_dt(0.8 * min(g.dx, g.dy) / b.diagnostics(0)["maxspeed"])
+        b.step(2)
+        el = timed(b, steps) / steps
+        per["%dx%d-%s" % (key[0], key[0], key[1])] = el * 1e3
+        total_t += el * len(members)
+        total_cells += g.nx * g.ny * len(members)
+        del b
+    return total_cells / total_t, per
 
 
 def main():
@@ -267,7 +303,7 @@ def main():
     if "roofline" in legs and b is not None:
         dom, shares, pr, achieved = roofline(b)
         kname = KERNEL_NAMES[dom] + "<double>"
-        traffic, src = load_traffic(KERNEL_NAMES[dom], name)
+        traffic, src = load_traffic(KERNEL_NAMES[dom], name, out["config"].get("tuning"))
         out["roofline"] = {
             "bound": "hbm",
             "kernel": kname + (" (all launches)" if dom == "coarse" else " finest-level launches"),
@@ -293,6 +329,14 @@ def main():
                           "speedup_vs_plain_fp64": (bval / out["value"]) if out["value"] else None,
                           "tuning": eb.tuning()}
         log("batched mixed: %.3e cell-updates/s" % bval)
+        if "seqplain" in legs:
+            sval, per = sequential_plain(a.ensemble, not a.no_tune)
+            out["batched"]["plain_fp64_same_workload"] = {
+                "value": sval, "unit": "cell-updates/s", "ms_per_step_by_class": per,
+                "method": "each (size, geometry) class timed alone in FP64, summed over the "
+                          "ensemble's instances"}
+            out["batched"]["speedup_vs_plain_fp64_same_workload"] = bval / sval
+            log("plain FP64, same ensemble one instance at a time: %.3e cell-updates/s" % sval)
         if "precision" in legs:
             mixed_states = [eb.state(i) for i in range(eb.n)]
             del eb

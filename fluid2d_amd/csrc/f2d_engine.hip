@@ -370,6 +370,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   constexpr int IT3 = TYV / 4;
   __shared__ T xs[XH][XW];
   __shared__ T j1[JH][JW];
+  __shared__ T bs[JH][JW];         // right-hand side on the first sweep's region
   __shared__ uint8_t sbs[XH][XW];  // stencil bits (finest level)
   const int ti = task_index();
   const Task tk = tasks[ti];
@@ -408,64 +409,67 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
 #pragma unroll
   for (int it = 0; it < IT2; it++) bv[it] = b[cc2[it]];
   const int Ic = x0 + threadIdx.x;
-  size_t c3[IT3];
-  bool ok3[IT3];
-  T bv3[IT3];
 #pragma unroll
-  for (int k = 0; k < IT3; k++) {
-    const int J = y0 + threadIdx.y + 4 * k;
-    ok3[k] = J < m - NH && Ic < n - NH;
-    c3[k] = ok3[k] ? at(J, Ic, n) : 0;
-    bv3[k] = b[c3[k]];
+  for (int it = 0; it < IT2; it++) {
+    const int q = tid + it * NT;
+    if (q < N2) {
+      const int rr = q / JW, cc = q - rr * JW;
+      bs[rr][cc] = ok2[it] ? bv[it] : T(0);
+    }
   }
   // phase 1: x (and the stencil bits) on rows [y0-2, y0+TY+1] x cols
-  // [x0-2, x0+TX+1], clipped to [1, m-2]
+  // [x0-2, x0+TX+1], clipped to [1, m-2]; loaded in groups of G1 elements
+  // per thread (all loads of a group in flight at once, registers bounded)
+  bool full = true;
   {
-    int jj[IT1], ii[IT1];
-    bool ok[IT1];
+    constexpr int G1 = 3;
 #pragma unroll
-    for (int it = 0; it < IT1; it++) {
-      const int q = tid + it * NT;
-      const int rr = q / XW, cc = q - rr * XW;
-      const int j = y0 - 2 + rr, i = x0 - 2 + cc;
-      ok[it] = q < N1 && j <= m - 2 && i <= n - 2;
-      jj[it] = ok[it] ? j : 1;
-      ii[it] = ok[it] ? i : 1;
-    }
-    T v[IT1];
-    uint8_t bits[IT1];
+    for (int g0 = 0; g0 < IT1; g0 += G1) {
+      int jj[G1], ii[G1];
+      bool ok[G1];
 #pragma unroll
-    for (int it = 0; it < IT1; it++) {
-      v[it] = T(0);
-      bits[it] = 0;
-    }
-    if (rdx) {
-#pragma unroll
-      for (int it = 0; it < IT1; it++) v[it] = xin[at(jj[it], ii[it], n)];
-    }
-    if (fly) {
-#pragma unroll
-      for (int it = 0; it < IT1; it++) bits[it] = sb[at(jj[it], ii[it], n)];
-    }
-    if (rdi) {
-      IpOps<T> op[IT1];
-#pragma unroll
-      for (int it = 0; it < IT1; it++) op[it] = ip_load<T>(msk, n, msk2, xc, C.n, jj[it], ii[it]);
-#pragma unroll
-      for (int it = 0; it < IT1; it++) {
-        const T r = ip_eval<T>(op[it]);
-        v[it] = MODE == 2 ? v[it] + r : r;
-      }
-    }
-    bool full = true;
-#pragma unroll
-    for (int it = 0; it < IT1; it++) {
-      const int q = tid + it * NT;
-      if (q < N1) {
+      for (int k = 0; k < G1; k++) {
+        const int q = tid + (g0 + k) * NT;
         const int rr = q / XW, cc = q - rr * XW;
-        xs[rr][cc] = ok[it] ? v[it] : T(0);
-        sbs[rr][cc] = ok[it] ? bits[it] : (uint8_t)0;
-        full = full && ok[it] && bits[it] == 31;
+        const int j = y0 - 2 + rr, i = x0 - 2 + cc;
+        ok[k] = g0 + k < IT1 && q < N1 && j <= m - 2 && i <= n - 2;
+        jj[k] = ok[k] ? j : 1;
+        ii[k] = ok[k] ? i : 1;
+      }
+      T v[G1];
+      uint8_t bits[G1];
+#pragma unroll
+      for (int k = 0; k < G1; k++) {
+        v[k] = T(0);
+        bits[k] = 0;
+      }
+      if (rdx) {
+#pragma unroll
+        for (int k = 0; k < G1; k++) v[k] = xin[at(jj[k], ii[k], n)];
+      }
+      if (fly) {
+#pragma unroll
+        for (int k = 0; k < G1; k++) bits[k] = sb[at(jj[k], ii[k], n)];
+      }
+      if (rdi) {
+        IpOps<T> op[G1];
+#pragma unroll
+        for (int k = 0; k < G1; k++) op[k] = ip_load<T>(msk, n, msk2, xc, C.n, jj[k], ii[k]);
+#pragma unroll
+        for (int k = 0; k < G1; k++) {
+          const T r = ip_eval<T>(op[k]);
+          v[k] = MODE == 2 ? v[k] + r : r;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < G1; k++) {
+        const int q = tid + (g0 + k) * NT;
+        if (g0 + k < IT1 && q < N1) {
+          const int rr = q / XW, cc = q - rr * XW;
+          xs[rr][cc] = ok[k] ? v[k] : T(0);
+          sbs[rr][cc] = ok[k] ? bits[k] : (uint8_t)0;
+          full = full && ok[k] && bits[k] == 31;
+        }
       }
     }
     // every cell of the tile has the full 9-point stencil: constant weights
@@ -501,12 +505,13 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
     // phase 3: second sweep on the tile interior, stored with its images
 #pragma unroll
     for (int k = 0; k < IT3; k++) {
-      if (!ok3[k]) continue;
       const int rr = threadIdx.y + 4 * k, cc = threadIdx.x;
-      const size_t c = c3[k];
-      const T v = Jac(c, rr + 2, cc + 2, bv3[k],
+      const int J = y0 + rr;
+      if (J >= m - NH || Ic >= n - NH) continue;
+      const size_t c = at(J, Ic, n);
+      const T v = Jac(c, rr + 2, cc + 2, bs[rr + 1][cc + 1],
                       [&](int dj, int di) { return j1[rr + 1 + dj][cc + 1 + di]; });
-      store_img_mask<T>(xout, mp, y0 + rr, Ic, v, m, n);
+      store_img_mask<T>(xout, mp, J, Ic, v, m, n);
     }
   };
   if (full_tile) {
@@ -583,67 +588,69 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   for (int k = 0; k < 5; k++) w[k] = R(in.w0[k]);
   const uint8_t* sb = (const uint8_t*)(B + in.sb0);
   double nr = 0., nb = 0.;
-  // Tiles whose fine points are all interior (no periodic images involved)
-  // read x (and the stencil bits) from LDS tiles loaded in one round trip;
-  // the others take the general path below.  Both evaluate the same
-  // expressions in the same order.
+  // x (and the stencil bits) of the tile's region in one round trip into
+  // LDS (rows/columns clamped to the array); points off the periodic halo
+  // take their operands there, the few halo points the restriction needs
+  // (their values are their interior images') read them around the image.
+  // Both forms evaluate the same expressions in the same order.
   constexpr int NT = 256, XRW = RX + 2, XRH = RY + 2, NX = XRW * XRH;
   constexpr int ITX = (NX + NT - 1) / NT, ITP = (RY * RX + NT - 1) / NT;
   __shared__ R xsr[XRH][XRW];
   __shared__ uint8_t sbr[XRH][XRW];
   const bool fly = tk.lev == 0 && in.a0fly;
-  const bool interior = fy0 >= NH && fx0 >= NH && fy0 + RY - 1 <= m1 - NH - 1 &&
-                        fx0 + RX - 1 <= n1 - NH - 1;
   int full = 0;
-  if (interior) {
-    if (!(flags & 2)) {
-      R xv[ITX];
-      uint8_t bv[ITX];
-      bool allfull = true;
+  if (!(flags & 2)) {
+    R xv[ITX];
+    uint8_t bv[ITX];
+    bool allfull = true;
 #pragma unroll
-      for (int it = 0; it < ITX; it++) {
-        const int q = min(tid + it * NT, NX - 1);
-        const int rr = q / XRW, cc = q - rr * XRW;
-        const size_t c = at(fy0 - 1 + rr, fx0 - 1 + cc, n1);
-        xv[it] = x[c];
-        bv[it] = fly ? sb[c] : (uint8_t)0;
-      }
-#pragma unroll
-      for (int it = 0; it < ITX; it++) {
-        const int q = tid + it * NT;
-        if (q < NX) {
-          const int rr = q / XRW, cc = q - rr * XRW;
-          xsr[rr][cc] = xv[it];
-          sbr[rr][cc] = bv[it];
-          allfull = allfull && bv[it] == 31;
-        }
-      }
-      // the whole tile has the full stencil: constant weights
-      full = __syncthreads_and(fly && allfull);
+    for (int it = 0; it < ITX; it++) {
+      const int q = min(tid + it * NT, NX - 1);
+      const int rr = q / XRW, cc = q - rr * XRW;
+      const size_t c = at(min(fy0 - 1 + rr, m1 - 1), min(fx0 - 1 + cc, n1 - 1), n1);
+      xv[it] = x[c];
+      bv[it] = fly ? sb[c] : (uint8_t)0;
     }
-    R bq[ITP];
-    int8_t mq[ITP];
 #pragma unroll
-    for (int it = 0; it < ITP; it++) {
-      const int q = min(tid + it * NT, RY * RX - 1);
-      const int rr = q / RX, cc = q - rr * RX;
-      const size_t c = at(fy0 + rr, fx0 + cc, n1);
-      bq[it] = bat(c);
-      mq[it] = msk1[c];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < ITP; it++) {
+    for (int it = 0; it < ITX; it++) {
       const int q = tid + it * NT;
-      if (q >= RY * RX) continue;
-      const int rr = q / RX, cc = q - rr * RX;
-      const int jf = fy0 + rr, if_ = fx0 + cc;
+      if (q < NX) {
+        const int rr = q / XRW, cc = q - rr * XRW;
+        xsr[rr][cc] = xv[it];
+        sbr[rr][cc] = bv[it];
+        allfull = allfull && bv[it] == 31;
+      }
+    }
+    // the whole region has the full stencil: constant weights
+    full = __syncthreads_and(fly && allfull);
+  }
+  R bq[ITP];
+  int8_t mq[ITP];
+#pragma unroll
+  for (int it = 0; it < ITP; it++) {
+    const int q = min(tid + it * NT, RY * RX - 1);
+    const int rr = q / RX, cc = q - rr * RX;
+    const int jf = min(fy0 + rr, m1 - NH), if_ = min(fx0 + cc, n1 - NH);
+    const size_t c = at(img(jf, m1), img(if_, n1), n1);
+    bq[it] = bat(c);
+    mq[it] = msk1[c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITP; it++) {
+    const int q = tid + it * NT;
+    if (q >= RY * RX) continue;
+    const int rr = q / RX, cc = q - rr * RX;
+    const int jf = fy0 + rr, if_ = fx0 + cc;
+    if (jf > m1 - NH || if_ > n1 - NH) continue;  // beyond the last needed point
+    const bool halo = jf >= m1 - NH || if_ >= n1 - NH;
+    R v;
+    if (flags & 2) {
+      v = bq[it];
+    } else if (!halo) {
       const size_t c = at(jf, if_, n1);
       auto xl = [&](int dj, int di) { return xsr[rr + 1 + dj][cc + 1 + di]; };
-      R v;
-      if (flags & 2) {
-        v = bq[it];
-      } else if (full) {
+      if (full) {
         v = residual_2d<R>(mq[it], bq[it], [&](int k, int, int) { return w[k]; }, xl);
       } else if (fly) {
         v = residual_2d<R>(mq[it], bq[it], ABitsTile2<R, XRW>{&sbr[rr + 1][cc + 1], w}, xl);
@@ -653,38 +660,22 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
                            [&](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * n1 + di); },
                            xl);
       }
-      rs[rr][cc] = v;
-      if (rr < 2 * TY2V && cc < 2 * TX2) {
-        if (flags & 1) store_img_mask<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
-        if (mq[it] != 0) {
-          nr += (double)v * (double)v;
-          const double bb = (double)bq[it];
-          nb += bb * bb;
-        }
-      }
+    } else {
+      const size_t c = at(img(jf, m1), img(if_, n1), n1);
+      if (fly)
+        v = residual_f<R>(mq[it], n1, bq[it], ABits<R>{sb, w, c}, [&](int dd) { return x[c + dd]; });
+      else
+        v = residual_x<R>(mq[it], A, PL, c, n1, bq[it], [&](int dd) { return x[c + dd]; });
     }
-  }
-  for (int q = tid; q < RY * RX && !interior; q += nt) {
-    const int rr = q / RX, cc = q % RX;
-    const int jf = fy0 + rr, if_ = fx0 + cc;
-    if (jf > m1 - NH || if_ > n1 - NH) continue;  // beyond the last needed point
-    const int j = img(jf, m1), i = img(if_, n1);
-    const size_t c = at(j, i, n1);
-    R v;
-    if (flags & 2)
-      v = bat(c);
-    else if (tk.lev == 0 && in.a0fly)
-      v = residual_f<R>(msk1[c], n1, bat(c), ABits<R>{sb, w, c}, [&](int dd) { return x[c + dd]; });
-    else
-      v = residual_x<R>(msk1[c], A, PL, c, n1, bat(c), [&](int dd) { return x[c + dd]; });
     rs[rr][cc] = v;
-    // owned fine cells: local rows/cols [0, 2*T-1] that are interior
-    if (rr < 2 * TY2V && cc < 2 * TX2 && jf == j && if_ == i && jf >= NH && if_ >= NH &&
-        jf < m1 - NH && if_ < n1 - NH && (tk.y0 + rr / 2) < m2 - NH && (tk.x0 + cc / 2) < n2 - NH) {
+    // owned fine cells: local rows/cols [0, 2*T-1] off the halo, under the
+    // coarse interior
+    if (rr < 2 * TY2V && cc < 2 * TX2 && !halo && (tk.y0 + rr / 2) < m2 - NH &&
+        (tk.x0 + cc / 2) < n2 - NH) {
       if (flags & 1) store_img_mask<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
-      if (msk1[c] != 0) {
+      if (mq[it] != 0) {
         nr += (double)v * (double)v;
-        const double bb = (double)bat(c);
+        const double bb = (double)bq[it];
         nb += bb * bb;
       }
     }
@@ -1895,19 +1886,21 @@ struct ProfSel {
   std::vector<const Launch*> which;
 };
 
+#define F2D_LAUNCH(K, G, B, S, STREAM, ...) hipLaunchKernelGGL(K, G, B, S, STREAM, __VA_ARGS__)
+
 template <typename T, typename R>
 static void launch_rr(int ty, unsigned g, hipStream_t s, const Task* TK, const InstD* dI,
                       const R* RP, T* Q, const int8_t* B, const DynD* dD, int pred, int flags,
                       double* partial) {
   const dim3 B2(64, 4);
   if (ty == 4)
-    hipLaunchKernelGGL((k_resid_restrict<T, R, 2>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD, pred,
+    F2D_LAUNCH((k_resid_restrict<T, R, 2>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD, pred,
                        flags, partial);
   else if (ty == 8)
-    hipLaunchKernelGGL((k_resid_restrict<T, R, 4>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD, pred,
+    F2D_LAUNCH((k_resid_restrict<T, R, 4>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD, pred,
                        flags, partial);
   else
-    hipLaunchKernelGGL((k_resid_restrict<T, R, TY2>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD,
+    F2D_LAUNCH((k_resid_restrict<T, R, TY2>), dim3(g), B2, 0, s, TK, dI, RP, Q, B, dD,
                        pred, flags, partial);
 }
 
@@ -1933,26 +1926,26 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
     }
     switch (L.kind) {
       case L_RHS:
-        hipLaunchKernelGGL(k_rhs<T>, dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
+        F2D_LAUNCH(k_rhs<T>, dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
                            L.outk);
         break;
       case L_C2C:
         if (L.flags == 1)
-          hipLaunchKernelGGL((k_c2c<double, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.dD,
+          F2D_LAUNCH((k_c2c<double, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.dD,
                              L.pred);
         else if (L.flags == 2)
-          hipLaunchKernelGGL((k_c2c<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.dD,
+          F2D_LAUNCH((k_c2c<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.dD,
                              L.pred);
         else
-          hipLaunchKernelGGL((k_c2c<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.dD, L.pred);
+          F2D_LAUNCH((k_c2c<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.dD, L.pred);
         break;
       case L_SMOOTH:
         if (L.ty == 4)
-          hipLaunchKernelGGL((k_smooth<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+          F2D_LAUNCH((k_smooth<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
         else if (L.ty == 8)
-          hipLaunchKernelGGL((k_smooth<T, 8>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+          F2D_LAUNCH((k_smooth<T, 8>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
         else
-          hipLaunchKernelGGL((k_smooth<T, TY>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+          F2D_LAUNCH((k_smooth<T, TY>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                              L.pred);
         break;
       case L_RR:
@@ -1963,40 +1956,40 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
                           b.partial);
         break;
       case L_COARSE:
-        hipLaunchKernelGGL(k_coarse<T>, dim3(g), dim3(COARSE_THREADS), L.shmem, b.s, TK, b.dI, Q, b.B,
+        F2D_LAUNCH(k_coarse<T>, dim3(g), dim3(COARSE_THREADS), L.shmem, b.s, TK, b.dI, Q, b.B,
                            b.dD, L.pred, L.mode, b.coarse_wave_cells);
         break;
       case L_ORTHO:
         if (L.flags == 1)
-          hipLaunchKernelGGL((k_ortho<double, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.B,
+          F2D_LAUNCH((k_ortho<double, double>), dim3(g), B2, 0, b.s, TK, b.dI, P, P, b.B,
                              b.dD);
         else if (L.flags == 2)
-          hipLaunchKernelGGL((k_ortho<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B,
+          F2D_LAUNCH((k_ortho<T, double>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B,
                              b.dD);
         else
-          hipLaunchKernelGGL((k_ortho<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.B, b.dD);
+          F2D_LAUNCH((k_ortho<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.B, b.dD);
         break;
       case L_COMBINE:
-        hipLaunchKernelGGL(k_combine<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD);
+        F2D_LAUNCH(k_combine<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD);
         break;
       case L_UPDATE:
-        hipLaunchKernelGGL(k_update<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD, L.pred);
+        F2D_LAUNCH(k_update<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD, L.pred);
         break;
       case L_FINPSI:
-        hipLaunchKernelGGL(k_finalize_psi, dim3(g), B1, 0, b.s, TK, b.dI, P, b.B, b.dD);
+        F2D_LAUNCH(k_finalize_psi, dim3(g), B1, 0, b.s, TK, b.dI, P, b.B, b.dD);
         break;
       case L_NPSUM:
-        hipLaunchKernelGGL(k_npsum, dim3(g), B1, 0, b.s, TK, b.dI, P, b.partial,
+        F2D_LAUNCH(k_npsum, dim3(g), B1, 0, b.s, TK, b.dI, P, b.partial,
                            b.dSlot + L.slot_off, b.dD);
         break;
       case L_DIAG:
-        hipLaunchKernelGGL(k_diag, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B, b.partial, b.dD);
+        F2D_LAUNCH(k_diag, dim3(g), B2, 0, b.s, TK, b.dI, P, b.B, b.partial, b.dD);
         break;
       case L_GATE:
-        hipLaunchKernelGGL(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
+        F2D_LAUNCH(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
       case L_FOLD:
-        hipLaunchKernelGGL(k_fold, dim3(b.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
+        F2D_LAUNCH(k_fold, dim3(b.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
                            b.partial, L.mode, b.maxite, b.tol, L.pred, b.adapt, b.cfl, b.dtmax);
         break;
     }

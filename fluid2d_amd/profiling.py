@@ -5,11 +5,11 @@ roofline.
 
     python -m fluid2d_amd.profiling kernels DB [PATTERN]
     python -m fluid2d_amd.profiling pmc DB COUNTER [PATTERN]
-    python -m fluid2d_amd.profiling traffic FETCH_DB WRITE_DB KERNEL GRID_X OUT_JSON
+    python -m fluid2d_amd.profiling traffic FETCH_DB WRITE_DB KERNEL GRID_X OUT_JSON [TUNING_JSON]
 
 ``traffic`` scales the per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the
 kernel launches with grid size GRID_X by what the same counters read for
-the plain copy of known size in the same run (the runtime's copy kernel,
+the plain copies of known size in the same run (the runtime's copy kernel,
 ``bench.py --legs copy,...``: 5 copies of 256 MiB).
 """
 import json
@@ -42,16 +42,21 @@ def pmc_table(path, counter, pattern="%"):
 
 
 def _copy_kib(path, counter):
+    """Total counter value (KiB) over the dispatches of the calibration copy
+    (the runtime copy kernel at its largest grid; one copy may be split
+    over several dispatches)."""
     rows = pmc_table(path, counter, "%copyBuffer%")
-    best = max(rows, key=lambda r: r[3]) if rows else None
-    return best[3] if best else None
+    if not rows:
+        return None
+    big = max(r[1] for r in rows)
+    return sum(r[3] * r[2] for r in rows if r[1] == big)
 
 
-def traffic(fetch_db, write_db, kernel, grid_x, copy_bytes=256 << 20):
+def traffic(fetch_db, write_db, kernel, grid_x, copy_bytes=5 * (256 << 20)):
     """HBM bytes per launch of `kernel` launches with grid size `grid_x`:
-    FETCH_SIZE and WRITE_SIZE scaled so that the copy of `copy_bytes`
-    reads and writes exactly `copy_bytes`."""
-    out = {"kernel": kernel, "grid_x": grid_x}
+    FETCH_SIZE and WRITE_SIZE scaled so that the calibration copies
+    (`copy_bytes` in total) read and write exactly `copy_bytes`."""
+    out = {"kernel": kernel, "grid_x": grid_x, "copy_bytes": copy_bytes}
     tot = 0.0
     for name, path in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
         rows = [r for r in pmc_table(path, name, "%" + kernel + "%") if r[1] == grid_x]
@@ -60,7 +65,7 @@ def traffic(fetch_db, write_db, kernel, grid_x, copy_bytes=256 << 20):
         kib = sum(r[3] * r[2] for r in rows) / sum(r[2] for r in rows)
         ckib = _copy_kib(path, name)
         factor = copy_bytes / (ckib * 1024.0) if ckib else 1.0
-        out[name] = {"kib_per_dispatch": kib, "copy_kib": ckib, "scale": factor,
+        out[name] = {"kib_per_dispatch": kib, "copy_kib_total": ckib, "scale": factor,
                      "bytes_per_launch": kib * 1024.0 * factor}
         tot += kib * 1024.0 * factor
     out["bytes_per_launch"] = tot
@@ -85,6 +90,8 @@ def main(argv):
             print("%-48s %9d %6d %14.1f" % (r[0][:48], r[1], r[2], r[3]))
     elif cmd == "traffic":
         t = traffic(argv[1], argv[2], argv[3], int(argv[4]))
+        if len(argv) > 6:
+            t["tuning"] = json.loads(argv[6])
         with open(argv[5], "w") as f:
             json.dump(t, f, indent=1)
         print(json.dumps(t))
