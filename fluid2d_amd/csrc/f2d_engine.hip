@@ -848,6 +848,88 @@ __device__ void c_interp(const CL<T, TA>& F, const CL<T, TA>& C, bool acc, Sync&
   sy.end(cnt);
 }
 
+// A cell's eight off-diagonal couplings (the reads of offdiag_2d), held in
+// registers; the arguments are literals after inlining
+template <typename T>
+struct ARow {
+  T a0, a1, a2, a3, a3e, a2s, a1s, a0s;
+  __device__ __forceinline__ T operator()(int k, int dj, int di) const {
+    if (dj == 0 && di == 0) return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
+    return k == 3 ? a3e : (k == 2 ? a2s : (k == 1 ? a1s : a0s));
+  }
+};
+
+// The coarsest-level solve: `iters` smoothings (two damped-Jacobi sweeps
+// each, hierarchy.py:117-118 / 144-145) of a level entered with x = 0.
+// Thread q owns cell q of the first sweep's region and keeps that cell's
+// couplings, factor c1 / |A_5|, rhs and mask in registers for all sweeps,
+// so a sweep reads only the nine x (or t) values from LDS.  The arithmetic
+// is jacobi_f's, operand for operand.  Levels whose first-sweep region
+// exceeds the workgroup take the generic smoother.
+template <typename T, typename TA>
+__device__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy, int iters) {
+  const int m = L.m, n = L.n, cells = m * n;
+  const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2, w = wj * wi;
+  if (w > sy.nt) {
+    for (int k = 0; k < iters; k++) c_smooth(L, c1, c2, sy, k == 0);
+    return;
+  }
+  // one wavefront when the region fits it (LDS operations of a wavefront
+  // execute in order: a wave-level barrier separates the sweeps)
+  const bool wave = w <= 64;
+  if (!wave && sy.prev_small) __syncthreads();
+  sy.prev_small = wave;
+  auto sync = [&]() {
+    if (wave) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      __syncthreads();
+    }
+  };
+  const size_t P = (size_t)cells;
+  const int q = sy.tid;
+  const bool own = q < w;
+  const int qj = own ? qdiv(q, wi) : 0;
+  const int j = NH - 1 + qj, i = NH - 1 + (own ? q - qj * wi : 0);
+  const size_t c = at(j, i, n);
+  const bool inner = own && j >= NH && j < m - NH && i >= NH && i < n - NH;
+  ARow<T> a;
+  T bc = T(0), c3 = T(0);
+  int8_t mk = 0;
+  if (own) {
+    a.a0 = T(L.A[c]);
+    a.a1 = T(L.A[P + c]);
+    a.a2 = T(L.A[2 * P + c]);
+    a.a3 = T(L.A[3 * P + c]);
+    a.a3e = T(L.A[3 * P + c + 1]);
+    a.a2s = T(L.A[2 * P + c + n - 1]);
+    a.a1s = T(L.A[P + c + n]);
+    a.a0s = T(L.A[c + n + 1]);
+    c3 = c1 / fabs(T(L.A[4 * P + c]));
+    bc = L.b[c];
+    mk = L.msk[c];
+  }
+  LDSP T* x = L.x;
+  LDSP T* t = L.t;
+  for (int k = 0; k < iters; k++) {
+    if (own) {
+      T v;
+      if (k == 0)
+        v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int, int) { return T(0); });
+      else
+        v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return x[c + dj * n + di]; });
+      t[c] = mk == 0 ? T(0) : v;
+    }
+    sync();
+    if (inner) {
+      const T v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return t[c + dj * n + di]; });
+      store_img_mask<T>(x, nullptr, j, i, mk == 0 ? T(0) : v, m, n);
+    }
+    sync();
+  }
+}
+
 // V-cycle from level lev1 down to level deep of the coarse part
 // (hierarchy.py:98-126)
 template <typename T, typename TA>
@@ -857,7 +939,7 @@ __device__ void c_vcycle(const CL<T, TA>* Ls, int lev1, int deep, bool zero_firs
     c_smooth(Ls[l], c1, c2, sy, l > lev1 || zero_first);
     c_resid_restrict(Ls[l], Ls[l + 1], false, sy);
   }
-  for (int k = 0; k < 16; k++) c_smooth(Ls[deep], c1, c2, sy, k == 0);
+  c_coarsest(Ls[deep], c1, c2, sy, 16);
   for (int l = deep - 1; l >= lev1; l--) {
     c_interp(Ls[l], Ls[l + 1], true, sy);
     c_smooth(Ls[l], c1, c2, sy);
@@ -958,7 +1040,7 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
     c_vcycle(Ls, 0, nl - 1, true, c1, c2, sy);
   } else {
     for (int l = 0; l < nl - 1; l++) c_resid_restrict(Ls[l], Ls[l + 1], true, sy);
-    for (int k = 0; k < 16; k++) c_smooth(Ls[nl - 1], c1, c2, sy, k == 0);
+    c_coarsest(Ls[nl - 1], c1, c2, sy, 16);
     for (int l = nl - 2; l >= 0; l--) {
       c_interp(Ls[l], Ls[l + 1], false, sy);
       c_vcycle(Ls, l, nl - 1, false, c1, c2, sy);
