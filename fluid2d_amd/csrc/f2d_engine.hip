@@ -2456,12 +2456,24 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
         std::vector<double> hA(5 * S);
         ok = ok && hipMemcpy(hA.data(), gA, 5 * S * sizeof(double), hipMemcpyDeviceToHost) ==
                        hipSuccess;
+        // bits 0-3: coupling A_k nonzero (all nonzero values of a plane
+        // equal); bit 4: the cell mask.  The centre coefficient is read
+        // only where the mask is set (Jacobi and residual are 0 elsewhere),
+        // and must equal one nonzero value there; the halo fill leaves
+        // nonzero centres under a zero mask, which are never read.
         std::vector<uint8_t> bits(S, 0);
         bool fly = ok;
         for (int k = 0; k < 5 && fly; k++) {
           double wk = 0.;
           for (size_t q = 0; q < S; q++) {
             const double a = hA[k * S + q];
+            if (k == 4) {
+              if (mp[q] == 0) continue;
+              if (a == 0.) {
+                fly = false;
+                break;
+              }
+            }
             if (a == 0.) continue;
             if (wk == 0.) wk = a;
             if (a != wk) {
@@ -2472,9 +2484,6 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
           }
           d.w0[k] = wk;
         }
-        // the smoother takes the cell mask from the centre bit
-        for (size_t q = 0; q < S && fly; q++)
-          if ((mp[q] != 0) != ((bits[q] >> 4) & 1)) fly = false;
         d.a0fly = fly ? 1 : 0;
         if (fly)
           ok = ok && hipMemcpy(b->B + d.sb0, bits.data(), S, hipMemcpyHostToDevice) == hipSuccess;

@@ -195,3 +195,18 @@ def test_retune_between_steps():
     b.retune({"coarse_lds_budget": 24576, "min_tiles": 512})
     b.step(3)
     assert (b.state(0) == ref.state(0)).all()
+
+
+@pytest.mark.parametrize("geom", ["perio", "closed", "xchannel", "disc"])
+def test_finest_operator_from_stencil_bits(geom):
+    """The finest operator (a constant stencil times masks) is applied from
+    one byte of stencil bits per cell, not from five FP64 planes: the
+    finest-level smoother's implemented bytes stay near its algorithmic
+    bytes (explicit planes add 40 bytes per cell to about 26)."""
+    b = Batch([spec("euler", 128, 128, 1.0, 1.0, geom)])
+    init_fields(b, 0, 3)
+    b.set_psi_from_vorticity()
+    b.set_dt(dt_for(b, 0, b.diagnostics(0)["maxspeed"]))
+    pr = b.profile("smooth", 0, nsteps=1)
+    assert pr["launches"] > 0
+    assert pr["impl_bytes"] < 1.5 * pr["alg_bytes"], pr
