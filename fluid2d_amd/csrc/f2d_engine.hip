@@ -394,7 +394,10 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   const int8_t* msk2 = B + C.msk;
   const T* xc = Q + tk.xc;
   int full_tile = 0;
-  // right-hand sides of both sweeps, loaded with the tile (one round trip)
+  // One round trip: every global load of the tile is issued before the
+  // first is used -- the right-hand side of both sweeps, x and the stencil
+  // bits on the first sweep's stencil region, and for a prolongation the
+  // fine mask there and the coarse tile under it.
   size_t cc2[IT2];
   bool ok2[IT2];
   T bv[IT2];
@@ -409,6 +412,41 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
 #pragma unroll
   for (int it = 0; it < IT2; it++) bv[it] = b[cc2[it]];
   const int Ic = x0 + threadIdx.x;
+  // x region: rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1], clipped to
+  // [1, m-2]; the coarse points its prolongation reads are rows
+  // [y0>>1, +CH) x cols [x0>>1, +CW) (clamped to the coarse array)
+  constexpr int CH = TYV / 2 + 4, CW = TX / 2 + 4, NC = CH * CW, ITC = (NC + NT - 1) / NT;
+  __shared__ T cxs[CH][CW];
+  __shared__ int8_t cms[CH][CW];
+  __shared__ int8_t ms1[XH][XW];
+  const int cy0 = y0 >> 1, cx0 = x0 >> 1;
+  bool ok1[IT1];
+  T xv[IT1];
+  uint8_t bt[IT1];
+  int8_t m1v[IT1];
+  T cxv[ITC];
+  int8_t cmv[ITC];
+#pragma unroll
+  for (int it = 0; it < IT1; it++) {
+    const int q = tid + it * NT;
+    const int rr = q / XW, cc = q - rr * XW;
+    const int j = y0 - 2 + rr, i = x0 - 2 + cc;
+    ok1[it] = q < N1 && j <= m - 2 && i <= n - 2;
+    const size_t c = ok1[it] ? at(j, i, n) : at(1, 1, n);
+    xv[it] = rdx ? xin[c] : T(0);
+    bt[it] = fly ? sb[c] : (uint8_t)0;
+    m1v[it] = rdi ? msk[c] : (int8_t)0;
+  }
+  if (rdi) {
+#pragma unroll
+    for (int it = 0; it < ITC; it++) {
+      const int q = min(tid + it * NT, NC - 1);
+      const int rr = q / CW, cc = q - rr * CW;
+      const size_t c = at(min(cy0 + rr, C.m - 1), min(cx0 + cc, C.n - 1), C.n);
+      cxv[it] = xc[c];
+      cmv[it] = msk2[c];
+    }
+  }
 #pragma unroll
   for (int it = 0; it < IT2; it++) {
     const int q = tid + it * NT;
@@ -417,64 +455,56 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       bs[rr][cc] = ok2[it] ? bv[it] : T(0);
     }
   }
-  // phase 1: x (and the stencil bits) on rows [y0-2, y0+TY+1] x cols
-  // [x0-2, x0+TX+1], clipped to [1, m-2]; loaded in groups of G1 elements
-  // per thread (all loads of a group in flight at once, registers bounded)
   bool full = true;
-  {
-    constexpr int G1 = 3;
 #pragma unroll
-    for (int g0 = 0; g0 < IT1; g0 += G1) {
-      int jj[G1], ii[G1];
-      bool ok[G1];
+  for (int it = 0; it < IT1; it++) {
+    const int q = tid + it * NT;
+    if (q < N1) {
+      const int rr = q / XW, cc = q - rr * XW;
+      xs[rr][cc] = ok1[it] ? xv[it] : T(0);
+      sbs[rr][cc] = ok1[it] ? bt[it] : (uint8_t)0;
+      ms1[rr][cc] = m1v[it];
+      full = full && ok1[it] && bt[it] == 31;
+    }
+  }
+  if (rdi) {
 #pragma unroll
-      for (int k = 0; k < G1; k++) {
-        const int q = tid + (g0 + k) * NT;
-        const int rr = q / XW, cc = q - rr * XW;
-        const int j = y0 - 2 + rr, i = x0 - 2 + cc;
-        ok[k] = g0 + k < IT1 && q < N1 && j <= m - 2 && i <= n - 2;
-        jj[k] = ok[k] ? j : 1;
-        ii[k] = ok[k] ? i : 1;
-      }
-      T v[G1];
-      uint8_t bits[G1];
-#pragma unroll
-      for (int k = 0; k < G1; k++) {
-        v[k] = T(0);
-        bits[k] = 0;
-      }
-      if (rdx) {
-#pragma unroll
-        for (int k = 0; k < G1; k++) v[k] = xin[at(jj[k], ii[k], n)];
-      }
-      if (fly) {
-#pragma unroll
-        for (int k = 0; k < G1; k++) bits[k] = sb[at(jj[k], ii[k], n)];
-      }
-      if (rdi) {
-        IpOps<T> op[G1];
-#pragma unroll
-        for (int k = 0; k < G1; k++) op[k] = ip_load<T>(msk, n, msk2, xc, C.n, jj[k], ii[k]);
-#pragma unroll
-        for (int k = 0; k < G1; k++) {
-          const T r = ip_eval<T>(op[k]);
-          v[k] = MODE == 2 ? v[k] + r : r;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < G1; k++) {
-        const int q = tid + (g0 + k) * NT;
-        if (g0 + k < IT1 && q < N1) {
-          const int rr = q / XW, cc = q - rr * XW;
-          xs[rr][cc] = ok[k] ? v[k] : T(0);
-          sbs[rr][cc] = ok[k] ? bits[k] : (uint8_t)0;
-          full = full && ok[k] && bits[k] == 31;
-        }
+    for (int it = 0; it < ITC; it++) {
+      const int q = tid + it * NT;
+      if (q < NC) {
+        const int rr = q / CW, cc = q - rr * CW;
+        cxs[rr][cc] = cxv[it];
+        cms[rr][cc] = cmv[it];
       }
     }
-    // every cell of the tile has the full 9-point stencil: constant weights
-    full_tile = __syncthreads_and(fly && full);
+    __syncthreads();
+    // prolongation (ip_eval: interp_at's expressions) of the coarse tile,
+    // added to x (MODE 2) or replacing it (MODE 3)
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+      const int q = tid + it * NT;
+      if (q < N1 && ok1[it]) {
+        const int rr = q / XW, cc = q - rr * XW;
+        const int j = y0 - 2 + rr, i = x0 - 2 + cc;
+        const int J2 = ((j + 2) >> 1) - cy0, I2 = ((i + 2) >> 1) - cx0;
+        IpOps<T> o;
+        o.m1 = ms1[rr][cc];
+        o.a = cxs[J2][I2];
+        o.b = cxs[J2][I2 + 1];
+        o.c = cxs[J2 + 1][I2];
+        o.d = cxs[J2 + 1][I2 + 1];
+        o.sa = cms[J2][I2];
+        o.sb = cms[J2][I2 + 1];
+        o.sc = cms[J2 + 1][I2];
+        o.sd = cms[J2 + 1][I2 + 1];
+        o.par = (i & 1) | ((j & 1) << 1);
+        const T r = ip_eval<T>(o);
+        xs[rr][cc] = MODE == 2 ? xv[it] + r : r;
+      }
+    }
   }
+  // every cell of the tile has the full 9-point stencil: constant weights
+  full_tile = __syncthreads_and(fly && full);
   const T c1 = T(in.omega), c2 = T(1. - in.omega);
   T* xout = Q + tk.xout;
   const int8_t* mp = final_mask ? B + in.mskp : nullptr;
