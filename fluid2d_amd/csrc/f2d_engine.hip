@@ -629,10 +629,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   __shared__ uint8_t sbr[XRH][XRW];
   const bool fly = tk.lev == 0 && in.a0fly;
   int full = 0;
+  R xv[ITX];
+  uint8_t bv[ITX];
   if (!(flags & 2)) {
-    R xv[ITX];
-    uint8_t bv[ITX];
-    bool allfull = true;
 #pragma unroll
     for (int it = 0; it < ITX; it++) {
       const int q = min(tid + it * NT, NX - 1);
@@ -641,6 +640,30 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       xv[it] = x[c];
       bv[it] = fly ? sb[c] : (uint8_t)0;
     }
+  }
+  // the coarse mask of the points restricted, then the rhs and mask of the
+  // points evaluated, issued with the x loads
+  constexpr int NR = TY2V * TX2, ITR = (NR + NT - 1) / NT;
+  int8_t m2v[ITR];
+#pragma unroll
+  for (int it = 0; it < ITR; it++) {
+    const int q = min(tid + it * NT, NR - 1);
+    const int ry = q / TX2, rx = q - ry * TX2;
+    m2v[it] = msk2[at(min(tk.y0 + ry, m2 - 1), min(tk.x0 + rx, n2 - 1), n2)];
+  }
+  R bq[ITP];
+  int8_t mq[ITP];
+#pragma unroll
+  for (int it = 0; it < ITP; it++) {
+    const int q = min(tid + it * NT, RY * RX - 1);
+    const int rr = q / RX, cc = q - rr * RX;
+    const int jf = min(fy0 + rr, m1 - NH), if_ = min(fx0 + cc, n1 - NH);
+    const size_t c = at(img(jf, m1), img(if_, n1), n1);
+    bq[it] = bat(c);
+    mq[it] = msk1[c];
+  }
+  if (!(flags & 2)) {
+    bool allfull = true;
 #pragma unroll
     for (int it = 0; it < ITX; it++) {
       const int q = tid + it * NT;
@@ -654,18 +677,6 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     // the whole region has the full stencil: constant weights
     full = __syncthreads_and(fly && allfull);
   }
-  R bq[ITP];
-  int8_t mq[ITP];
-#pragma unroll
-  for (int it = 0; it < ITP; it++) {
-    const int q = min(tid + it * NT, RY * RX - 1);
-    const int rr = q / RX, cc = q - rr * RX;
-    const int jf = min(fy0 + rr, m1 - NH), if_ = min(fx0 + cc, n1 - NH);
-    const size_t c = at(img(jf, m1), img(if_, n1), n1);
-    bq[it] = bat(c);
-    mq[it] = msk1[c];
-  }
-  __syncthreads();
 #pragma unroll
   for (int it = 0; it < ITP; it++) {
     const int q = tid + it * NT;
@@ -731,12 +742,15 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     partial[2 * ti + 1] = c;
   }
   T* b2 = Q + Lc.b;
-  for (int q = tid; q < TY2V * TX2; q += nt) {
-    const int ry = q / TX2, rx = q % TX2;
+#pragma unroll
+  for (int it = 0; it < ITR; it++) {
+    const int q = tid + it * NT;
+    if (q >= NR) continue;
+    const int ry = q / TX2, rx = q - ry * TX2;
     const int J2 = tk.y0 + ry, I2 = tk.x0 + rx;
     if (J2 >= m2 - NH || I2 >= n2 - NH) continue;
     R v = R(0);
-    if (msk2[at(J2, I2, n2)] != 0) {
+    if (m2v[it] != 0) {
       const int cy = 2 * ry + 1, cx = 2 * rx + 1;
       v = restrict_r<R>([&](int dj, int di) { return rs[cy + dj][cx + di]; });
     }
