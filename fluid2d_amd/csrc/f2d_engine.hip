@@ -551,9 +551,12 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       return jacobi_c3_2d<T>(bc, c3, c2, [&](int k, int, int) { return w[k]; }, X);
     });
   } else if (fly) {
+    // A_5 = w_5 wherever the mask (bit 4) is set, and the value is
+    // discarded where it is not: one c1 / |A_5| serves the tile
+    const T c3 = c1 / fabs(w[4]);
     sweeps([&](size_t, int r, int q, T bc, auto X) {
-      return jacobi_2d<T>((int8_t)((sbs[r][q] >> 4) & 1), bc, c1, c2,
-                          ABitsTile2<T, XW>{&sbs[r][q], w}, X);
+      const T v = jacobi_c3_2d<T>(bc, c3, c2, ABitsTile2<T, XW>{&sbs[r][q], w}, X);
+      return ((sbs[r][q] >> 4) & 1) ? v : T(0);
     });
   } else {
     sweeps([&](size_t c, int, int, T bc, auto X) {
@@ -1577,13 +1580,19 @@ static Task mk_task(int inst, int lev, int y0, int x0) {
 }
 
 // tiles of ty x TX over rows [r0, r1) x cols [c0, c1)
+// Tiles row by row; each row's columns start at a rotation of 5 per row.
+// Workgroups go to the compute units of an XCD round-robin, so without the
+// rotation a unit would get tiles of one column only, and the units holding
+// the (slower) boundary columns would finish last.
 static void tiles(std::vector<Task>& out, const Task& proto, int r0, int r1, int c0, int c1,
                   int ty = TY) {
-  for (int y = r0; y < r1; y += ty)
-    for (int x = c0; x < c1; x += TX) {
+  const int nc = (c1 - c0 + TX - 1) / TX;
+  int row = 0;
+  for (int y = r0; y < r1; y += ty, row++)
+    for (int k = 0; k < nc; k++) {
       Task t = proto;
       t.y0 = y;
-      t.x0 = x;
+      t.x0 = c0 + ((k + 5 * row) % nc) * TX;
       out.push_back(t);
     }
 }
@@ -1667,11 +1676,14 @@ struct Builder {
     t.xin = xin;
     t.xout = rhs;
     t.xc = rstore;
-    for (int y = NH; y < C.m - NH; y += ty2)
-      for (int x = NH; x < C.n - NH; x += TX2) {
+    // rows of coarse tiles, columns rotated per row as in tiles()
+    const int nc = (C.n - 2 * NH + TX2 - 1) / TX2;
+    int row = 0;
+    for (int y = NH; y < C.m - NH; y += ty2, row++)
+      for (int k = 0; k < nc; k++) {
         Task u = t;
         u.y0 = y;
-        u.x0 = x;
+        u.x0 = NH + ((k + 5 * row) % nc) * TX2;
         out.push_back(u);
       }
   }
