@@ -895,35 +895,53 @@ __device__ void c_interp(const CL<T, TA>& F, const CL<T, TA>& C, bool acc, Sync&
   sy.end(cnt);
 }
 
-// A cell's eight off-diagonal couplings (the reads of offdiag_2d), held in
-// registers; the arguments are literals after inlining
+// A cell's operator row in registers: the eight couplings offdiag_2d reads
+// and the centre A_5 (residual_2d); the arguments are literals after
+// inlining
 template <typename T>
 struct ARow {
-  T a0, a1, a2, a3, a3e, a2s, a1s, a0s;
+  T a0, a1, a2, a3, a4, a3e, a2s, a1s, a0s;
   __device__ __forceinline__ T operator()(int k, int dj, int di) const {
-    if (dj == 0 && di == 0) return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
+    if (dj == 0 && di == 0)
+      return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : (k == 3 ? a3 : a4)));
     return k == 3 ? a3e : (k == 2 ? a2s : (k == 1 ? a1s : a0s));
   }
 };
 
-// The coarsest-level solve: `iters` smoothings (two damped-Jacobi sweeps
-// each, hierarchy.py:117-118 / 144-145) of a level entered with x = 0.
-// Thread q owns cell q of the first sweep's region and keeps that cell's
-// couplings, factor c1 / |A_5|, rhs and mask in registers for all sweeps,
-// so a sweep reads only the nine x (or t) values from LDS.  The arithmetic
-// is jacobi_f's, operand for operand.  Levels whose first-sweep region
-// exceeds the workgroup take the generic smoother.
+// A cell of the first sweep's region held by its owning thread: position,
+// operator row, c1 / |A_5|, rhs and mask (read once per level visit)
+template <typename T>
+struct OwnCell {
+  int j, i;
+  int c;
+  bool own, inner;
+  ARow<T> a;
+  T c3, bc;
+  int8_t mk;
+};
+
+// Level visits where each thread (of the workgroup, or of wavefront 0 when
+// the level is small) owns at most OWNK cells of the first sweep's region
+// keep those cells' operands in registers: a sweep, and the residual, read
+// only x or t from LDS.  The arithmetic is jacobi_f's and residual_f's,
+// operand for operand (c1 / |A_5| is the same division, done once).
+constexpr int OWNK = 1;
+
+// `iters` smoothings (two damped-Jacobi sweeps each; the first reads x = 0
+// when zero_in), then with `resid` the residual b - A x on the interior into
+// t with its images (the first half of c_resid_restrict).  Returns false,
+// doing nothing, when the level has more cells than the owners hold.
 template <typename T, typename TA>
-__device__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy, int iters) {
+__device__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in, int iters,
+                             bool resid) {
   const int m = L.m, n = L.n, cells = m * n;
   const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2, w = wj * wi;
-  if (w > sy.nt) {
-    for (int k = 0; k < iters; k++) c_smooth(L, c1, c2, sy, k == 0);
-    return;
-  }
-  // one wavefront when the region fits it (LDS operations of a wavefront
-  // execute in order: a wave-level barrier separates the sweeps)
-  const bool wave = w <= 64;
+  // one wavefront when the region fits it or the level is small (LDS
+  // operations of a wavefront execute in order: a wave-level barrier
+  // separates the phases)
+  const bool wave = w <= 64 || cells <= sy.small_cells;
+  const int st = wave ? 64 : sy.nt;
+  if (w > OWNK * st) return false;
   if (!wave && sy.prev_small) __syncthreads();
   sy.prev_small = wave;
   auto sync = [&]() {
@@ -934,47 +952,109 @@ __device__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy, int iters) 
       __syncthreads();
     }
   };
-  const size_t P = (size_t)cells;
-  const int q = sy.tid;
-  const bool own = q < w;
-  const int qj = own ? qdiv(q, wi) : 0;
-  const int j = NH - 1 + qj, i = NH - 1 + (own ? q - qj * wi : 0);
-  const size_t c = at(j, i, n);
-  const bool inner = own && j >= NH && j < m - NH && i >= NH && i < n - NH;
-  ARow<T> a;
-  T bc = T(0), c3 = T(0);
-  int8_t mk = 0;
-  if (own) {
-    a.a0 = T(L.A[c]);
-    a.a1 = T(L.A[P + c]);
-    a.a2 = T(L.A[2 * P + c]);
-    a.a3 = T(L.A[3 * P + c]);
-    a.a3e = T(L.A[3 * P + c + 1]);
-    a.a2s = T(L.A[2 * P + c + n - 1]);
-    a.a1s = T(L.A[P + c + n]);
-    a.a0s = T(L.A[c + n + 1]);
-    c3 = c1 / fabs(T(L.A[4 * P + c]));
-    bc = L.b[c];
-    mk = L.msk[c];
+  const int P = cells;
+  const bool active = !wave || sy.tid < 64;
+  OwnCell<T> o[OWNK];
+#pragma unroll
+  for (int k = 0; k < OWNK; k++) {
+    const int q = sy.tid + k * st;
+    o[k].own = active && q < w;
+    const int qj = o[k].own ? qdiv(q, wi) : 0;
+    o[k].j = NH - 1 + qj;
+    o[k].i = NH - 1 + (o[k].own ? q - qj * wi : 0);
+    o[k].c = o[k].j * n + o[k].i;
+    o[k].inner = o[k].own && o[k].j >= NH && o[k].j < m - NH && o[k].i >= NH && o[k].i < n - NH;
+    if (o[k].own) {
+      const int c = o[k].c;
+      o[k].a.a0 = T(L.A[c]);
+      o[k].a.a1 = T(L.A[P + c]);
+      o[k].a.a2 = T(L.A[2 * P + c]);
+      o[k].a.a3 = T(L.A[3 * P + c]);
+      o[k].a.a4 = T(L.A[4 * P + c]);
+      o[k].a.a3e = T(L.A[3 * P + c + 1]);
+      o[k].a.a2s = T(L.A[2 * P + c + n - 1]);
+      o[k].a.a1s = T(L.A[P + c + n]);
+      o[k].a.a0s = T(L.A[c + n + 1]);
+      o[k].bc = L.b[c];
+      o[k].mk = L.msk[c];
+    }
   }
+#pragma unroll
+  for (int k = 0; k < OWNK; k++)
+    if (o[k].own) o[k].c3 = c1 / fabs(o[k].a.a4);
   LDSP T* x = L.x;
   LDSP T* t = L.t;
-  for (int k = 0; k < iters; k++) {
-    if (own) {
-      T v;
-      if (k == 0)
-        v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int, int) { return T(0); });
-      else
-        v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return x[c + dj * n + di]; });
-      t[c] = mk == 0 ? T(0) : v;
-    }
+  for (int it = 0; it < iters; it++) {
+    const bool z = zero_in && it == 0;
+#pragma unroll
+    for (int k = 0; k < OWNK; k++)
+      if (o[k].own) {
+        const int c = o[k].c;
+        T v;
+        if (z)
+          v = jacobi_c3_2d<T>(o[k].bc, o[k].c3, c2, o[k].a, [&](int, int) { return T(0); });
+        else
+          v = jacobi_c3_2d<T>(o[k].bc, o[k].c3, c2, o[k].a,
+                              [&](int dj, int di) { return x[c + dj * n + di]; });
+        t[c] = o[k].mk == 0 ? T(0) : v;
+      }
     sync();
-    if (inner) {
-      const T v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return t[c + dj * n + di]; });
-      store_img_mask<T>(x, nullptr, j, i, mk == 0 ? T(0) : v, m, n);
-    }
+#pragma unroll
+    for (int k = 0; k < OWNK; k++)
+      if (o[k].inner) {
+        const int c = o[k].c;
+        const T v = jacobi_c3_2d<T>(o[k].bc, o[k].c3, c2, o[k].a,
+                                    [&](int dj, int di) { return t[c + dj * n + di]; });
+        store_img_mask<T>(x, nullptr, o[k].j, o[k].i, o[k].mk == 0 ? T(0) : v, m, n);
+      }
     sync();
   }
+  if (resid) {
+#pragma unroll
+    for (int k = 0; k < OWNK; k++)
+      if (o[k].inner) {
+        const int c = o[k].c;
+        const T v = residual_2d<T>(o[k].mk, o[k].bc, o[k].a,
+                                   [&](int dj, int di) { return x[c + dj * n + di]; });
+        store_img_mask<T>(t, nullptr, o[k].j, o[k].i, v, m, n);
+      }
+    sync();
+  }
+  return true;
+}
+
+// the second half of c_resid_restrict: full weighting of `src` (level F)
+// onto the coarse interior of C (+ images)
+template <typename T, typename TA>
+__device__ void c_restrict(const CL<T, TA>& F, const CL<T, TA>& C, LDSP T* src, Sync& sy) {
+  const int n = F.n;
+  const int N2 = C.n - 2 * NH, M2 = C.m - 2 * NH, ccells = C.m * C.n;
+  if (sy.begin(ccells))
+    for (int q = sy.tid; q < M2 * N2; q += sy.stride(ccells)) {
+      const int qj = qdiv(q, N2);
+      const int J2 = NH + qj, I2 = NH + (q - qj * N2);
+      T v = T(0);
+      if (C.msk[at(J2, I2, C.n)] != 0) {
+        const size_t c = at(2 * J2 + 1 - NH, 2 * I2 + 1 - NH, n);
+        v = restrict_r<T>([&](int dj, int di) { return src[c + (ptrdiff_t)dj * n + di]; });
+      }
+      store_img_mask<T>(C.b, nullptr, J2, I2, v, C.m, C.n);
+    }
+  sy.end(ccells);
+}
+
+// a smoothing of level L (one smoothtwice), owned or generic
+template <typename T, typename TA>
+__device__ void c_smooth_level(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in) {
+  if (!c_smooth_own(L, c1, c2, sy, zero_in, 1, false)) c_smooth(L, c1, c2, sy, zero_in);
+}
+
+// the coarsest-level solve: 16 smoothings of a level entered with x = 0
+// (hierarchy.py:117-118 / 144-145)
+template <typename T, typename TA>
+__device__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy) {
+  if (!c_smooth_own(L, c1, c2, sy, true, 16, false))
+    for (int k = 0; k < 16; k++) c_smooth(L, c1, c2, sy, k == 0);
 }
 
 // V-cycle from level lev1 down to level deep of the coarse part
@@ -983,13 +1063,18 @@ template <typename T, typename TA>
 __device__ void c_vcycle(const CL<T, TA>* Ls, int lev1, int deep, bool zero_first, T c1, T c2,
                          Sync& sy) {
   for (int l = lev1; l < deep; l++) {
-    c_smooth(Ls[l], c1, c2, sy, l > lev1 || zero_first);
-    c_resid_restrict(Ls[l], Ls[l + 1], false, sy);
+    const bool z = l > lev1 || zero_first;
+    if (c_smooth_own(Ls[l], c1, c2, sy, z, 1, true)) {
+      c_restrict(Ls[l], Ls[l + 1], Ls[l].t, sy);
+    } else {
+      c_smooth(Ls[l], c1, c2, sy, z);
+      c_resid_restrict(Ls[l], Ls[l + 1], false, sy);
+    }
   }
-  c_coarsest(Ls[deep], c1, c2, sy, 16);
+  c_coarsest(Ls[deep], c1, c2, sy);
   for (int l = deep - 1; l >= lev1; l--) {
     c_interp(Ls[l], Ls[l + 1], true, sy);
-    c_smooth(Ls[l], c1, c2, sy);
+    c_smooth_level(Ls[l], c1, c2, sy, false);
   }
 }
 
@@ -1087,7 +1172,7 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
     c_vcycle(Ls, 0, nl - 1, true, c1, c2, sy);
   } else {
     for (int l = 0; l < nl - 1; l++) c_resid_restrict(Ls[l], Ls[l + 1], true, sy);
-    c_coarsest(Ls[nl - 1], c1, c2, sy, 16);
+    c_coarsest(Ls[nl - 1], c1, c2, sy);
     for (int l = nl - 2; l >= 0; l--) {
       c_interp(Ls[l], Ls[l + 1], false, sy);
       c_vcycle(Ls, l, nl - 1, false, c1, c2, sy);
