@@ -190,18 +190,30 @@ def cpu_baseline(name, seconds):
                       "gcc -O2 -ffp-contract=off, %.1f s" % (name, steps, el)}
 
 
-def load_traffic(kernel, workload, tuning):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), when
-    it was measured with the same launch configuration."""
+def finest_tile_height(nx, ny, min_tiles):
+    """Tile height of the finest-level smoother launches: the tallest of
+    16, 8, 4 rows (64 columns) giving at least min_tiles tiles."""
+    if min_tiles <= 0:
+        return 16
+    for ty in (16, 8):
+        if nx * ny / (64.0 * ty) >= min_tiles:
+            return ty
+    return 4
+
+
+def load_traffic(kernel, workload, ty):
+    """HBM bytes per launch of the finest-level launches of `kernel` with
+    tile height `ty`, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
     path = os.path.join(REPO, "profiles", "traffic_%s_%s.json" % (kernel, workload))
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         t = json.load(f)
-    if t.get("tuning") and tuning and t["tuning"].get("min_tiles") != tuning.get("min_tiles"):
-        return None, os.path.relpath(path, REPO) + " (other tile configuration)"
-    return t.get("bytes_per_launch"), os.path.relpath(path, REPO)
+    key = "%s<double, %d>" % (kernel, ty)
+    if key not in t:
+        return None, os.path.relpath(path, REPO) + " (no entry for %s)" % key
+    return t[key].get("bytes_per_launch"), os.path.relpath(path, REPO) + " [%s]" % key
 
 
 def sequential_plain(n, tune, steps=20):
@@ -262,6 +274,19 @@ def main():
         F.load().f2d_synchronize()
         log("copy: 5 x %d bytes" % nbytes)
         del src, dst
+        # the same volume through a streaming stencil kernel of this library
+        # (celltocorner: each cell read once from HBM, its neighbours from
+        # cache, one value written), whose loads are plain global loads like
+        # the solver's
+        from fluid2d_amd import kernels
+        m, n = 4096, nbytes // 8 // 4096
+        xs = DeviceArray((m, n), np.float64)
+        xd = DeviceArray((m, n), np.float64)
+        for _ in range(5):
+            kernels.celltocorner(xs, xd)
+        F.load().f2d_synchronize()
+        log("stream copy (celltocorner): 5 x %d bytes" % nbytes)
+        del xs, xd
     global USE_GRAPH, FIXED_TUNING
     USE_GRAPH = not a.eager
     if a.tuning:
@@ -303,7 +328,10 @@ def main():
     if "roofline" in legs and b is not None:
         dom, shares, pr, achieved = roofline(b)
         kname = KERNEL_NAMES[dom] + "<double>"
-        traffic, src = load_traffic(KERNEL_NAMES[dom], name, out["config"].get("tuning"))
+        tun = out["config"].get("tuning") or {}
+        ty = finest_tile_height(out["config"]["nx"], out["config"]["ny"],
+                                int(tun.get("min_tiles", 0)))
+        traffic, src = load_traffic(KERNEL_NAMES[dom], name, ty)
         out["roofline"] = {
             "bound": "hbm",
             "kernel": kname + (" (all launches)" if dom == "coarse" else " finest-level launches"),

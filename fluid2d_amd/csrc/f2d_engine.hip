@@ -2101,8 +2101,13 @@ struct Builder {
 };
 
 // ---- execution ------------------------------------------------------------
-// Optional timing of one launch kind (eager runs only): events recorded
-// on the batch stream around every matching launch.
+// Optional timing of one launch kind (eager runs only): every matching
+// launch is issued PROF_REP times back to back between one pair of events
+// on the batch stream, so the events' own latency is spread over the
+// repeats.  Every launch kind reads buffers it does not write (smoother
+// ping-pong buffers, residual into its own buffer, tendencies from stage
+// states), so a repeat leaves the state as one launch does.
+constexpr int PROF_REP = 8;
 struct ProfSel {
   int kind = -1, level = -1;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -2147,7 +2152,8 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
       prof->which.push_back(&L);
       timed = true;
     }
-    switch (L.kind) {
+    const int nrep = timed ? PROF_REP : 1;
+    for (int rep = 0; rep < nrep; rep++) switch (L.kind) {
       case L_RHS:
         F2D_LAUNCH(k_rhs<T>, dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
                            L.outk);
@@ -2849,7 +2855,7 @@ int f2d_batch_profile(f2d_batch* b, int kind, int level, int nsteps, double* avg
   for (size_t q = 0; q < ps.ev.size(); q++) {
     float ms = 0.f;
     ECHK(hipEventElapsedTime(&ms, ps.ev[q].first, ps.ev[q].second));
-    tot += ms;
+    tot += ms / PROF_REP;
     double a, im;
     launch_bytes(*b, *ps.which[q], &a, &im);
     ab += a;

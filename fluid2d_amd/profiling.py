@@ -9,8 +9,11 @@ roofline.
 
 ``traffic`` scales the per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the
 kernel launches with grid size GRID_X by what the same counters read for
-the plain copies of known size in the same run (the runtime's copy kernel,
-``bench.py --legs copy,...``: 5 copies of 256 MiB).
+the plain copies of known size in the same run (``bench.py --legs
+copy,...``: 5 passes of the library's celltocorner kernel over 256 MiB
+arrays, a streaming kernel with plain global loads; the runtime's copy
+kernel is timed beside it).  The entry for KERNEL is added to OUT_JSON
+(a dict keyed by kernel name), keeping the entries of other kernels.
 """
 import json
 import sqlite3
@@ -41,11 +44,11 @@ def pmc_table(path, counter, pattern="%"):
     return [(r[0].split("(")[0], r[1], r[2], r[3]) for r in cur.execute(q, (counter, pattern))]
 
 
-def _copy_kib(path, counter):
-    """Total counter value (KiB) over the dispatches of the calibration copy
-    (the runtime copy kernel at its largest grid; one copy may be split
-    over several dispatches)."""
-    rows = pmc_table(path, counter, "%copyBuffer%")
+def _copy_kib(path, counter, pattern="%copyBuffer%"):
+    """Total counter value (KiB) over the dispatches of a calibration copy
+    (the kernel matching `pattern` at its largest grid; one copy may be
+    split over several dispatches)."""
+    rows = pmc_table(path, counter, pattern)
     if not rows:
         return None
     big = max(r[1] for r in rows)
@@ -54,18 +57,25 @@ def _copy_kib(path, counter):
 
 def traffic(fetch_db, write_db, kernel, grid_x, copy_bytes=5 * (256 << 20)):
     """HBM bytes per launch of `kernel` launches with grid size `grid_x`:
-    FETCH_SIZE and WRITE_SIZE scaled so that the calibration copies
-    (`copy_bytes` in total) read and write exactly `copy_bytes`."""
-    out = {"kernel": kernel, "grid_x": grid_x, "copy_bytes": copy_bytes}
+    FETCH_SIZE and WRITE_SIZE scaled so that the five streaming calibration
+    passes (celltocorner over 256 MiB arrays; `copy_bytes` read and written
+    in total) read and write exactly `copy_bytes`.  The scale from the
+    runtime's copy kernel is reported beside it; that one also counts any
+    other copies of the same grid size in the run."""
+    out = {"kernel": kernel, "grid_x": grid_x, "copy_bytes": copy_bytes,
+           "calibration": "k_celltocorner<double>, 5 passes over 256 MiB"}
     tot = 0.0
     for name, path in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
         rows = [r for r in pmc_table(path, name, "%" + kernel + "%") if r[1] == grid_x]
         if not rows:
             raise ValueError("no %s rows for %s grid %d" % (name, kernel, grid_x))
         kib = sum(r[3] * r[2] for r in rows) / sum(r[2] for r in rows)
+        skib = _copy_kib(path, name, "%k_celltocorner%")
         ckib = _copy_kib(path, name)
-        factor = copy_bytes / (ckib * 1024.0) if ckib else 1.0
-        out[name] = {"kib_per_dispatch": kib, "copy_kib_total": ckib, "scale": factor,
+        factor = copy_bytes / (skib * 1024.0) if skib else 1.0
+        cfactor = copy_bytes / (ckib * 1024.0) if ckib else None
+        out[name] = {"kib_per_dispatch": kib, "calibration_kib_total": skib, "scale": factor,
+                     "runtime_copy_kib_total": ckib, "runtime_copy_scale": cfactor,
                      "bytes_per_launch": kib * 1024.0 * factor}
         tot += kib * 1024.0 * factor
     out["bytes_per_launch"] = tot
@@ -92,8 +102,16 @@ def main(argv):
         t = traffic(argv[1], argv[2], argv[3], int(argv[4]))
         if len(argv) > 6:
             t["tuning"] = json.loads(argv[6])
+        try:
+            with open(argv[5]) as f:
+                allk = json.load(f)
+        except (OSError, ValueError):
+            allk = {}
+        if "kernel" in allk:  # a single-kernel file: keep it under its name
+            allk = {allk["kernel"]: allk}
+        allk[argv[3]] = t
         with open(argv[5], "w") as f:
-            json.dump(t, f, indent=1)
+            json.dump(allk, f, indent=1, sort_keys=True)
         print(json.dumps(t))
     else:
         print(__doc__)

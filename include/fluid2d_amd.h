@@ -268,10 +268,12 @@ int f2d_batch_step_async(f2d_batch *b, int nsteps);
 int f2d_batch_synchronize(f2d_batch *b);
 /* done = 1 when all work enqueued on the batch's stream has completed */
 int f2d_batch_query(f2d_batch *b, int *done);
-/* Runs nsteps steps eagerly with HIP events around every launch of kind
- * (0 smoother, 1 residual/restriction, 2 advection, 3 coarse levels) at
- * top-down level `level` (-1: any).  avg_ms: mean launch duration;
- * alg_bytes / impl_bytes: mean algorithmic / implemented bytes per launch. */
+/* Runs nsteps steps eagerly; every launch of kind (0 smoother, 1
+ * residual/restriction, 2 advection, 3 coarse levels) at top-down level
+ * `level` (-1: any) is issued 8 times back to back between a pair of HIP
+ * events (the repeats leave the state as one launch does).  avg_ms: mean
+ * duration per launch (elapsed / 8); alg_bytes / impl_bytes: mean
+ * algorithmic / implemented bytes per launch. */
 int f2d_batch_profile(f2d_batch *b, int kind, int level, int nsteps, double *avg_ms,
                       long long *nlaunch, double *alg_bytes, double *impl_bytes);
 /* out8 as f2d_model_diagnostics (raw sums; maxspeed and ke scaled) */
