@@ -134,12 +134,14 @@ struct TrVal {
   int8_t m;
 };
 
-template <typename T>
+// NTR: tracers advected (1 Euler, 2 Boussinesq); each launch holds instances
+// of one model
+template <typename T, int NTR>
 __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
                                              const InstD* __restrict__ I, double* P, T* Q,
                                              const int8_t* __restrict__ B,
                                              const DynD* __restrict__ dyn, int stage, int out_k) {
-  __shared__ double qs[2][QH * QW];
+  __shared__ double qs[NTR][QH * QW];
   __shared__ double us[TY * (TX + 1)];
   __shared__ double vs[(TY + 1) * TX];
   __shared__ int8_t ms[QH * QW];
@@ -166,16 +168,14 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
     S.c = stage == 1 ? dt / 3. : 0.5 * dt;
   }
   const int8_t* msk = B + in.msk;
-  const int ntr = in.model == 0 ? 1 : 2;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
   const int y0 = tk.y0, x0 = tk.x0;
   // tiles: tracer rows [y0-3, y0+TY+2] x cols [x0-3, x0+TX+2]; each tile's
   // loads are issued together (compile-time trip counts, clamped in-bounds
   // addresses), then stored
   (void)nt;
-  auto load_tiles = [&](auto kind, auto ntrc) {
+  auto load_tiles = [&](auto kind) {
     constexpr int K = decltype(kind)::value;
-    constexpr int NTR = decltype(ntrc)::value;
     tile_load<(QH * QW + 255) / 256>(tid, QH * QW, [&](int q) {
       const int r = q / QW, c = q - r * QW;
       const int j = y0 - 3 + r, i = x0 - 3 + c;
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
     }, [&](int q, const TrVal& v) {
       ms[q] = v.m;
       qs[0][q] = v.q0;
-      qs[1][q] = v.q1;
+      if constexpr (NTR > 1) qs[NTR - 1][q] = v.q1;
     });
     tile_load<(TY * (TX + 1) + 255) / 256>(tid, TY * (TX + 1), [&](int q) {
       const int r = q / (TX + 1), c = q - r * (TX + 1);  // u on rows y0.., cols x0-1..
@@ -210,18 +210,15 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
       return ok ? v : 0.;
     }, [&](int q, double v) { vs[q] = v; });
   };
-  auto by_kind = [&](auto ntrc) {
+  auto by_kind = [&]() {
     switch (S.kind) {
-      case 0: load_tiles(std::integral_constant<int, 0>(), ntrc); break;
-      case 1: load_tiles(std::integral_constant<int, 1>(), ntrc); break;
-      case 2: load_tiles(std::integral_constant<int, 2>(), ntrc); break;
-      default: load_tiles(std::integral_constant<int, 3>(), ntrc); break;
+      case 0: load_tiles(std::integral_constant<int, 0>()); break;
+      case 1: load_tiles(std::integral_constant<int, 1>()); break;
+      case 2: load_tiles(std::integral_constant<int, 2>()); break;
+      default: load_tiles(std::integral_constant<int, 3>()); break;
     }
   };
-  if (ntr > 1)
-    by_kind(std::integral_constant<int, 2>());
-  else
-    by_kind(std::integral_constant<int, 1>());
+  by_kind();
   __syncthreads();
   // advection parameters (core/fortran_advection.f90:46-60), cst[3] = umax
   UpParams<double> p;
@@ -277,7 +274,8 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
       }
       Dc = -p.zdx * (ce - cw) - p.zdy * (cn - cs);
     }
-    for (int t = 0; t < ntr; t++) {
+#pragma unroll
+    for (int t = 0; t < NTR; t++) {
       const double* qt = qs[t];
       const double qc = MIXED ? qt[lq] : 0.;
       // tracer at offset k (T-rounded; relative to qc in mixed mode)
@@ -302,13 +300,16 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
       T val = -pt.zdx * (fe - fw) - pt.zdy * (fn - fs);
       if (MIXED) val = T((double)val + qc * Dc);
       const int8_t mc = ms[lq];
-      if (in.model == 1 && t == 0) {  // y *= msk; add_torque
-        val = val * T(mc);
-        int ml = mc + ms[lq - 1], mr = ms[lq + 1] + mc;
-        ml = ml < 1 ? 1 : ml;
-        mr = mr < 1 ? 1 : mr;
-        if (ml + mr == 4)
-          val = val + T(qs[1][lq + 1] - qs[1][lq - 1]) * T(0.5 * in.gravity / in.dx) * T(mc);
+      if constexpr (NTR > 1) {
+        if (t == 0) {  // Boussinesq: y *= msk; add_torque
+          val = val * T(mc);
+          int ml = mc + ms[lq - 1], mr = ms[lq + 1] + mc;
+          ml = ml < 1 ? 1 : ml;
+          mr = mr < 1 ? 1 : mr;
+          if (ml + mr == 4)
+            val = val + T(qs[NTR - 1][lq + 1] - qs[NTR - 1][lq - 1]) * T(0.5 * in.gravity / in.dx) *
+                            T(mc);
+        }
       }
       if (in.diffusion && kforce && mc == 1) {  // add_diffusion
         const double coefd =
@@ -2065,14 +2066,19 @@ struct Builder {
     const bool ssp = b.H[0].d.tscheme == 0;
     const int kforcing = ssp ? 2 : 0;
     for (int stage = 0; stage < 3; stage++) {
-      std::vector<Task> ts;
-      for (int i : who) {
-        HostInst& h = b.H[i];
-        tiles(ts, mk_task(i, 0, 0, 0), NH, h.d.m - NH, NH, h.d.n - NH);
+      // one advection launch per model (tracer count)
+      for (int model = 0; model < 2; model++) {
+        std::vector<Task> ts;
+        for (int i : who) {
+          HostInst& h = b.H[i];
+          if (h.d.model == model) tiles(ts, mk_task(i, 0, 0, 0), NH, h.d.m - NH, NH, h.d.n - NH);
+        }
+        if (ts.empty()) continue;
+        Launch& L = add(L_RHS, ts);
+        L.stage = stage;
+        L.outk = stage;
+        L.flags = model + 1;
       }
-      Launch& L = add(L_RHS, ts);
-      L.stage = stage;
-      L.outk = stage;
       // Euler inverts except at stage kforcing (unless diffusion);
       // Boussinesq inverts at every stage (core/euler.py:156-169,
       // core/boussinesq.py:114)
@@ -2155,8 +2161,12 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
     const int nrep = timed ? PROF_REP : 1;
     for (int rep = 0; rep < nrep; rep++) switch (L.kind) {
       case L_RHS:
-        F2D_LAUNCH(k_rhs<T>, dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
-                           L.outk);
+        if (L.flags == 2)
+          F2D_LAUNCH((k_rhs<T, 2>), dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
+                     L.outk);
+        else
+          F2D_LAUNCH((k_rhs<T, 1>), dim3(g), B2, 0, b.s, TK, b.dI, P, Q, b.B, b.dD, L.stage,
+                     L.outk);
         break;
       case L_C2C:
         if (L.flags == 1)
