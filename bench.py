@@ -114,7 +114,9 @@ def config_batch(name, precision, dt=None, tune=False):
     return b, g, dt
 
 
-def ensemble_batch(n, precision, dts=None, tune=False):
+def ensemble_batch(n, precision, dts=None, tune=False, perturb=0.0):
+    """The ensemble as one batch; `perturb` > 0 multiplies every initial
+    vorticity value by (1 + perturb * r), r seeded uniform in [-1, 1]."""
     from fluid2d_amd import workloads
     from fluid2d_amd.batch import Batch
     from fluid2d_amd.grid import Grid
@@ -126,8 +128,12 @@ def ensemble_batch(n, precision, dts=None, tune=False):
         p = Param(**kw)
         pg.append((p, Grid(p)))
     b = Batch(pg, precision=precision, use_graph=USE_GRAPH, tuning=tuned(pg, precision, tune))
+    rng = np.random.default_rng(12345)
     for i, (_, seed) in enumerate(specs):
-        b.set_field(i, "vorticity", workloads.turbulence(b.grids[i], seed))
+        w = workloads.turbulence(b.grids[i], seed)
+        if perturb > 0:
+            w = w * (1.0 + perturb * rng.uniform(-1.0, 1.0, w.shape))
+        b.set_field(i, "vorticity", w)
     b.set_psi_from_vorticity()
     if dts is None:
         dts = [0.8 * min(g.dx, g.dy) / b.diagnostics(i)["maxspeed"] for i, g in enumerate(b.grids)]
@@ -373,10 +379,25 @@ def main():
             fb.step(2)
             el64 = timed(fb, a.ensemble_steps - 2)
             out["batched"]["fp64_value"] = ecells * (a.ensemble_steps - 2) / el64
-            err = max(max_rel(mixed_states[i], fb.state(i)) for i in range(fb.n))
+            f64_states = [fb.state(i) for i in range(fb.n)]
+            sizes = [g.nx for g in fb.grids]
             del fb
-            out.setdefault("precision", {})["ensemble_max_rel_l2"] = err
+            errs = [max_rel(mixed_states[i], f64_states[i]) for i in range(len(sizes))]
+            out.setdefault("precision", {})["ensemble_max_rel_l2"] = max(errs)
             out["precision"]["ensemble_steps"] = a.ensemble_steps
+            out["precision"]["ensemble_max_rel_l2_by_size"] = {
+                str(nx): max(e for e, s_ in zip(errs, sizes) if s_ == nx) for nx in sorted(set(sizes))}
+            # the flows' own sensitivity: FP64 from initial vorticity perturbed
+            # by half an FP32 ulp (relative 2^-24), against unperturbed FP64
+            log("ensemble FP64 from initial vorticity perturbed by 2^-24, %d steps" %
+                a.ensemble_steps)
+            pb, _, _ = ensemble_batch(a.ensemble, "fp64", dts, tune=not a.no_tune,
+                                      perturb=2.0 ** -24)
+            pb.step(a.ensemble_steps)
+            perrs = [max_rel(pb.state(i), f64_states[i]) for i in range(len(sizes))]
+            del pb
+            out["precision"]["ensemble_fp64_perturbed_2^-24_max_rel_l2_by_size"] = {
+                str(nx): max(e for e, s_ in zip(perrs, sizes) if s_ == nx) for nx in sorted(set(sizes))}
         else:
             del eb
     if "precision" in legs and plain_state is not None:
