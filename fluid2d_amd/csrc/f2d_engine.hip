@@ -436,7 +436,9 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
     const size_t c = ok1[it] ? at(j, i, n) : at(1, 1, n);
     xv[it] = rdx ? xin[c] : T(0);
     bt[it] = fly ? sb[c] : (uint8_t)0;
-    m1v[it] = rdi ? msk[c] : (int8_t)0;
+    // the fine mask of the prolongation: bit 4 of the stencil bits where
+    // they exist (the level's 0/1 mask), else loaded
+    m1v[it] = (rdi && !fly) ? msk[c] : (int8_t)0;
   }
   if (rdi) {
 #pragma unroll
@@ -464,7 +466,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       const int rr = q / XW, cc = q - rr * XW;
       xs[rr][cc] = ok1[it] ? xv[it] : T(0);
       sbs[rr][cc] = ok1[it] ? bt[it] : (uint8_t)0;
-      ms1[rr][cc] = m1v[it];
+      ms1[rr][cc] = fly ? (int8_t)((bt[it] >> 4) & 1) : m1v[it];
       full = full && ok1[it] && bt[it] == 31;
     }
   }
