@@ -275,24 +275,21 @@ __device__ __forceinline__ IpOps<T> ip_load(const int8_t* msk1, int n1, const in
   o.par = (i & 1) | ((j & 1) << 1);
   return o;
 }
+// Evaluated without a branch: all four parity forms are computed and the
+// one of this point's parity is selected (a wavefront holds every parity,
+// so branches would run all four forms one after another anyway).
 template <typename T>
 __device__ __forceinline__ T ip_eval(const IpOps<T>& o) {
   auto c2 = [](int s) { return s == 2 ? T(0.5) : (s == 1 ? T(1) : T(0)); };
   const int pj = o.par >> 1, pi = o.par & 1;
-  T r;
-  if (!pj && !pi) {
-    r = o.a;
-  } else if (!pj) {
-    r = (o.a + o.b) * c2(o.sa + o.sb);
-  } else if (!pi) {
-    r = (o.a + o.c) * c2(o.sa + o.sc);
-  } else {
-    const int s4 = o.sa + o.sb + o.sc + o.sd;
-    const T w = s4 == 4 ? T(0.25)
-                        : (s4 == 3 ? T((double)0.3333333333333333333333333333f)
-                                   : (s4 == 2 ? T(0.5) : (s4 == 1 ? T(1) : T(0))));
-    r = w * (o.a + o.b + o.c + o.d);
-  }
+  const T r1 = (o.a + o.b) * c2(o.sa + o.sb);
+  const T r2 = (o.a + o.c) * c2(o.sa + o.sc);
+  const int s4 = o.sa + o.sb + o.sc + o.sd;
+  const T w = s4 == 4 ? T(0.25)
+                      : (s4 == 3 ? T((double)0.3333333333333333333333333333f)
+                                 : (s4 == 2 ? T(0.5) : (s4 == 1 ? T(1) : T(0))));
+  const T r3 = w * (o.a + o.b + o.c + o.d);
+  const T r = pj ? (pi ? r3 : r2) : (pi ? r1 : o.a);
   return o.m1 <= 0 ? T(0) : r;
 }
 

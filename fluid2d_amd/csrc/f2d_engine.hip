@@ -57,6 +57,18 @@ __device__ __forceinline__ int task_index() {
   return x * q + min(x, r) + i;
 }
 
+// AND of a predicate over the 256 threads of a (64, 4) workgroup: one vote
+// per wavefront, combined through LDS.  (The runtime's block-wide AND reads
+// the block size from the dispatch arguments in the middle of the kernel, a
+// memory round trip on the critical path.)  Once per kernel.
+__device__ __forceinline__ int block_all4(bool p) {
+  __shared__ int wf[4];
+  const int w = __all(p);
+  if (threadIdx.x == 0) wf[threadIdx.y] = w;
+  __syncthreads();
+  return wf[0] & wf[1] & wf[2] & wf[3];
+}
+
 // Step gate (first launch of a step): an instance runs this step while its
 // step budget lasts; every kernel of the step skips instances that do not.
 __global__ void k_gate(DynD* dyn, int ninst) {
@@ -419,7 +431,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   constexpr int CH = TYV / 2 + 4, CW = TX / 2 + 4, NC = CH * CW, ITC = (NC + NT - 1) / NT;
   __shared__ T cxs[CH][CW];
   __shared__ int8_t cms[CH][CW];
-  __shared__ int8_t ms1[XH][XW];
+  __shared__ int8_t ms1[XH][XW];  // fine mask of the prolongation
   const int cy0 = y0 >> 1, cx0 = x0 >> 1;
   bool ok1[IT1];
   T xv[IT1];
@@ -482,32 +494,41 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
     }
     __syncthreads();
     // prolongation (ip_eval: interp_at's expressions) of the coarse tile,
-    // added to x (MODE 2) or replacing it (MODE 3)
+    // added to x (MODE 2) or replacing it (MODE 3).  Work goes by coarse
+    // cell: coarse (J2, I2) and its right/lower neighbours give the four
+    // fine points (2 J2 - 2 + dj, 2 I2 - 2 + di), whose parities are
+    // (dj, di), so the operands are read once for four points.
+    constexpr int CRH = XH / 2 + 1, CRW = XW / 2 + 1, NCR = CRH * CRW;
 #pragma unroll
-    for (int it = 0; it < IT1; it++) {
+    for (int it = 0; it < (NCR + NT - 1) / NT; it++) {
       const int q = tid + it * NT;
-      if (q < N1 && ok1[it]) {
-        const int rr = q / XW, cc = q - rr * XW;
-        const int j = y0 - 2 + rr, i = x0 - 2 + cc;
-        const int J2 = ((j + 2) >> 1) - cy0, I2 = ((i + 2) >> 1) - cx0;
-        IpOps<T> o;
-        o.m1 = ms1[rr][cc];
-        o.a = cxs[J2][I2];
-        o.b = cxs[J2][I2 + 1];
-        o.c = cxs[J2 + 1][I2];
-        o.d = cxs[J2 + 1][I2 + 1];
-        o.sa = cms[J2][I2];
-        o.sb = cms[J2][I2 + 1];
-        o.sc = cms[J2 + 1][I2];
-        o.sd = cms[J2 + 1][I2 + 1];
-        o.par = (i & 1) | ((j & 1) << 1);
-        const T r = ip_eval<T>(o);
-        xs[rr][cc] = MODE == 2 ? xv[it] + r : r;
-      }
+      if (q >= NCR) continue;
+      const int a = q / CRW, bc = q - a * CRW;
+      IpOps<T> o;
+      o.a = cxs[a][bc];
+      o.b = cxs[a][bc + 1];
+      o.c = cxs[a + 1][bc];
+      o.d = cxs[a + 1][bc + 1];
+      o.sa = cms[a][bc];
+      o.sb = cms[a][bc + 1];
+      o.sc = cms[a + 1][bc];
+      o.sd = cms[a + 1][bc + 1];
+#pragma unroll
+      for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+        for (int di = 0; di < 2; di++) {
+          const int j = 2 * (cy0 + a) - 2 + dj, i = 2 * (cx0 + bc) - 2 + di;
+          const int rr = j - (y0 - 2), cc = i - (x0 - 2);
+          if (rr < 0 || rr >= XH || cc < 0 || cc >= XW || j > m - 2 || i > n - 2) continue;
+          o.m1 = ms1[rr][cc];
+          o.par = di | (dj << 1);
+          const T r = ip_eval<T>(o);
+          xs[rr][cc] = MODE == 2 ? xs[rr][cc] + r : r;
+        }
     }
   }
   // every cell of the tile has the full 9-point stencil: constant weights
-  full_tile = __syncthreads_and(fly && full);
+  full_tile = block_all4(fly && full);
   const T c1 = T(in.omega), c2 = T(1. - in.omega);
   T* xout = Q + tk.xout;
   const int8_t* mp = final_mask ? B + in.mskp : nullptr;
@@ -681,7 +702,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       }
     }
     // the whole region has the full stencil: constant weights
-    full = __syncthreads_and(fly && allfull);
+    full = block_all4(fly && allfull);
   }
 #pragma unroll
   for (int it = 0; it < ITP; it++) {
