@@ -40,8 +40,10 @@ namespace eng {
 // ---------------------------------------------------------------------------
 // device helpers
 // ---------------------------------------------------------------------------
+// both flags read at once, no branch between the loads
 __device__ __forceinline__ bool skip(const DynD* dyn, int inst, int pred) {
-  return !dyn[inst].run || (pred && !dyn[inst].active);
+  const int run = dyn[inst].run, act = dyn[inst].active;
+  return (run == 0) | ((pred != 0) & (act == 0));
 }
 __device__ __forceinline__ bool idle(const DynD* dyn, int inst) { return !dyn[inst].run; }
 
@@ -387,7 +389,9 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   __shared__ uint8_t sbs[XH][XW];  // stencil bits (finest level)
   const int ti = task_index();
   const Task tk = tasks[ti];
-  if (skip(dyn, tk.inst, pred)) return;
+  // the skip test is taken after the tile loads are issued, so that the
+  // metadata, the flags and the tile data cost one memory round trip each
+  const bool skp = skip(dyn, tk.inst, pred);
   const int MODE = tk.mode;
   const int final_mask = tk.flags & 1;
   const InstD& in = I[tk.inst];
@@ -462,6 +466,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       cmv[it] = msk2[c];
     }
   }
+  if (skp) return;
 #pragma unroll
   for (int it = 0; it < IT2; it++) {
     const int q = tid + it * NT;
@@ -613,13 +618,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   const int ti = task_index();
   const Task tk = tasks[ti];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
-  if (skip(dyn, tk.inst, pred)) {
-    if ((flags & 12) && tid == 0) {
-      partial[2 * ti] = 0.;
-      partial[2 * ti + 1] = 0.;
-    }
-    return;
-  }
+  // the skip test is taken after the tile loads are issued (one round trip
+  // for the metadata and flags, one for the data)
+  const bool skp = skip(dyn, tk.inst, pred);
   const InstD& in = I[tk.inst];
   const LevD& Lf = in.L[tk.lev];
   const LevD& Lc = in.L[tk.lev + 1];
@@ -688,6 +689,13 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     const size_t c = at(img(jf, m1), img(if_, n1), n1);
     bq[it] = bat(c);
     mq[it] = msk1[c];
+  }
+  if (skp) {
+    if ((flags & 12) && tid == 0) {
+      partial[2 * ti] = 0.;
+      partial[2 * ti + 1] = 0.;
+    }
+    return;
   }
   if (!(flags & 2)) {
     bool allfull = true;
