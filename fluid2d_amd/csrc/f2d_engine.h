@@ -67,6 +67,12 @@ struct Task {
   int inst, lev, y0, x0;
   long long xin, xout, xc;  // buffer offsets used by the phase
   int mode, flags;          // phase variant (smoother input form, final mask)
+  // geometry of level `lev` and of the next coarser level, copied from the
+  // instance's level table when the task list is uploaded, so that a tile
+  // kernel can address its data right after reading its task
+  int m, n, m2, n2;
+  long long b, A, msk, msk2;  // level rhs, operator (T pool), mask; coarser mask (byte pool)
+  long long sb;               // finest-level stencil bits (byte pool), -1 where the level has none
 };
 
 }  // namespace eng

@@ -389,26 +389,26 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   __shared__ uint8_t sbs[XH][XW];  // stencil bits (finest level)
   const int ti = task_index();
   const Task tk = tasks[ti];
-  // the skip test is taken after the tile loads are issued, so that the
-  // metadata, the flags and the tile data cost one memory round trip each
-  const bool skp = skip(dyn, tk.inst, pred);
+  // the skip test is read and taken after the tile loads are issued (the
+  // task holds the addressing, so the data loads follow the task read)
   const int MODE = tk.mode;
   const int final_mask = tk.flags & 1;
   const InstD& in = I[tk.inst];
-  const LevD& L = in.L[tk.lev];
-  const int m = L.m, n = L.n;
+  const int m = tk.m, n = tk.n;
   const size_t PL = (size_t)m * n;
-  const int8_t* msk = B + L.msk;
-  const T* A = Q + L.A;
-  const T* b = Q + L.b;
+  const int8_t* msk = B + tk.msk;
+  const T* A = Q + tk.A;
+  const T* b = Q + tk.b;
   const T* xin = Q + tk.xin;
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.y * 64 + threadIdx.x;
   const int y0 = tk.y0, x0 = tk.x0;
-  const bool fly = tk.lev == 0 && in.a0fly;
-  const uint8_t* sb = (const uint8_t*)(B + in.sb0);
+  const bool fly = tk.sb >= 0;
+  const uint8_t* sb = (const uint8_t*)(B + (fly ? tk.sb : 0));
   const bool rdx = MODE == 0 || MODE == 2, rdi = MODE == 2 || MODE == 3;
-  const LevD& C = in.L[tk.lev + (rdi ? 1 : 0)];
-  const int8_t* msk2 = B + C.msk;
+  struct {
+    int m, n;
+  } C = {tk.m2, tk.n2};
+  const int8_t* msk2 = B + tk.msk2;
   const T* xc = Q + tk.xc;
   int full_tile = 0;
   // One round trip: every global load of the tile is issued before the
@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       cmv[it] = msk2[c];
     }
   }
-  if (skp) return;
+  if (skip(dyn, tk.inst, pred)) return;
 #pragma unroll
   for (int it = 0; it < IT2; it++) {
     const int q = tid + it * NT;
@@ -618,21 +618,18 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   const int ti = task_index();
   const Task tk = tasks[ti];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
-  // the skip test is taken after the tile loads are issued (one round trip
-  // for the metadata and flags, one for the data)
-  const bool skp = skip(dyn, tk.inst, pred);
+  // the skip test is read and taken after the tile loads are issued (the
+  // task holds the addressing, so the data loads follow the task read)
   const InstD& in = I[tk.inst];
-  const LevD& Lf = in.L[tk.lev];
-  const LevD& Lc = in.L[tk.lev + 1];
-  const int m1 = Lf.m, n1 = Lf.n, m2 = Lc.m, n2 = Lc.n;
+  const int m1 = tk.m, n1 = tk.n, m2 = tk.m2, n2 = tk.n2;
   const size_t PL = (size_t)m1 * n1;
-  const int8_t* msk1 = B + Lf.msk;
-  const int8_t* msk2 = B + Lc.msk;
+  const int8_t* msk1 = B + tk.msk;
+  const int8_t* msk2 = B + tk.msk2;
   // operands in RP: the level's A, x and b, or (flag 32) the FP64 level-0
   // operator with an explicit rhs buffer (flag 16) -- the outer residual
-  const R* A = RP + ((flags & 32) ? in.A0d : Lf.A);
+  const R* A = RP + ((flags & 32) ? in.A0d : tk.A);
   const R* x = RP + tk.xin;
-  const R* bp = (flags & 64) ? x : RP + ((flags & 16) ? tk.xout : Lf.b);
+  const R* bp = (flags & 64) ? x : RP + ((flags & 16) ? tk.xout : tk.b);
   // flag 64: the rhs is celltocorner of the T plane at tk.xout
   // (core/fortran_operators.f90:44-64, evaluated as k_c2c does)
   const T* cq = Q + tk.xout;
@@ -644,7 +641,8 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   const int fy0 = 2 * tk.y0 - 3, fx0 = 2 * tk.x0 - 3;
   R w[5];
   for (int k = 0; k < 5; k++) w[k] = R(in.w0[k]);
-  const uint8_t* sb = (const uint8_t*)(B + in.sb0);
+  const bool fly = tk.sb >= 0;
+  const uint8_t* sb = (const uint8_t*)(B + (fly ? tk.sb : 0));
   double nr = 0., nb = 0.;
   // x (and the stencil bits) of the tile's region in one round trip into
   // LDS (rows/columns clamped to the array); points off the periodic halo
@@ -655,7 +653,6 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   constexpr int ITX = (NX + NT - 1) / NT, ITP = (RY * RX + NT - 1) / NT;
   __shared__ R xsr[XRH][XRW];
   __shared__ uint8_t sbr[XRH][XRW];
-  const bool fly = tk.lev == 0 && in.a0fly;
   int full = 0;
   R xv[ITX];
   uint8_t bv[ITX];
@@ -690,7 +687,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     bq[it] = bat(c);
     mq[it] = msk1[c];
   }
-  if (skp) {
+  if (skip(dyn, tk.inst, pred)) {
     if ((flags & 12) && tid == 0) {
       partial[2 * ti] = 0.;
       partial[2 * ti + 1] = 0.;
@@ -776,7 +773,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     partial[2 * ti] = a;
     partial[2 * ti + 1] = c;
   }
-  T* b2 = Q + Lc.b;
+  T* b2 = Q + in.L[tk.lev + 1].b;
 #pragma unroll
   for (int it = 0; it < ITR; it++) {
     const int q = tid + it * NT;
@@ -2379,6 +2376,20 @@ static int build_programs(Batch& b) {
   ECHK(hipMalloc(&b.dT, std::max<size_t>(1, b.tasks.size()) * sizeof(Task)));
   ECHK(hipMalloc(&b.dS, std::max<size_t>(1, b.spans.size()) * sizeof(FoldSpan)));
   ECHK(hipMalloc(&b.dSlot, std::max<size_t>(1, b.slots.size()) * sizeof(int)));
+  for (Task& t : b.tasks) {
+    const InstD& d = b.H[t.inst].d;
+    const LevD& L = d.L[t.lev];
+    t.m = L.m;
+    t.n = L.n;
+    t.b = L.b;
+    t.A = L.A;
+    t.msk = L.msk;
+    const bool cl = t.lev + 1 < d.nlev;
+    t.m2 = cl ? d.L[t.lev + 1].m : 0;
+    t.n2 = cl ? d.L[t.lev + 1].n : 0;
+    t.msk2 = cl ? d.L[t.lev + 1].msk : 0;
+    t.sb = (t.lev == 0 && d.a0fly) ? d.sb0 : -1;
+  }
   ECHK(hipMemcpy(b.dT, b.tasks.data(), b.tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
   ECHK(hipMemcpy(b.dS, b.spans.data(), b.spans.size() * sizeof(FoldSpan), hipMemcpyHostToDevice));
   if (!b.slots.empty())
