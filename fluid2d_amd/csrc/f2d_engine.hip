@@ -847,7 +847,7 @@ __device__ __forceinline__ int qdiv(int q, int w) {
 // zero_in: the smoother's input x is zero (a level entered from above);
 // the zero is read, not stored, with the arithmetic of a stored zero
 template <typename T, typename TA>
-__device__ void c_smooth(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in = false) {
+__device__ __forceinline__ void c_smooth(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in = false) {
   const int m = L.m, n = L.n, cells = m * n;
   const size_t P = (size_t)m * n;
   const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2;
@@ -879,8 +879,8 @@ __device__ void c_smooth(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in 
   sy.end(cells);
 }
 template <typename T, typename TA>
-__device__ void c_resid_restrict(const CL<T, TA>& F, const CL<T, TA>& C, bool restrict_b,
-                                 Sync& sy) {
+__device__ __forceinline__ void c_resid_restrict(const CL<T, TA>& F, const CL<T, TA>& C,
+                                                 bool restrict_b, Sync& sy) {
   const int m = F.m, n = F.n, cells = m * n;
   const size_t P = (size_t)m * n;
   const int N = n - 2 * NH, M = m - 2 * NH;
@@ -913,7 +913,7 @@ __device__ void c_resid_restrict(const CL<T, TA>& F, const CL<T, TA>& C, bool re
   sy.end(ccells);
 }
 template <typename T, typename TA>
-__device__ void c_interp(const CL<T, TA>& F, const CL<T, TA>& C, bool acc, Sync& sy) {
+__device__ __forceinline__ void c_interp(const CL<T, TA>& F, const CL<T, TA>& C, bool acc, Sync& sy) {
   const int cnt = F.m * F.n;
   if (sy.begin(cnt))
     for (int q = sy.tid; q < cnt; q += sy.stride(cnt)) {
@@ -961,7 +961,7 @@ constexpr int OWNK = 1;
 // t with its images (the first half of c_resid_restrict).  Returns false,
 // doing nothing, when the level has more cells than the owners hold.
 template <typename T, typename TA>
-__device__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in, int iters,
+__device__ __forceinline__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in, int iters,
                              bool resid) {
   const int m = L.m, n = L.n, cells = m * n;
   const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2, w = wj * wi;
@@ -1055,7 +1055,7 @@ __device__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero
 // the second half of c_resid_restrict: full weighting of `src` (level F)
 // onto the coarse interior of C (+ images)
 template <typename T, typename TA>
-__device__ void c_restrict(const CL<T, TA>& F, const CL<T, TA>& C, LDSP T* src, Sync& sy) {
+__device__ __forceinline__ void c_restrict(const CL<T, TA>& F, const CL<T, TA>& C, LDSP T* src, Sync& sy) {
   const int n = F.n;
   const int N2 = C.n - 2 * NH, M2 = C.m - 2 * NH, ccells = C.m * C.n;
   if (sy.begin(ccells))
@@ -1074,14 +1074,99 @@ __device__ void c_restrict(const CL<T, TA>& F, const CL<T, TA>& C, LDSP T* src, 
 
 // a smoothing of level L (one smoothtwice), owned or generic
 template <typename T, typename TA>
-__device__ void c_smooth_level(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in) {
+__device__ __forceinline__ void c_smooth_level(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in) {
   if (!c_smooth_own(L, c1, c2, sy, zero_in, 1, false)) c_smooth(L, c1, c2, sy, zero_in);
+}
+
+// a double from another lane of the wavefront (two 32-bit permutes)
+__device__ __forceinline__ double lane_get(double v, int src_lane) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)u);
+  const int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// The coarsest-level solve on one wavefront with x and t in registers:
+// lane q holds cell q of the first sweep's region (interior + one ring) with
+// its operator row, c1 / |A_5|, rhs and mask; a sweep fetches its nine
+// neighbour values from other lanes.  x lives on the interior lanes (x at a
+// halo position is x at its periodic image, which is what the LDS copy of
+// the level holds there), t on all lanes of the region.  Same arithmetic as
+// c_smooth_own, sweep for sweep; x (interior + images) is written to LDS at
+// the end, t is not needed afterwards.  Levels whose region exceeds a
+// wavefront, or with an interior narrower than 2, return false.
+template <typename TA>
+__device__ __forceinline__ bool c_coarsest_wave(const CL<double, TA>& L, double c1, double c2, Sync& sy,
+                                int iters) {
+  typedef double T;
+  const int m = L.m, n = L.n, M = m - 2 * NH, N = n - 2 * NH;
+  const int wj = M + 2, wi = N + 2, w = wj * wi;
+  if (w > 64 || M < 2 || N < 2) return false;
+  sy.prev_small = true;  // a workgroup-wide phase after this one synchronises first
+  if (sy.tid >= 64) return true;
+  const int lane = sy.tid, P = m * n;
+  const bool own = lane < w;
+  const int qj = own ? qdiv(lane, wi) : 0;
+  const int j = NH - 1 + qj, i = NH - 1 + (own ? lane - qj * wi : 0);
+  const int c = j * n + i;
+  const bool inner = own && j >= NH && j < m - NH && i >= NH && i < n - NH;
+  ARow<T> a = {};
+  T c3 = T(0), bc = T(0);
+  int8_t mk = 0;
+  if (own) {
+    a.a0 = T(L.A[c]);
+    a.a1 = T(L.A[P + c]);
+    a.a2 = T(L.A[2 * P + c]);
+    a.a3 = T(L.A[3 * P + c]);
+    a.a4 = T(L.A[4 * P + c]);
+    a.a3e = T(L.A[3 * P + c + 1]);
+    a.a2s = T(L.A[2 * P + c + n - 1]);
+    a.a1s = T(L.A[P + c + n]);
+    a.a0s = T(L.A[c + n + 1]);
+    c3 = c1 / fabs(a.a4);
+    bc = L.b[c];
+    mk = L.msk[c];
+  }
+  // source lanes of the nine neighbours: for x, the image of the position;
+  // for t, the position itself (inner cells' neighbours lie in the region)
+  int sx[9], st[9];
+#pragma unroll
+  for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+    for (int di = -1; di <= 1; di++) {
+      const int k = (dj + 1) * 3 + di + 1;
+      sx[k] = own ? (img(j + dj, m) - (NH - 1)) * wi + img(i + di, n) - (NH - 1) : 0;
+      st[k] = inner ? (j + dj - (NH - 1)) * wi + i + di - (NH - 1) : 0;
+    }
+  T xv = T(0), tv = T(0);
+  for (int it = 0; it < iters; it++) {
+    T v;
+    if (it == 0) {
+      v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int, int) { return T(0); });
+    } else {
+      T xn[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) xn[k] = lane_get(xv, sx[k]);
+      v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return xn[(dj + 1) * 3 + di + 1]; });
+    }
+    tv = (own && mk != 0) ? v : T(0);
+    T tn[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) tn[k] = lane_get(tv, st[k]);
+    const T v2 = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return tn[(dj + 1) * 3 + di + 1]; });
+    xv = inner ? (mk == 0 ? T(0) : v2) : T(0);
+  }
+  if (inner) store_img_mask<T>(L.x, nullptr, j, i, xv, m, n);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return true;
 }
 
 // the coarsest-level solve: 16 smoothings of a level entered with x = 0
 // (hierarchy.py:117-118 / 144-145)
 template <typename T, typename TA>
-__device__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy) {
+__device__ __forceinline__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy) {
+  if (c_coarsest_wave(L, c1, c2, sy, 16)) return;
   if (!c_smooth_own(L, c1, c2, sy, true, 16, false))
     for (int k = 0; k < 16; k++) c_smooth(L, c1, c2, sy, k == 0);
 }
@@ -1089,8 +1174,8 @@ __device__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy) {
 // V-cycle from level lev1 down to level deep of the coarse part
 // (hierarchy.py:98-126)
 template <typename T, typename TA>
-__device__ void c_vcycle(const CL<T, TA>* Ls, int lev1, int deep, bool zero_first, T c1, T c2,
-                         Sync& sy) {
+__device__ __forceinline__ void c_vcycle(const CL<T, TA>* Ls, int lev1, int deep, bool zero_first,
+                                         T c1, T c2, Sync& sy) {
   for (int l = lev1; l < deep; l++) {
     const bool z = l > lev1 || zero_first;
     if (c_smooth_own(Ls[l], c1, c2, sy, z, 1, true)) {
