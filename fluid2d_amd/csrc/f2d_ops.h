@@ -65,6 +65,10 @@ void launch_sub_dev_scalar(T* x, const T* sc, size_t cnt, hipStream_t s);
 template <typename T>
 void launch_rk(int mode, T* out, const T* x, const T* k0, const T* k1, const T* k2, double c,
                size_t cnt, hipStream_t s);
+void launch_roll_mean(const double* u, double* out, int axis, int m, int n, hipStream_t s);
+void launch_negate(double* x, size_t cnt, hipStream_t s);
+void launch_flux_split(const double* a, const double* b, double* rev, double* irr, double sign,
+                       double cff, size_t cnt, hipStream_t s);
 
 void launch_coarsen(const double* Af, const int8_t* msk1, const int8_t* msk2, double* Ac, int m1,
                     int n1, int m2, int n2, hipStream_t s);

@@ -480,6 +480,35 @@ __global__ void k_sub_dev_scalar(T* __restrict__ x, const T* __restrict__ s, siz
   if (q < cnt) x[q] = x[q] - s[0];
 }
 
+// Flux diagnostics (core/fluxes.py:118-170).
+// out = 0.5*(u + np.roll(u, 1, axis)): the west (axis 1) or south (axis 0)
+// neighbour, periodic over the whole array
+template <typename T>
+__global__ void k_roll_mean(const T* __restrict__ u, T* __restrict__ out, int axis, int m, int n) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)m * n) return;
+  const int j = (int)(q / n), i = (int)(q - (size_t)j * n);
+  const size_t w = axis == 1 ? (size_t)j * n + (i == 0 ? n - 1 : i - 1)
+                             : (size_t)(j == 0 ? m - 1 : j - 1) * n + i;
+  out[q] = T(0.5) * (u[q] + u[w]);
+}
+// x *= -1
+template <typename T>
+__global__ void k_negate(T* __restrict__ x, size_t cnt) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < cnt) x[q] = -x[q];
+}
+// rev = cff*(a + sign*b), irr = cff*(a - sign*b), sign = +-1
+template <typename T>
+__global__ void k_flux_split(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ rev,
+                             T* __restrict__ irr, T sign, T cff, size_t cnt) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= cnt) return;
+  const T sb = sign * b[q];
+  rev[q] = cff * (a[q] + sb);
+  irr[q] = cff * (a[q] - sb);
+}
+
 // RK3_SSP / RK3 combinations (core/timescheme.py:152-180), NumPy order:
 // scalar factor, array sums left to right, scaled add.
 // mode 0: xs = x + c*k0             (RK3_SSP stage 1: c = dt; RK3: c = dt/3)
@@ -831,6 +860,17 @@ template <typename T>
 void launch_rk(int mode, T* out, const T* x, const T* k0, const T* k1, const T* k2, double c,
                size_t cnt, hipStream_t s) {
   hipLaunchKernelGGL(k_rk<T>, grid1(cnt), B1, 0, s, mode, out, x, k0, k1, k2, T(c), cnt);
+}
+void launch_roll_mean(const double* u, double* out, int axis, int m, int n, hipStream_t s) {
+  const size_t cnt = (size_t)m * n;
+  hipLaunchKernelGGL(k_roll_mean<double>, grid1(cnt), B1, 0, s, u, out, axis, m, n);
+}
+void launch_negate(double* x, size_t cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_negate<double>, grid1(cnt), B1, 0, s, x, cnt);
+}
+void launch_flux_split(const double* a, const double* b, double* rev, double* irr, double sign,
+                       double cff, size_t cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_split<double>, grid1(cnt), B1, 0, s, a, b, rev, irr, sign, cff, cnt);
 }
 
 void launch_coarsen(const double* Af, const int8_t* msk1, const int8_t* msk2, double* Ac, int m1,

@@ -347,6 +347,10 @@ struct f2d_model {
   double diags[8];
   int last_nite = 0;
   double last_res = 0.;
+  // flux diagnostics (core/fluxes.py): extended state, input copy, forward
+  // result, stage state, persistent tendencies; allocated on first use
+  double *fx_x = nullptr, *fx_xe = nullptr, *fx_xw = nullptr, *fx_xs = nullptr;
+  double* fx_k[3] = {nullptr, nullptr, nullptr};
   double* F(double* base, int k) const { return base + (size_t)k * fsz; }
 };
 
@@ -362,6 +366,11 @@ static void model_free(f2d_model* M) {
   dfree(M->yr);
   dfree(M->partial);
   dfree(M->dscal);
+  dfree(M->fx_x);
+  dfree(M->fx_xe);
+  dfree(M->fx_xw);
+  dfree(M->fx_xs);
+  for (auto& q : M->fx_k) dfree(q);
   if (M->hscal) (void)hipHostFree(M->hscal);
   gmg_free(M->gmg);
   delete M;
@@ -447,30 +456,134 @@ static int dynamics(f2d_model* M, const double* x, double* dxdt) {
   return invert_vorticity(M, dxdt, false);
 }
 
-// core/timescheme.py:167-180 (RK3_SSP), :152-164 (RK3)
-static int forward(f2d_model* M, double dt) {
-  const size_t cnt = M->ssz;
-  double *x = M->state, *xs = M->xs, *k0 = M->k[0], *k1 = M->k[1], *k2 = M->k[2];
+// core/timescheme.py:167-180 (RK3_SSP), :152-164 (RK3) over cnt values of
+// x with stage state xs, tendencies k[0..2] and right-hand side rhs
+using RhsFn = int (*)(f2d_model*, const double*, double*);
+static int rk_forward(f2d_model* M, double* x, double* xs, double* const* k, size_t cnt,
+                      double dt, RhsFn rhs) {
+  double *k0 = k[0], *k1 = k[1], *k2 = k[2];
   if (M->p.timestepping == 0) {
     M->kstage = 0;
-    TRY(dynamics(M, x, k0));
+    TRY(rhs(M, x, k0));
     launch_rk(0, xs, x, k0, k1, k2, dt, cnt, M->s);
     M->kstage = 1;
-    TRY(dynamics(M, xs, k1));
+    TRY(rhs(M, xs, k1));
     launch_rk(1, xs, x, k0, k1, k2, 0.25 * dt, cnt, M->s);
     M->kstage = 2;
-    TRY(dynamics(M, xs, k2));
+    TRY(rhs(M, xs, k2));
     launch_rk(2, x, x, k0, k1, k2, dt / 6., cnt, M->s);
   } else {
     M->kstage = 0;
-    TRY(dynamics(M, x, k0));
+    TRY(rhs(M, x, k0));
     launch_rk(0, xs, x, k0, k1, k2, dt / 3., cnt, M->s);
     M->kstage = 1;
-    TRY(dynamics(M, xs, k1));
+    TRY(rhs(M, xs, k1));
     launch_rk(3, xs, x, k0, k1, k2, 0.5 * dt, cnt, M->s);
     M->kstage = 2;
-    TRY(dynamics(M, xs, k2));
+    TRY(rhs(M, xs, k2));
     launch_rk(4, x, x, k0, k1, k2, dt, cnt, M->s);
+  }
+  LAUNCHCHK();
+  return F2D_OK;
+}
+
+static int forward(f2d_model* M, double dt) {
+  return rk_forward(M, M->state, M->xs, M->k, M->ssz, dt, dynamics);
+}
+
+// ---------------------------------------------------------------------------
+// Flux diagnostics, core/fluxes.py:20-213 (class Fluxes).  Extended state:
+// the model variables, the cell-centred velocities uc, vc, then an (x, y)
+// edge-flux plane pair per advected quantity: vorticity [buoyancy] uc vc.
+// ---------------------------------------------------------------------------
+static int fx_tracers(const f2d_model* M, int* tr) {
+  int k = 0;
+  tr[k++] = I_VORT;
+  if (M->p.model != 0) tr[k++] = I_BUOY;
+  tr[k++] = M->nvar;      // uc
+  tr[k++] = M->nvar + 1;  // vc
+  return k;
+}
+
+// Fluxes.advection + rhs_adv (core/fluxes.py:172-213): advection with the
+// edge-flux outputs (core/fortran_fluxes.f90), fillhalo of the three
+// results, the torque for Boussinesq, the fast inversion
+static int fx_advection(f2d_model* M, const double* x, double* dxdt) {
+  int tr[4];
+  const int nt = fx_tracers(M, tr), nvs = M->nvar + 2;
+  const double* u = x + (size_t)I_U * M->fsz;
+  const double* v = x + (size_t)I_V * M->fsz;
+  for (int t = 0; t < nt; t++) {
+    double* y = M->F(dxdt, tr[t]);
+    double* xf = M->F(dxdt, nvs + 2 * t);
+    double* yf = M->F(dxdt, nvs + 2 * t + 1);
+    const double* q = x + (size_t)tr[t] * M->fsz;
+    if (M->p.order % 2 == 0)
+      launch_adv_centered<double>(M->msk, q, y, u, v, xf, yf, M->cst, M->p.order, M->m, M->n,
+                                  M->s);
+    else
+      launch_adv_upwind<double>(M->msk, q, y, u, v, xf, yf, M->cst, M->p.flux_splitting,
+                                M->p.order, M->m, M->n, M->s);
+    launch_fillhalo(y, M->m, M->n, M->s);
+    launch_fillhalo(xf, M->m, M->n, M->s);
+    launch_fillhalo(yf, M->m, M->n, M->s);
+  }
+  if (M->p.model != 0) rhs_torque(M, x, dxdt);
+  LAUNCHCHK();
+  return invert_vorticity(M, dxdt, false);
+}
+
+// Fluxes.diag_fluxes (core/fluxes.py:100-170): one step dt forward from x,
+// one step -dt from the time-reversed x (u, v, psi, vorticity, uc, vc
+// negated); per flux plane, with a the forward and b the backward result
+// and cff = 0.5/dt: flx[k] = cff*(a + sign*b) ('rev'), flx[nflx + k] =
+// cff*(a - sign*b) ('irr'), sign = -1 for vorticity, uc, vc, +1 for
+// buoyancy.  The tendencies persist between calls (psi initial guesses).
+static int diag_fluxes(f2d_model* M, const double* x, double dt, double* flx) {
+  int tr[4];
+  const int nt = fx_tracers(M, tr), nvs = M->nvar + 2, nflx = 2 * nt, ne = nvs + nflx;
+  const size_t sz = M->fsz, cnt = (size_t)ne * sz;
+  if (!M->fx_x) {
+    int rc;
+    if ((rc = dalloc(&M->fx_x, cnt)) || (rc = dalloc(&M->fx_xe, (size_t)nvs * sz)) ||
+        (rc = dalloc(&M->fx_xw, cnt)) || (rc = dalloc(&M->fx_xs, cnt)) ||
+        (rc = dalloc(&M->fx_k[0], cnt)) || (rc = dalloc(&M->fx_k[1], cnt)) ||
+        (rc = dalloc(&M->fx_k[2], cnt)))
+      return rc;
+    for (double* q : {M->fx_x, M->fx_xw, M->fx_xs, M->fx_k[0], M->fx_k[1], M->fx_k[2]})
+      HIPCHK(hipMemsetAsync(q, 0, cnt * sizeof(double), M->s));
+    HIPCHK(hipMemsetAsync(M->fx_xe, 0, (size_t)nvs * sz * sizeof(double), M->s));
+  }
+  double *xe = M->fx_xe, *X = M->fx_x;
+  HIPCHK(hipMemcpyAsync(xe, x, (size_t)M->nvar * sz * sizeof(double), hipMemcpyDeviceToDevice,
+                        M->s));
+  double* uc = M->F(xe, M->nvar);
+  double* vc = M->F(xe, M->nvar + 1);
+  launch_roll_mean(x + (size_t)I_U * sz, uc, 1, M->m, M->n, M->s);
+  launch_fillhalo(uc, M->m, M->n, M->s);
+  launch_roll_mean(x + (size_t)I_V * sz, vc, 0, M->m, M->n, M->s);
+  launch_fillhalo(vc, M->m, M->n, M->s);
+  LAUNCHCHK();
+  auto load = [&](bool reverse) -> int {
+    HIPCHK(hipMemcpyAsync(X, xe, (size_t)nvs * sz * sizeof(double), hipMemcpyDeviceToDevice,
+                          M->s));
+    if (reverse)
+      for (int a : {(int)I_U, (int)I_V, (int)I_PSI, (int)I_VORT, M->nvar, M->nvar + 1})
+        launch_negate(M->F(X, a), sz, M->s);
+    launch_fill_value(M->F(X, nvs), 0., (size_t)nflx * sz, M->s);
+    LAUNCHCHK();
+    return F2D_OK;
+  };
+  TRY(load(false));
+  TRY(rk_forward(M, X, M->fx_xs, M->fx_k, cnt, dt, fx_advection));
+  HIPCHK(hipMemcpyAsync(M->fx_xw, X, cnt * sizeof(double), hipMemcpyDeviceToDevice, M->s));
+  TRY(load(true));
+  TRY(rk_forward(M, X, M->fx_xs, M->fx_k, cnt, -dt, fx_advection));
+  const double cff = 0.5 / dt;
+  for (int k = 0; k < nflx; k++) {
+    const double sign = (k < 2 || k >= nflx - 4) ? -1. : 1.;
+    launch_flux_split(M->F(M->fx_xw, nvs + k), M->F(X, nvs + k), flx + (size_t)k * sz,
+                      flx + (size_t)(nflx + k) * sz, sign, cff, sz, M->s);
   }
   LAUNCHCHK();
   return F2D_OK;
@@ -871,6 +984,13 @@ int f2d_model_diagnostics(f2d_model* M, const double* xr_host, const double* yr_
   if (xr_host) HIPCHK(hipMemcpy(M->xr, xr_host, M->fsz * 8, hipMemcpyHostToDevice));
   if (yr_host) HIPCHK(hipMemcpy(M->yr, yr_host, M->fsz * 8, hipMemcpyHostToDevice));
   return model_diag(M, out);
+}
+int f2d_model_diag_fluxes(f2d_model* M, const double* x_dev, double dt, double* flx_dev) {
+  if (!M || !x_dev || !flx_dev) return fail(F2D_E_ARG, "f2d_model_diag_fluxes: null argument");
+  if (!(dt != 0.)) return fail(F2D_E_ARG, "f2d_model_diag_fluxes: dt must be nonzero");
+  TRY(diag_fluxes(M, x_dev, dt, flx_dev));
+  HIPCHK(hipStreamSynchronize(M->s));
+  return F2D_OK;
 }
 int f2d_model_run(f2d_model* M, int nsteps, double dt, int* nite_log) {
   double out[8];

@@ -1,5 +1,6 @@
 """Fluid2d driver on the device: core/fluid2d.py ``Fluid2d`` (model factory,
-``loop``, ``set_dt``) without output, plotting or logging."""
+``loop``, ``set_dt``, ``enforce_zero_momentum``) and the output schedule of
+core/output.py, without files, plotting or logging."""
 import time
 
 import numpy as np
@@ -7,25 +8,74 @@ import numpy as np
 from .models import Boussinesq, Euler
 
 
+class Output:
+    """The history/diagnostic schedule of core/output.py:56-96 (``Output.do``).
+
+    A diagnostic record is taken whenever ``t >= tnextdiag`` (then
+    ``tnextdiag += freq_diag``) and a history record whenever
+    ``t >= tnexthis`` (then ``tnexthis += freq_his``); both clocks start at 0.
+    Records stay in host memory: ``diag`` holds ``(t, kt, diags)`` and
+    ``his`` holds ``(t, kt, state)`` (a host copy of the packed state;
+    ``keep_his = False`` keeps only the times); with flux diagnostics on,
+    ``flx`` holds ``(t, kt, fluxes)`` at the history times.  The reference
+    writes them to NetCDF4 files, which this package does not do.
+    """
+
+    def __init__(self, param, keep_his=True):
+        self.freq_his = float(param.freq_his)
+        self.freq_diag = float(param.freq_diag)
+        self.tnexthis = 0.0
+        self.tnextdiag = 0.0
+        self.keep_his = keep_his
+        self.his = []
+        self.diag = []
+        self.flx = []
+
+    def do(self, data, t, kt):
+        """``data``: {'diag': dict, 'his': callable returning the state
+        [, 'flx': callable returning the fluxes]}."""
+        if t >= self.tnextdiag:
+            self.tnextdiag += self.freq_diag
+            self.diag.append((t, kt, dict(data["diag"])))
+        if t >= self.tnexthis:
+            self.tnexthis += self.freq_his
+            self.his.append((t, kt, np.array(data["his"]()) if self.keep_his else None))
+            if "flx" in data:
+                self.flx.append((t, kt, data["flx"]()))
+
+
 class Fluid2d:
     def __init__(self, param, grid):
         param.checkall()
         self.param = param
         for k in ("modelname", "tend", "adaptable_dt", "dt", "cfl", "dtmax", "geometry",
-                  "nx", "ny", "npx", "npy", "decay", "nprint"):
+                  "nx", "ny", "npx", "npy", "decay", "nprint", "exacthistime", "rescaledtime",
+                  "enforce_momentum", "diag_fluxes"):
             setattr(self, k, getattr(param, k))
         self.dt0 = self.dt
         self.dx, self.dy = grid.dx, grid.dy
+        self.grid = grid
+        # core/fluid2d.py:67-77: momentum is enforced for Euler in closed
+        # domains only
         if param.modelname == "euler":
+            if self.geometry not in ("closed", "disc"):
+                self.enforce_momentum = False
             self.model = Euler(param, grid)
         elif param.modelname == "boussinesq":
+            self.enforce_momentum = False
             self.model = Boussinesq(param, grid)
         else:
             raise ValueError("model %s is outside the device path" % param.modelname)
+        self.enstrophyname = "enstrophy"  # core/fluid2d.py:102-108
+        if self.diag_fluxes:  # core/fluid2d.py:116-118
+            from .fluxes import Fluxes
+            self.flx = Fluxes(param, grid, self.model.ope)
+        self.output = Output(param)
         self.t = 0.0
         self.kt = 0
         self.stop = False
         self.walltime = 0.0
+        self.ke_increases = 0
 
     def set_dt(self, kt=0):
         """core/fluid2d.py:376-397"""
@@ -38,21 +88,89 @@ class Fluid2d:
             self.dt = self.dt0
         self.model.ope.cst[3] = ms
 
+    def _exact_his_dt(self):
+        """core/fluid2d.py:239-255: with exacthistime and an adaptive dt, the
+        step is shortened over the 8 steps before the next history time so
+        that one step lands on it."""
+        tnext = self.output.tnexthis
+        if self.t + 8 * self.dt > tnext and self._reduce == 0:
+            self._reduce = 8
+        if self._reduce > 0:
+            self.dt = (tnext - self.t) / (self._reduce * 0.95)
+            self._reduce -= 1
+        if self.t + self.dt > tnext:
+            self._reduce = 0
+            self.dt = tnext - self.t
+
+    def _state(self):
+        # the device state, read without marking the host mirror as written
+        self.model.var.sync_to_device()
+        return self.model.var.dev.get()
+
+    def _data(self):
+        d = {"diag": self.model.diags, "his": self._state}
+        if self.diag_fluxes:
+            d["flx"] = self.flx.flx.get
+        return d
+
     def loop(self, nsteps=None):
-        """core/fluid2d.py:188-349: until t >= tend (or ``nsteps`` steps),
-        set_dt, step, t += dt, diagnostics, blow-up check."""
+        """core/fluid2d.py:188-349: until t >= tend (or ``nsteps`` steps):
+        set_dt (shortened to hit history times), step, advance t (by dt, or
+        by dt * sqrt(enstrophy) with rescaledtime = 'enstrophy'),
+        diagnostics, momentum enforcement, dke/dt and denstrophy/dt, the
+        flux diagnostics before each history record, the output schedule,
+        blow-up check."""
         self.model.diagnostics(self.model.var, self.t)
+        self.model.diags["dkedt"] = 0.0
+        self.model.diags["dvdt"] = 0.0
+        if self.diag_fluxes:  # core/fluid2d.py:203-210: dt first
+            self.set_dt(self.kt)
+            self.flx.diag_fluxes(self.model.var.dev, self.t, self.dt)
+        self.output.do(self._data(), self.t, self.kt)
+        self._reduce = 0
         t0 = time.perf_counter()
         k = 0
         while (self.t < self.tend if nsteps is None else k < nsteps) and not self.stop:
             self.set_dt(self.kt)
+            if self.exacthistime and self.adaptable_dt:
+                self._exact_his_dt()
             self.model.step(self.t, self.dt)
-            self.t += self.dt
+            if self.rescaledtime == "enstrophy":
+                self.t += self.dt * np.sqrt(self.model.diags["enstrophy"])
+            else:
+                self.t += self.dt
             self.kt += 1
             k += 1
+            ke_old = self.model.diags["ke"]
+            ens_old = self.model.diags[self.enstrophyname]
             self.model.diagnostics(self.model.var, self.t)
+            if self.enforce_momentum:
+                self.enforce_zero_momentum()
+                self.model.diagnostics(self.model.var, self.t)
+            ke = self.model.diags["ke"]
+            ens = self.model.diags[self.enstrophyname]
+            self.model.diags["dkedt"] = (ke - ke_old) / self.dt
+            self.model.diags["dvdt"] = (ens - ens_old) / self.dt
+            if ke > ke_old and self.decay and self.modelname == "euler":
+                self.ke_increases += 1
+            if self.diag_fluxes and self.t >= self.output.tnexthis:
+                # core/fluid2d.py:293-295: only before a history record
+                self.flx.diag_fluxes(self.model.var.dev, self.t, self.dt)
+            self.output.do(self._data(), self.t, self.kt)
             if self.model.diags["maxspeed"] > 1e3:
                 self.stop = True
                 print("max|u| > 1000, blow-up detected, stopping")
         self.walltime = time.perf_counter() - t0
         return k
+
+    def enforce_zero_momentum(self):
+        """core/fluid2d.py:351-358: remove the linear vorticity profile that
+        carries the domain's momentum, omega -= xr0 px / x2 + yr0 py / y2,
+        then the fast inversion of the state."""
+        if self.enforce_momentum:
+            g = self.grid
+            px = self.model.diags["px"] / g.x2
+            py = self.model.diags["py"] / g.y2
+            vor = self.model.var.get("vorticity")
+            vor[:] -= g.xr0 * px + g.yr0 * py
+            self.model.ope.invert_vorticity(self.model.var.dev, flag="fast")

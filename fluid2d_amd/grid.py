@@ -42,7 +42,7 @@ class Grid:
         self.finalize_msk()
 
     def finalize_msk(self):
-        """core/grid.py:104-130 (area and centred coordinates)."""
+        """core/grid.py:104-130 (area, centred coordinates, second moments)."""
         msk = self.msk
         self.area = self.domain_integration(msk)
         x0 = self.domain_integration(self.xr * msk) / self.area
@@ -50,6 +50,9 @@ class Grid:
         self.xr0 = (self.xr - x0) * msk
         self.yr0 = (self.yr - y0) * msk
         self.x0, self.y0 = x0, y0
+        self.x2 = self.domain_integration(self.xr0 ** 2 * msk) / self.area
+        self.y2 = self.domain_integration(self.yr0 ** 2 * msk) / self.area
+        self.r2 = self.xr0 ** 2 + self.yr0 ** 2
 
     def domain_integration(self, z2d):
         nh = self.nh

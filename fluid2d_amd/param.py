@@ -19,6 +19,8 @@ DEFAULTS = {
     "nh": 3,
     # time
     "adaptable_dt": True,
+    "exacthistime": True,
+    "rescaledtime": "none",
     "dt": 0.1,
     "cfl": 0.5,
     "dtmax": 5.0,
@@ -42,7 +44,10 @@ DEFAULTS = {
     "isisland": False,
     "enforce_momentum": False,
     "decay": True,
-    # output (accepted, unused on the device path)
+    # output schedule (core/defaults.json:255-273); files are not written
+    "freq_his": 1.0,
+    "diag_fluxes": False,
+    "freq_diag": 1.0,
     "nprint": 20,
     "plot_interactive": False,
 }
@@ -55,9 +60,10 @@ AVAIL = {
     "flux_splitting_method": ["minmax", "parabolic"],
     "geometry": ["disc", "perio", "closed", "ychannel", "xchannel", "xperio", "yperio"],
     "relaxation": ["default"],
+    "rescaledtime": ["none", "enstrophy"],
 }
 
-UNSUPPORTED_TRUE = ["noslip", "forcing", "isisland", "enforce_momentum"]
+UNSUPPORTED_TRUE = ["noslip", "forcing", "isisland"]
 
 
 class Param:

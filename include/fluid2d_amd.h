@@ -202,6 +202,14 @@ int f2d_model_diagnostics(f2d_model *M, const double *xr_host, const double *yr_
 /* Fluid2d.loop body with fixed dt (core/fluid2d.py:235-268): per step
  * cst[3] = maxspeed, step, diagnostics.  nite_log (host, nsteps ints) or NULL. */
 int f2d_model_run(f2d_model *M, int nsteps, double dt, int *nite_log);
+/* Fluxes.diag_fluxes (core/fluxes.py:100-170) of the model-shaped device
+ * state x: reversible and irreversible advective edge fluxes over one step
+ * dt (one step forward, one step -dt from the time-reversed state), into
+ * flx_dev, 4*ntr planes of m*n (ntr = 3 Euler: vorticity uc vc; 4
+ * Boussinesq: vorticity buoyancy uc vc) in the order of fullflx_list
+ * (core/fluxes.py:45-50): rev then irr; per quantity x then y.  Uses the
+ * model's umax (f2d_model_set_umax).  dt != 0. */
+int f2d_model_diag_fluxes(f2d_model *M, const double *x_dev, double dt, double *flx_dev);
 /* F-cycles and final relative residual of the last full solve */
 int f2d_model_last_solve(f2d_model *M, int *nite, double *res);
 

@@ -28,6 +28,8 @@
  *                   :292-314 (rhs_adv, rhs_diffusion, rhs_torque)
  *   time scheme     core/timescheme.py:152-180 (RK3, RK3_SSP)
  *   models          core/euler.py:103-169, :185-221; core/boussinesq.py:87-153
+ *   flux split      core/fluxes.py:100-213 (Fluxes.diag_fluxes, advection,
+ *                   rhs_adv), pinned by tests/test_oracle_fluxes.py
  *   NumPy sum       core/grid.py:132-140 (np.sum over the interior slice):
  *                   NumPy 2.x reduces a strided slice in 8192-element
  *                   buffered chunks, each summed pairwise (blocks of 128
@@ -871,6 +873,9 @@ typedef struct {
   int last_nite, last_status;
   double last_res;
   double diags[8]; /* maxspeed, ke, z, z2, px|pe, py|b, angmom|b2, source */
+  /* Fluxes (core/fluxes.py): extended state, its copy, the forward result,
+   * the stage state and the three persistent tendencies */
+  double *fx_x, *fx_xe, *fx_xw, *fx_xs, *fx_k[3];
 } orc_model;
 
 /* field indices */
@@ -961,33 +966,145 @@ static int dynamics(orc_model *M, double *x, double *dxdt) {
   return invert_vorticity(M, dxdt, 0);
 }
 
-/* core/timescheme.py:167-180 (RK3_SSP) and :152-164 (RK3).  The NumPy
- * expressions are restated with their evaluation order: scalar factor
- * first, array sums left to right, then the scaled add. */
-static int forward(orc_model *M, double dt) {
-  const size_t tot = (size_t)M->nvar * M->m * M->n;
-  double *x = M->state, *xs = M->xs, *k0 = M->dx0, *k1 = M->dx1, *k2 = M->dx2;
+/* core/timescheme.py:167-180 (RK3_SSP) and :152-164 (RK3) on `tot`
+ * values of x, with stage state xs, tendencies k[0..2] and right-hand side
+ * rhs.  The NumPy expressions are restated with their evaluation order:
+ * scalar factor first, array sums left to right, then the scaled add. */
+typedef int (*orc_rhs)(orc_model *M, double *x, double *dxdt);
+static int rk_forward(orc_model *M, double *x, double *xs, double **k, size_t tot, double dt,
+                      orc_rhs rhs) {
+  double *k0 = k[0], *k1 = k[1], *k2 = k[2];
   int st = 0;
   if (M->tscheme == TS_RK3_SSP) {
-    M->kstage = 0; st |= dynamics(M, x, k0);
+    M->kstage = 0; st |= rhs(M, x, k0);
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + dt * k0[q];
-    M->kstage = 1; st |= dynamics(M, xs, k1);
+    M->kstage = 1; st |= rhs(M, xs, k1);
     const double c = 0.25 * dt;
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + c * (k0[q] + k1[q]);
-    M->kstage = 2; st |= dynamics(M, xs, k2);
+    M->kstage = 2; st |= rhs(M, xs, k2);
     const double c6 = dt / 6.;
     for (size_t q = 0; q < tot; q++) x[q] = x[q] + c6 * ((k0[q] + k1[q]) + 4. * k2[q]);
   } else {
-    M->kstage = 0; st |= dynamics(M, x, k0);
+    M->kstage = 0; st |= rhs(M, x, k0);
     const double c3 = dt / 3.;
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + c3 * k0[q];
-    M->kstage = 1; st |= dynamics(M, xs, k1);
+    M->kstage = 1; st |= rhs(M, xs, k1);
     const double c2 = 0.5 * dt;
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + c2 * k1[q];
-    M->kstage = 2; st |= dynamics(M, xs, k2);
+    M->kstage = 2; st |= rhs(M, xs, k2);
     for (size_t q = 0; q < tot; q++) x[q] = x[q] + dt * k2[q];
   }
   return st;
+}
+
+static int forward(orc_model *M, double dt) {
+  double *k[3] = {M->dx0, M->dx1, M->dx2};
+  return rk_forward(M, M->state, M->xs, k, (size_t)M->nvar * M->m * M->n, dt, dynamics);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Flux diagnostics, core/fluxes.py:20-213 (class Fluxes).  The extended     */
+/* state holds the model variables, the cell-centred velocities uc, vc and  */
+/* two edge-flux planes (x, y) per advected quantity: vorticity [buoyancy]  */
+/* uc vc.                                                                    */
+/* ------------------------------------------------------------------------ */
+static int fx_tracers(orc_model *M, int *tr) {
+  int k = 0;
+  tr[k++] = I_VORT;
+  if (M->kind != MODEL_EULER) tr[k++] = I_BUOY;
+  tr[k++] = M->nvar;     /* uc */
+  tr[k++] = M->nvar + 1; /* vc */
+  return k;
+}
+
+/* Fluxes.advection + rhs_adv (core/fluxes.py:172-213): fortran_fluxes
+ * advection with the edge-flux outputs, fillhalo of all three, the torque
+ * for Boussinesq, the fast inversion. */
+static int fx_advection(orc_model *M, double *x, double *dxdt) {
+  int tr[4];
+  const int nt = fx_tracers(M, tr), nvs = M->nvar + 2;
+  for (int t = 0; t < nt; t++) {
+    double *y = FLD(M, dxdt, tr[t]), *xf = FLD(M, dxdt, nvs + 2 * t);
+    double *yf = FLD(M, dxdt, nvs + 2 * t + 1);
+    if (M->order % 2 == 0)
+      orc_adv_centered(M->msk, FLD(M, x, tr[t]), y, FLD(M, x, I_U), FLD(M, x, I_V), xf, yf,
+                       M->cst, M->nh, M->order, M->m, M->n);
+    else
+      orc_adv_upwind(M->msk, FLD(M, x, tr[t]), y, FLD(M, x, I_U), FLD(M, x, I_V), xf, yf,
+                     M->cst, M->nh, M->method, M->order, M->m, M->n);
+    orc_fillhalo(y, M->nh, M->m, M->n);
+    orc_fillhalo(xf, M->nh, M->m, M->n);
+    orc_fillhalo(yf, M->nh, M->m, M->n);
+  }
+  if (M->kind != MODEL_EULER) rhs_torque(M, x, dxdt);
+  return invert_vorticity(M, dxdt, 0);
+}
+
+/* Fluxes.diag_fluxes (core/fluxes.py:100-170): one step dt forward from
+ * x, one step -dt from the time-reversed x (u, v, psi, vorticity, uc, vc
+ * negated), then per flux plane l, with xw the forward and xb the backward
+ * result and cff = 0.5/dt:
+ *   flx[k]        = cff*(xw_l + sign*xb_l)   (fullflx_list 'rev_...')
+ *   flx[nflx + k] = cff*(xw_l - sign*xb_l)   (fullflx_list 'irr_...')
+ * sign = -1 for the vorticity, uc and vc fluxes, +1 for buoyancy.  The
+ * tendencies persist between calls (the psi initial guesses). */
+int orc_model_diag_fluxes(orc_model *M, const double *x, double dt, double *flx) {
+  int tr[4];
+  const int nt = fx_tracers(M, tr), nvs = M->nvar + 2, nflx = 2 * nt, ne = nvs + nflx;
+  const int m = M->m, n = M->n;
+  const size_t sz = (size_t)m * n, tot = (size_t)ne * sz;
+  if (!M->fx_x) {
+    M->fx_x = (double *)calloc(tot, sizeof(double));
+    M->fx_xe = (double *)calloc((size_t)nvs * sz, sizeof(double));
+    M->fx_xw = (double *)calloc(tot, sizeof(double));
+    M->fx_xs = (double *)calloc(tot, sizeof(double));
+    for (int k = 0; k < 3; k++) M->fx_k[k] = (double *)calloc(tot, sizeof(double));
+    if (!M->fx_x || !M->fx_xe || !M->fx_xw || !M->fx_xs || !M->fx_k[0] || !M->fx_k[1] ||
+        !M->fx_k[2])
+      return 2;
+  }
+  double *xe = M->fx_xe, *X = M->fx_x;
+  memcpy(xe, x, (size_t)M->nvar * sz * sizeof(double));
+  /* uc = 0.5*(u + np.roll(u, 1, axis=1)), vc = 0.5*(v + np.roll(v, 1,
+   * axis=0)), each followed by fill_halo */
+  const double *u = x + (size_t)I_U * sz, *v = x + (size_t)I_V * sz;
+  double *uc = xe + (size_t)M->nvar * sz, *vc = uc + sz;
+  for (int j = 0; j < m; j++)
+    for (int i = 0; i < n; i++) {
+      uc[(size_t)j * n + i] = 0.5 * (u[(size_t)j * n + i] + u[(size_t)j * n + (i + n - 1) % n]);
+      vc[(size_t)j * n + i] = 0.5 * (v[(size_t)j * n + i] + v[(size_t)((j + m - 1) % m) * n + i]);
+    }
+  orc_fillhalo(uc, M->nh, m, n);
+  orc_fillhalo(vc, M->nh, m, n);
+  int st = 0;
+  memcpy(X, xe, (size_t)nvs * sz * sizeof(double));
+  memset(X + (size_t)nvs * sz, 0, (size_t)nflx * sz * sizeof(double));
+  st |= rk_forward(M, X, M->fx_xs, M->fx_k, tot, dt, fx_advection);
+  memcpy(M->fx_xw, X, tot * sizeof(double));
+  memcpy(X, xe, (size_t)nvs * sz * sizeof(double));
+  const int neg[6] = {I_U, I_V, I_PSI, I_VORT, M->nvar, M->nvar + 1};
+  for (int a = 0; a < 6; a++)
+    for (size_t q = 0; q < sz; q++) X[(size_t)neg[a] * sz + q] = -X[(size_t)neg[a] * sz + q];
+  memset(X + (size_t)nvs * sz, 0, (size_t)nflx * sz * sizeof(double));
+  st |= rk_forward(M, X, M->fx_xs, M->fx_k, tot, -dt, fx_advection);
+  const double cff = 0.5 / dt;
+  for (int k = 0; k < nflx; k++) {
+    const double sign = (k < 2 || k >= nflx - 4) ? -1. : 1.;
+    const double *a = M->fx_xw + (size_t)(nvs + k) * sz, *b = X + (size_t)(nvs + k) * sz;
+    double *rev = flx + (size_t)k * sz, *irr = flx + (size_t)(nflx + k) * sz;
+    for (size_t q = 0; q < sz; q++) {
+      rev[q] = cff * (a[q] + sign * b[q]);
+      irr[q] = cff * (a[q] - sign * b[q]);
+    }
+  }
+  return st;
+}
+
+/* The extended states of the last diag_fluxes call, for tests: which = 0
+ * the input (model variables, uc, vc), 1 the forward result, 2 the
+ * backward result; NULL before the first call. */
+const double *orc_model_fx_state(orc_model *M, int which) {
+  return which == 0 ? M->fx_xe : (which == 1 ? M->fx_xw : M->fx_x);
 }
 
 /* Model set-up.  msk: cell mask (m x n int8).  kdiff[0] for vorticity,
@@ -1042,6 +1159,8 @@ void orc_model_destroy(orc_model *M) {
   orc_gmg_destroy(M->gmg);
   free(M->msk); free(M->mskp); free(M->state); free(M->dx0); free(M->dx1);
   free(M->dx2); free(M->xs); free(M->work);
+  free(M->fx_x); free(M->fx_xe); free(M->fx_xw); free(M->fx_xs);
+  for (int k = 0; k < 3; k++) free(M->fx_k[k]);
   free(M);
 }
 

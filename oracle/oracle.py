@@ -110,6 +110,8 @@ def lib():
         "orc_model_step": (_i, [_vp, _d]),
         "orc_model_diagnostics": (None, [_vp, _dp, _dp, _dp]),
         "orc_model_run": (_i, [_vp, _i, _d, _dp, _dp, _vp]),
+        "orc_model_diag_fluxes": (_i, [_vp, _dp, _d, _dp]),
+        "orc_model_fx_state": (_vp, [_vp, _i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -425,6 +427,33 @@ class OracleModel:
         if st:
             raise RuntimeError("solver is not converging")
         return self.diagnostics()
+
+    def flux_names(self):
+        """fullflx_list of core/fluxes.py:45-50."""
+        tr = ["vorticity"] + (["buoyancy"] if self.modelname == "boussinesq" else []) + ["uc", "vc"]
+        return ["%s_%s_%s" % (r, d, v) for r in ("rev", "irr") for v in tr for d in ("x", "y")]
+
+    def diag_fluxes(self, dt):
+        """Fluxes.diag_fluxes (core/fluxes.py:100-170) of the current state:
+        the reversible and irreversible edge fluxes, (len(flux_names()), m, n)."""
+        flx = np.zeros((len(self.flux_names()), self.m, self.n))
+        x = np.ascontiguousarray(self.state.copy())
+        if lib().orc_model_diag_fluxes(self._h, x, dt, flx):
+            raise RuntimeError("diag_fluxes failed")
+        return flx
+
+    def fx_states(self):
+        """Extended states of the last diag_fluxes call: (input, forward
+        result, backward result); the input holds the model variables and
+        uc, vc only."""
+        nt = len(self.flux_names()) // 4
+        nvs, ne = self.nvar + 2, self.nvar + 2 + 2 * nt
+        out = []
+        for which, planes in ((0, nvs), (1, ne), (2, ne)):
+            p = lib().orc_model_fx_state(self._h, which)
+            buf = (C.c_byte * (planes * self.m * self.n * 8)).from_address(p)
+            out.append(np.frombuffer(buf, dtype=np.float64).reshape(planes, self.m, self.n).copy())
+        return out
 
     def run(self, nsteps, dt):
         """Fluid2d.loop body with fixed dt; returns the F-cycle count per step."""

@@ -33,7 +33,7 @@ def setup_pair(kind, nx, ny, Lx, Ly, geom, seed, **kw):
     f = F.Fluid2d(p, g)
     o = orc.OracleModel(kind, nx, ny, Lx, Ly, geometry=geom,
                         diffusion=kw.get("diffusion", False), Kdiff=kw.get("Kdiff", 0.0),
-                        gravity=kw.get("gravity", 1.0))
+                        gravity=kw.get("gravity", 1.0), order=kw.get("order", 5))
     msk = g.msk
     w = workloads.turbulence(g, seed)
     f.model.var.get("vorticity")[...] = w
@@ -149,3 +149,115 @@ def test_fluid2d_loop_drop_in():
     assert n > 0 and abs(f.t - 0.05) < f.dt + 1e-12
     ke = f.model.diags["ke"]
     assert np.isfinite(ke) and ke > 0
+
+
+def test_fluid2d_loop_exacthistime_matches_oracle():
+    """Fluid2d.loop with adaptive dt shortened onto the history times
+    (core/fluid2d.py:239-255): the history records land on multiples of
+    freq_his, and the state equals the oracle stepped with the same dt
+    sequence (each step's umax is the oracle's own maxspeed)."""
+    p = F.Param(modelname="euler", nx=64, ny=64, geometry="closed", adaptable_dt=True, cfl=0.8,
+                tend=0.03, freq_his=0.01, exacthistime=True)
+    g = F.Grid(p)
+    f = F.Fluid2d(p, g)
+    o = orc.OracleModel("euler", 64, 64, geometry="closed")
+    w = workloads.turbulence(g, 5)
+    f.model.var.get("vorticity")[...] = w
+    o.get("vorticity")[...] = w
+    f.model.set_psi_from_vorticity()
+    o.set_psi_from_vorticity()
+    dts = []
+    step = f.model.step
+    f.model.step = lambda t, dt: (dts.append(dt), step(t, dt))
+    f.loop()
+    his_t = [t for t, _, _ in f.output.his]
+    assert np.allclose(his_t, [0.0, 0.01, 0.02, 0.03], rtol=0, atol=1e-14)
+    o.diagnostics()
+    states = {}
+    for k, dt in enumerate(dts, 1):
+        o.step(dt)
+        states[k] = o.state.copy()
+    compare_states(f, o, "loop end")
+    for t, kt, s in f.output.his[1:]:
+        for v, name in enumerate(f.model.var.varname_list):
+            r = rel_l2(s[v], states[kt][v])
+            assert r <= TOL, "history t=%g %s rel-L2 %.3e" % (t, name, r)
+
+
+def test_fluid2d_loop_enforce_momentum():
+    """enforce_momentum (core/fluid2d.py:270-272, 351-358): after each step
+    the vorticity's linear profile is removed and psi, u, v re-inverted, so
+    the momentum diagnostics px, py stay at rounding level."""
+    p = F.Param(modelname="euler", nx=64, ny=64, geometry="closed", adaptable_dt=True, cfl=0.8,
+                tend=1e9, enforce_momentum=True)
+    g = F.Grid(p)
+    f = F.Fluid2d(p, g)
+    w = workloads.turbulence(g, 6)
+    f.model.var.get("vorticity")[...] = w
+    f.model.set_psi_from_vorticity()
+    d0 = dict(f.model.diagnostics(f.model.var, 0))
+    p0 = max(abs(d0["px"]), abs(d0["py"]))
+    assert p0 > 1e-6
+    f.loop(nsteps=4)
+    d = f.model.diags
+    assert abs(d["px"]) < 1e-10 * p0 and abs(d["py"]) < 1e-10 * p0
+    s = f.model.var.state
+    assert np.isfinite(s).all()
+    # (u, v) is the orthogonal gradient of the stored psi (operators.py:487-497)
+    u, v = np.zeros_like(s[2]), np.zeros_like(s[3])
+    orc.orthogradient(f.model.msk, np.ascontiguousarray(s[1]), g.dx, g.dy, u, v)
+    assert (u == s[2]).all() and (v == s[3]).all()
+    # and psi was re-inverted from the corrected vorticity: a fresh full
+    # solve moves it by no more than two V-cycles' error
+    psi = s[1].copy()
+    f.model.set_psi_from_vorticity()
+    assert rel_l2(f.model.var.get("psi"), psi) < 1e-3
+
+
+@pytest.mark.parametrize("kind,geom,order", [("euler", "closed", 5), ("euler", "perio", 4),
+                                             ("boussinesq", "xchannel", 5)])
+def test_diag_fluxes_matches_oracle(kind, geom, order):
+    """Fluxes.diag_fluxes (core/fluxes.py:100-170) on the GPU against the
+    oracle, plane by plane, on two calls in a row (the second starts its
+    fast inversions from the tendencies the first left)."""
+    from fluid2d_amd.fluxes import Fluxes
+
+    nx, ny, Lx = (128, 64, 2.0) if kind == "boussinesq" else (64, 64, 1.0)
+    f, o = setup_pair(kind, nx, ny, Lx, 1.0, geom, 8, order=order)
+    d0 = f.model.diagnostics(f.model.var, 0)
+    o.diagnostics()
+    dt = 0.5 * f.model.grid.dx / max(d0["maxspeed"], 1.0)
+    f.model.run(2, dt)
+    o.run(2, dt)
+    compare_states(f, o, "before the flux diagnostics")
+    umax = f.model.diags["maxspeed"]
+    assert umax == o.diags["maxspeed"]
+    f.model.ope.cst[3] = umax
+    orc.lib().orc_model_set_umax(o._h, umax)
+    fl = Fluxes(f.param, f.model.grid, f.model.ope)
+    assert fl.fullflx_list == o.flux_names()
+    for call in range(2):
+        fl.diag_fluxes(f.model.var.dev, 0.0, dt)
+        got = fl.flx.get()
+        ref = o.diag_fluxes(dt)
+        for k, name in enumerate(fl.fullflx_list):
+            r = rel_l2(got[k], ref[k])
+            assert r <= TOL, "call %d %s rel-L2 %.3e" % (call, name, r)
+    compare_states(f, o, "after the flux diagnostics")
+
+
+def test_fluid2d_loop_diag_fluxes():
+    """diag_fluxes = True: the loop records the fluxes with each history
+    record (core/fluid2d.py:203-210, 293-295)."""
+    p = F.Param(modelname="euler", nx=64, ny=64, geometry="closed", adaptable_dt=True, cfl=0.8,
+                tend=0.02, freq_his=0.01, diag_fluxes=True)
+    g = F.Grid(p)
+    f = F.Fluid2d(p, g)
+    f.model.var.get("vorticity")[...] = workloads.turbulence(g, 9)
+    f.model.set_psi_from_vorticity()
+    f.loop()
+    assert [t for t, _, _ in f.output.flx] == [t for t, _, _ in f.output.his]
+    assert len(f.output.flx) == 3
+    for t, kt, flx in f.output.flx:
+        assert flx.shape == (12, 70, 70) and np.isfinite(flx).all()
+        assert np.abs(flx[0]).max() > 0
