@@ -89,6 +89,8 @@ SIGNATURES = {
     "f2d_model_diagnostics": [_p, _p, _p, _dp],
     "f2d_model_run": [_p, _i, _d, _p],
     "f2d_model_diag_fluxes": [_p, _p, _d, _p],
+    "f2d_model_set_noslip": [_p, _p, _p, _p, _p, _d, _d, _d, _i],
+    "f2d_model_set_islands": [_p, _p, _p],
     "f2d_model_last_solve": [_p, _ip, _dp],
 }
 _RESTYPE = {"f2d_last_error": C.c_char_p}

@@ -126,6 +126,10 @@ class Batch:
                 prm = Param(**s)
                 g = Grid(prm)
             prm.checkall()
+            for k in ("noslip", "isisland"):
+                if getattr(prm, k, False):
+                    raise ValueError("%s runs on the single-model path (Fluid2d), not in a "
+                                     "batch" % k)
             self.params.append(prm)
             self.grids.append(g)
         n = len(self.params)

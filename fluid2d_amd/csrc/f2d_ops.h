@@ -65,6 +65,15 @@ void launch_sub_dev_scalar(T* x, const T* sc, size_t cnt, hipStream_t s);
 template <typename T>
 void launch_rk(int mode, T* out, const T* x, const T* k0, const T* k1, const T* k2, double c,
                size_t cnt, hipStream_t s);
+void launch_noslip_source(const int8_t* msk, const double* psi, double* y, double dx, double dy,
+                          int m, int n, hipStream_t s);
+void launch_sub_mean_mask(double* x, const int8_t* mk, const double* d, size_t cnt, hipStream_t s);
+void launch_noslip_momentum(double* x, const int8_t* mk, const double* xr0, const double* yr0,
+                            const double* d, double x2bc, double y2bc, size_t cnt, hipStream_t s);
+void launch_sub_inplace(double* x, const double* y, size_t cnt, hipStream_t s);
+void launch_div_scalar(double* x, double c, size_t cnt, hipStream_t s);
+void launch_dot2(const int8_t* msk, const double* x, const double* a, const double* b, int m,
+                 int n, double* partial, double* out, hipStream_t s);
 void launch_roll_mean(const double* u, double* out, int axis, int m, int n, hipStream_t s);
 void launch_negate(double* x, size_t cnt, hipStream_t s);
 void launch_flux_split(const double* a, const double* b, double* rev, double* irr, double sign,

@@ -480,6 +480,66 @@ __global__ void k_sub_dev_scalar(T* __restrict__ x, const T* __restrict__ s, siz
   if (q < cnt) x[q] = x[q] - s[0];
 }
 
+// No-slip wall source (core/fortran_operators.f90:221-268).  The routine
+// scatters each edge term into the fluid cell beside the wall; gathered
+// here per cell in the order the routine's loop adds them: the cell's own
+// x-edge and y-edge terms, then the term of its east neighbour's x-edge
+// (loop index i+1) and of its north neighbour's y-edge (j+1).  Interior
+// cells only; the halo is zeroed (the caller's periodic fill rewrites it).
+__global__ void k_noslip_source(const int8_t* __restrict__ msk, const double* __restrict__ x,
+                                double* __restrict__ y, double cff, int m, int n) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)m * n) return;
+  const int j = (int)(q / n), i = (int)(q - (size_t)j * n);
+  if (j < NH || j >= m - NH || i < NH || i >= n - NH) {
+    y[q] = 0.;
+    return;
+  }
+  auto M = [&](int jj, int ii) { return (int)msk[at(jj, ii, n)]; };
+  auto X = [&](int jj, int ii) { return x[at(jj, ii, n)]; };
+  // v at the west edge of (jj, ii), u at its south edge
+  auto vx = [&](int jj, int ii) {
+    return (X(jj, ii) + X(jj - 1, ii) - X(jj, ii - 2) - X(jj - 1, ii - 2)) * cff;
+  };
+  auto uy = [&](int jj, int ii) {
+    return -(X(jj, ii) + X(jj, ii - 1) - X(jj - 2, ii) - X(jj - 2, ii - 1)) * cff;
+  };
+  const int mc = M(j, i), me = M(j, i + 1), mn = M(j + 1, i);
+  double r = 0.;
+  if (mc != 0 && M(j, i - 1) + mc == 1) r = r - vx(j, i);
+  if (mc != 0 && M(j - 1, i) + mc == 1) r = r + uy(j, i);
+  if (me == 0 && mc + me == 1) r = r + vx(j, i + 1);
+  if (mn == 0 && mc + mn == 1) r = r - uy(j + 1, i);
+  y[q] = r;
+}
+// s -= d[0]*mskbc (core/operators.py:275-278)
+__global__ void k_sub_mean_mask(double* __restrict__ s, const int8_t* __restrict__ mk,
+                                const double* __restrict__ d, size_t cnt) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < cnt) s[q] = s[q] - d[0] * (double)mk[q];
+}
+// s -= (px*xr0 + py*yr0)*mskbc, px = d[0]/x2bc, py = d[1]/y2bc
+// (core/operators.py:280-287)
+__global__ void k_noslip_momentum(double* __restrict__ s, const int8_t* __restrict__ mk,
+                                  const double* __restrict__ xr0, const double* __restrict__ yr0,
+                                  const double* __restrict__ d, double x2bc, double y2bc,
+                                  size_t cnt) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= cnt) return;
+  const double px = d[0] / x2bc, py = d[1] / y2bc;
+  s[q] = s[q] - (px * xr0[q] + py * yr0[q]) * (double)mk[q];
+}
+// x -= y
+__global__ void k_sub_inplace(double* __restrict__ x, const double* __restrict__ y, size_t cnt) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < cnt) x[q] = x[q] - y[q];
+}
+// x /= c (core/euler.py:112)
+__global__ void k_div_scalar(double* __restrict__ x, double c, size_t cnt) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < cnt) x[q] = x[q] / c;
+}
+
 // Flux diagnostics (core/fluxes.py:118-170).
 // out = 0.5*(u + np.roll(u, 1, axis)): the west (axis 1) or south (axis 0)
 // neighbour, periodic over the whole array
@@ -650,6 +710,28 @@ __global__ void __launch_bounds__(256) k_kemaxu_partial(const int8_t* __restrict
   }
   const bool mx[2] = {false, true};
   block_reduce_store<2>(a, mx, partial);
+}
+
+// {sum s*a, sum s*b} over the interior where msk != 0
+// (core/fortran_diag.f90:3-30, computedotprod, twice)
+__global__ void __launch_bounds__(256) k_dot2_partial(const int8_t* __restrict__ msk,
+                                                      const double* __restrict__ s,
+                                                      const double* __restrict__ a,
+                                                      const double* __restrict__ b, int m, int n,
+                                                      double* partial) {
+  const int N = n - 2 * NH;
+  const size_t tot = (size_t)(m - 2 * NH) * N;
+  double acc[2] = {0., 0.};
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = at(NH + (int)(q / N), NH + (int)(q % N), n);
+    if (msk[c] != 0) {
+      acc[0] += s[c] * a[c];
+      acc[1] += s[c] * b[c];
+    }
+  }
+  const bool mx[2] = {false, false};
+  block_reduce_store<2>(acc, mx, partial);
 }
 
 // fold nblk partial rows of NV values in block order; scale[k] multiplies
@@ -861,6 +943,27 @@ void launch_rk(int mode, T* out, const T* x, const T* k0, const T* k1, const T* 
                size_t cnt, hipStream_t s) {
   hipLaunchKernelGGL(k_rk<T>, grid1(cnt), B1, 0, s, mode, out, x, k0, k1, k2, T(c), cnt);
 }
+void launch_noslip_source(const int8_t* msk, const double* psi, double* y, double dx, double dy,
+                          int m, int n, hipStream_t s) {
+  const double cff = 1. / (2 * dx * dy);  // core/fortran_operators.f90:240
+  hipLaunchKernelGGL(k_noslip_source, grid1((size_t)m * n), B1, 0, s, msk, psi, y, cff, m, n);
+}
+void launch_sub_mean_mask(double* x, const int8_t* mk, const double* d, size_t cnt,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_sub_mean_mask, grid1(cnt), B1, 0, s, x, mk, d, cnt);
+}
+void launch_noslip_momentum(double* x, const int8_t* mk, const double* xr0, const double* yr0,
+                            const double* d, double x2bc, double y2bc, size_t cnt,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_noslip_momentum, grid1(cnt), B1, 0, s, x, mk, xr0, yr0, d, x2bc, y2bc,
+                     cnt);
+}
+void launch_sub_inplace(double* x, const double* y, size_t cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub_inplace, grid1(cnt), B1, 0, s, x, y, cnt);
+}
+void launch_div_scalar(double* x, double c, size_t cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_div_scalar, grid1(cnt), B1, 0, s, x, c, cnt);
+}
 void launch_roll_mean(const double* u, double* out, int axis, int m, int n, hipStream_t s) {
   const size_t cnt = (size_t)m * n;
   hipLaunchKernelGGL(k_roll_mean<double>, grid1(cnt), B1, 0, s, u, out, axis, m, n);
@@ -925,6 +1028,14 @@ void launch_diag(const int8_t* msk, const double* state, int kind, const double*
   sc.v[0] = 0.5;
   sc.v[1] = 0.25;
   hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, s, partial, nb, 8, MaxMask{1}, sc, out);
+}
+void launch_dot2(const int8_t* msk, const double* x, const double* a, const double* b, int m,
+                 int n, double* partial, double* out, hipStream_t s) {
+  const int nb = red_blocks(m, n);
+  hipLaunchKernelGGL(k_dot2_partial, dim3(nb), B1, 0, s, msk, x, a, b, m, n, partial);
+  Scale8 sc;
+  for (int k = 0; k < 8; k++) sc.v[k] = 1.;
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, s, partial, nb, 2, MaxMask{0}, sc, out);
 }
 void launch_domain_sum(const double* x, int m, int n, double div, double* partial, double* out,
                        hipStream_t s) {

@@ -351,6 +351,12 @@ struct f2d_model {
   // result, stage state, persistent tendencies; allocated on first use
   double *fx_x = nullptr, *fx_xe = nullptr, *fx_xw = nullptr, *fx_xs = nullptr;
   double* fx_k[3] = {nullptr, nullptr, nullptr};
+  // no-slip walls (core/operators.py:157-185, :245-290) and islands
+  // (core/island.py:21-43)
+  bool noslip = false, enforce = false, isisland = false;
+  int8_t *msknoslip = nullptr, *mskbc = nullptr;
+  double *xr0 = nullptr, *yr0 = nullptr, *source_b = nullptr, *rhsp = nullptr, *ipsi = nullptr;
+  double bcarea = 0., x2bc = 0., y2bc = 0.;
   double* F(double* base, int k) const { return base + (size_t)k * fsz; }
 };
 
@@ -371,6 +377,13 @@ static void model_free(f2d_model* M) {
   dfree(M->fx_xw);
   dfree(M->fx_xs);
   for (auto& q : M->fx_k) dfree(q);
+  dfree(M->msknoslip);
+  dfree(M->mskbc);
+  dfree(M->xr0);
+  dfree(M->yr0);
+  dfree(M->source_b);
+  dfree(M->rhsp);
+  dfree(M->ipsi);
   if (M->hscal) (void)hipHostFree(M->hscal);
   gmg_free(M->gmg);
   delete M;
@@ -414,9 +427,13 @@ static void rhs_torque(f2d_model* M, const double* x, double* dxdt) {
 }
 
 // core/operators.py:421-498
-static int invert_vorticity(f2d_model* M, double* x, bool full) {
+// island: the islands' rhs correction and boundary values (core/euler.py
+// passes island=isisland for the state's inversions)
+static int invert_vorticity(f2d_model* M, double* x, bool full, bool island = false) {
   double* psi = M->F(x, I_PSI);
+  island = island && M->isisland;
   launch_celltocorner(M->F(x, I_VORT), M->work, M->m, M->n, M->s);
+  if (island) launch_sub_inplace(M->work, M->rhsp, M->fsz, M->s);
   if (!full) {
     g_twovcycle(M->gmg, psi, M->work);
   } else {
@@ -432,10 +449,33 @@ static int invert_vorticity(f2d_model* M, double* x, bool full) {
     }
   }
   launch_mask_mul(psi, M->mskp, M->fsz, M->s);
+  if (island) launch_add_inplace(psi, (const double*)M->ipsi, M->fsz, M->s);
   launch_orthogradient(M->msk, (const double*)psi, M->dx, M->dy, M->F(x, I_U), M->F(x, I_V), M->m,
                        M->n, M->s);
   LAUNCHCHK();
   return F2D_OK;
+}
+
+// Operators.rhs_noslip (core/operators.py:245-290) into `source`, then
+// vorticity -= source, then the fast inversion with islands: add_noslip of
+// core/euler.py:171-179 and core/boussinesq.py:116-118.  The reference
+// first writes cornertocell(psi) into the work array that the source term
+// then overwrites; the cells it leaves are halo cells, which the final
+// periodic fill rewrites, so that pass has no effect and is not run.
+static int add_noslip(f2d_model* M, double* x, double* src) {
+  const int m = M->m, n = M->n;
+  launch_noslip_source(M->msknoslip, (const double*)M->F(x, I_PSI), src, M->dx, M->dy, m, n, M->s);
+  launch_domain_sum(src, m, n, M->bcarea, M->partial, M->dscal, M->s);  // meansource
+  launch_sub_mean_mask(src, M->mskbc, M->dscal, M->fsz, M->s);
+  if (M->enforce) {
+    launch_dot2(M->msk, src, M->xr0, M->yr0, m, n, M->partial, M->dscal + 2, M->s);
+    launch_noslip_momentum(src, M->mskbc, M->xr0, M->yr0, M->dscal + 2, M->x2bc, M->y2bc,
+                           M->fsz, M->s);
+  }
+  launch_fillhalo(src, m, n, M->s);
+  launch_sub_inplace(M->F(x, I_VORT), src, M->fsz, M->s);
+  LAUNCHCHK();
+  return invert_vorticity(M, x, false, true);
 }
 
 // core/euler.py:149-169, core/boussinesq.py:102-114
@@ -591,7 +631,12 @@ static int diag_fluxes(f2d_model* M, const double* x, double dt, double* flx) {
 
 static int model_step(f2d_model* M, double dt) {
   TRY(forward(M, dt));
-  if (M->p.model == 0) return invert_vorticity(M, M->state, true);
+  if (M->noslip) {  // core/euler.py:108-112, core/boussinesq.py:92-94
+    double* src = M->p.model == 0 ? M->F(M->state, I_SRC) : M->source_b;
+    TRY(add_noslip(M, M->state, src));
+    if (M->p.model == 0) launch_div_scalar(src, dt, M->fsz, M->s);
+  }
+  if (M->p.model == 0) return invert_vorticity(M, M->state, true, true);
   // banom = buoyancy - bref, bref = the all-zero buoyancy at construction
   launch_copy(M->F(M->state, I_BANOM), (const double*)M->F(M->state, I_BUOY), M->fsz, M->s);
   LAUNCHCHK();
@@ -956,13 +1001,51 @@ int f2d_model_gmg(f2d_model* M, f2d_gmg** g) {
   return F2D_OK;
 }
 int f2d_model_set_psi_from_vorticity(f2d_model* M) {
-  TRY(invert_vorticity(M, M->state, true));
+  TRY(invert_vorticity(M, M->state, true, true));
   HIPCHK(hipStreamSynchronize(M->s));
   return F2D_OK;
 }
-int f2d_model_invert_vorticity(f2d_model* M, double* x_dev, int full) {
-  TRY(invert_vorticity(M, x_dev, full != 0));
+int f2d_model_invert_vorticity(f2d_model* M, double* x_dev, int flags) {
+  TRY(invert_vorticity(M, x_dev, (flags & 1) != 0, (flags & 2) != 0));
   HIPCHK(hipStreamSynchronize(M->s));
+  return F2D_OK;
+}
+int f2d_model_set_noslip(f2d_model* M, const int8_t* msknoslip_host, const int8_t* mskbc_host,
+                         const double* xr0_host, const double* yr0_host, double bcarea,
+                         double x2bc, double y2bc, int enforce_momentum) {
+  if (!M || !msknoslip_host || !mskbc_host || !xr0_host || !yr0_host)
+    return fail(F2D_E_ARG, "f2d_model_set_noslip: null argument");
+  if (!(bcarea > 0.))
+    return fail(F2D_E_ARG, "f2d_model_set_noslip: no boundary cells (bcarea = %g)", bcarea);
+  const size_t sz = M->fsz;
+  if (!M->msknoslip) {
+    int rc;
+    if ((rc = dalloc(&M->msknoslip, sz)) || (rc = dalloc(&M->mskbc, sz)) ||
+        (rc = dalloc(&M->xr0, sz)) || (rc = dalloc(&M->yr0, sz)) || (rc = dalloc(&M->source_b, sz)))
+      return rc;
+    HIPCHK(hipMemset(M->source_b, 0, sz * sizeof(double)));
+  }
+  HIPCHK(hipMemcpy(M->msknoslip, msknoslip_host, sz, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(M->mskbc, mskbc_host, sz, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(M->xr0, xr0_host, sz * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(M->yr0, yr0_host, sz * sizeof(double), hipMemcpyHostToDevice));
+  M->bcarea = bcarea;
+  M->x2bc = x2bc;
+  M->y2bc = y2bc;
+  M->enforce = enforce_momentum != 0;
+  M->noslip = true;
+  return F2D_OK;
+}
+int f2d_model_set_islands(f2d_model* M, const double* rhsp_host, const double* psi_host) {
+  if (!M || !rhsp_host || !psi_host) return fail(F2D_E_ARG, "f2d_model_set_islands: null argument");
+  const size_t sz = M->fsz;
+  if (!M->rhsp) {
+    int rc;
+    if ((rc = dalloc(&M->rhsp, sz)) || (rc = dalloc(&M->ipsi, sz))) return rc;
+  }
+  HIPCHK(hipMemcpy(M->rhsp, rhsp_host, sz * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(M->ipsi, psi_host, sz * sizeof(double), hipMemcpyHostToDevice));
+  M->isisland = true;
   return F2D_OK;
 }
 int f2d_model_rhs_adv(f2d_model* M, const double* x_dev, double* dxdt_dev) {

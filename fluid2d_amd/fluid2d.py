@@ -55,6 +55,7 @@ class Fluid2d:
         self.dt0 = self.dt
         self.dx, self.dy = grid.dx, grid.dy
         self.grid = grid
+        grid.finalize_msk()  # core/fluid2d.py:64: the user may have edited msk
         # core/fluid2d.py:67-77: momentum is enforced for Euler in closed
         # domains only
         if param.modelname == "euler":
@@ -66,6 +67,9 @@ class Fluid2d:
             self.model = Boussinesq(param, grid)
         else:
             raise ValueError("model %s is outside the device path" % param.modelname)
+        if getattr(param, "isisland", False):  # core/fluid2d.py:110-113
+            grid.island.finalize(self.model.mskp)
+            self.model.set_islands(grid.island.rhsp, grid.island.psi)
         self.enstrophyname = "enstrophy"  # core/fluid2d.py:102-108
         if self.diag_fluxes:  # core/fluid2d.py:116-118
             from .fluxes import Fluxes

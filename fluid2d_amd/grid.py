@@ -7,6 +7,49 @@ directions), and ``domain_integration`` sums the interior in NumPy's order.
 import numpy as np
 
 
+def _celltocorner(xr, xp):
+    """core/fortran_operators.f90:44-64 on host arrays (set-up only): the
+    mean of the four cells around each upper-right corner; the last row and
+    column are left as they are."""
+    xp[:-1, :-1] = 0.25 * (((xr[:-1, :-1] + xr[:-1, 1:]) + xr[1:, :-1]) + xr[1:, 1:])
+
+
+class Island:
+    """core/island.py: islands are solid cell sets (``idx``, a NumPy index
+    as from ``np.where``) whose walls carry a fixed streamfunction ``psi0``.
+    ``finalize`` builds the inversion's rhs correction ``rhsp`` and the
+    boundary values ``psi`` (core/island.py:21-43)."""
+
+    def __init__(self, grid):
+        self.nxl, self.nyl, self.dx, self.dy = grid.nxl, grid.nyl, grid.dx, grid.dy
+        self.rhsp = np.zeros((self.nyl, self.nxl))
+        self.psi = np.zeros((self.nyl, self.nxl))
+        self.nbisland = 0
+        self.data = []
+
+    def add(self, idx, psi0):
+        self.data.append({"idx": idx, "psi0": psi0})
+        self.nbisland += 1
+
+    def finalize(self, mskp_model=None):
+        mskp = np.zeros((self.nyl, self.nxl), dtype=np.int8)
+        work = np.zeros((self.nyl, self.nxl))
+        mskr = np.zeros((self.nyl, self.nxl))
+        for d in self.data:
+            idx, psi0 = d["idx"], d["psi0"]
+            mskr[:, :] = 1.0
+            mskp[:, :] = 0
+            mskr[idx] = 0.0
+            _celltocorner(mskr, work)
+            mskp[work == 1] = 1
+            mskp = 1 - mskp  # the island's corners
+            vort = (np.roll(mskp, -1, axis=1) + np.roll(mskp, -1, axis=0)
+                    + np.roll(mskp, +1, axis=1) + np.roll(mskp, +1, axis=0))
+            z = vort * psi0 / (self.dx * self.dy)
+            self.rhsp[vort > 0] = z[vort > 0]
+            self.psi[mskp == 1] = psi0
+
+
 class Grid:
     def __init__(self, param):
         param.checkall()
@@ -25,6 +68,8 @@ class Grid:
         xr, yr = np.meshgrid(self.x1d, self.y1d)
         self.xr, self.yr = np.ascontiguousarray(xr), np.ascontiguousarray(yr)
         self.set_msk()
+        if getattr(param, "isisland", False):
+            self.island = Island(self)
 
     def set_msk(self):
         """core/grid.py:67-102"""
@@ -39,6 +84,7 @@ class Grid:
         if self.geometry == "disc":
             r = np.sqrt((self.xr / self.Lx - 0.5) ** 2 + (self.yr / self.Ly - 0.5) ** 2)
             self.msk[r >= 0.5] = 0
+        self.msknoslip = self.msk.copy()
         self.finalize_msk()
 
     def finalize_msk(self):
@@ -53,6 +99,23 @@ class Grid:
         self.x2 = self.domain_integration(self.xr0 ** 2 * msk) / self.area
         self.y2 = self.domain_integration(self.yr0 ** 2 * msk) / self.area
         self.r2 = self.xr0 ** 2 + self.yr0 ** 2
+
+    def boundary_mask(self):
+        """Operators.set_boundary_msk (core/operators.py:157-185): mskbc = 1
+        on the no-slip fluid cells next to a wall (periodic halo), its area
+        and the second moments x2bc, y2bc."""
+        msk = self.msknoslip
+        z = (np.roll(msk, -1, axis=1) + np.roll(msk, -1, axis=0)
+             + np.roll(msk, +1, axis=1) + np.roll(msk, +1, axis=0) - 4 * msk)
+        z = z * msk
+        mskbc = self.msk * 0
+        mskbc[z < 0] = 1
+        mskbc *= self.msknoslip
+        self.fill_halo(mskbc)
+        bcarea = self.domain_integration(mskbc)
+        x2bc = self.domain_integration(self.xr0 ** 2 * mskbc * self.msknoslip)
+        y2bc = self.domain_integration(self.yr0 ** 2 * mskbc * self.msknoslip)
+        return mskbc, bcarea, x2bc, y2bc
 
     def domain_integration(self, z2d):
         nh = self.nh

@@ -84,11 +84,11 @@ class Operators:
         _lib.call("f2d_model_rhs_adv", self._m._h, ptr(x), ptr(dxdt))
 
     def invert_vorticity(self, x, flag="full", island=False):
-        """operators.py:421-498 on a device state-shaped array."""
-        if island:
-            raise ValueError("islands are outside the device path")
+        """operators.py:421-498 on a device state-shaped array (island: the
+        islands' rhs correction and boundary psi, when the model has them)."""
         self._m.var.sync_to_device()
-        _lib.call("f2d_model_invert_vorticity", self._m._h, ptr(x), 1 if flag == "full" else 0)
+        _lib.call("f2d_model_invert_vorticity", self._m._h, ptr(x),
+                  (1 if flag == "full" else 0) | (2 if island else 0))
         if ptr(x) == self._m.var.dev.ptr:
             self._m.var.device_changed()
 
@@ -143,6 +143,24 @@ class _Model:
         self.diags = {"maxspeed": 0.0}
         self._coords_sent = False
         self.last_nite, self.last_res = 0, 0.0
+        self.noslip = bool(getattr(param, "noslip", False))
+        self.isisland = bool(getattr(param, "isisland", False))
+        if self.noslip:  # Operators.set_boundary_msk (core/operators.py:97, 157-185)
+            mskbc, bcarea, x2bc, y2bc = grid.boundary_mask()
+            a = [np.ascontiguousarray(v) for v in
+                 (grid.msknoslip.astype(np.int8), mskbc.astype(np.int8), grid.xr0, grid.yr0)]
+            _lib.call("f2d_model_set_noslip", self._h, *[v.ctypes.data_as(C.c_void_p) for v in a],
+                      float(bcarea), float(x2bc), float(y2bc),
+                      1 if getattr(param, "enforce_momentum", False) else 0)
+            self.mskbc, self.bcarea, self.x2bc, self.y2bc = mskbc, bcarea, x2bc, y2bc
+
+    def set_islands(self, rhsp, psi):
+        """core/fluid2d.py:110-113: the islands' rhs correction and boundary
+        psi (core/island.py:21-43) for the state's inversions."""
+        r = np.ascontiguousarray(rhsp, dtype=np.float64)
+        p = np.ascontiguousarray(psi, dtype=np.float64)
+        _lib.call("f2d_model_set_islands", self._h, r.ctypes.data_as(C.c_void_p),
+                  p.ctypes.data_as(C.c_void_p))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -211,7 +229,7 @@ class _Model:
 
 
 class Euler(_Model):
-    """core/euler.py (no forcing, no-slip, sponge or extra tracers)."""
+    """core/euler.py (no forcing, sponge or extra tracers)."""
     kind = "euler"
 
     def diagnostics(self, var, t):
@@ -224,7 +242,7 @@ class Euler(_Model):
 
 
 class Boussinesq(_Model):
-    """core/boussinesq.py (no forcing or no-slip)."""
+    """core/boussinesq.py (no forcing)."""
     kind = "boussinesq"
 
     def diagnostics(self, var, t):

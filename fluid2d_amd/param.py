@@ -63,7 +63,7 @@ AVAIL = {
     "rescaledtime": ["none", "enstrophy"],
 }
 
-UNSUPPORTED_TRUE = ["noslip", "forcing", "isisland"]
+UNSUPPORTED_TRUE = ["forcing"]
 
 
 class Param:

@@ -186,9 +186,22 @@ int f2d_model_corner_mask(f2d_model *M, int8_t **dev);
 int f2d_model_gmg(f2d_model *M, f2d_gmg **g);
 /* core/euler.py:182-183 set_psi_from_vorticity (full inversion) */
 int f2d_model_set_psi_from_vorticity(f2d_model *M);
-/* Operators.invert_vorticity(x, flag) on a model-shaped device array
- * (core/operators.py:421-498); full=0 'fast', 1 'full' */
-int f2d_model_invert_vorticity(f2d_model *M, double *x_dev, int full);
+/* Operators.invert_vorticity(x, flag, island) on a model-shaped device
+ * array (core/operators.py:421-498); flags bit 0: 'full' (else 'fast'),
+ * bit 1: island (the islands set by f2d_model_set_islands, if any) */
+int f2d_model_invert_vorticity(f2d_model *M, double *x_dev, int flags);
+/* No-slip walls (param.noslip): the vorticity source of
+ * Operators.rhs_noslip (core/operators.py:245-290) is added after each step
+ * (core/euler.py:108-112, core/boussinesq.py:92-94).  Host inputs, m*n
+ * each: msknoslip and mskbc (int8) and xr0, yr0, with bcarea, x2bc, y2bc of
+ * Operators.set_boundary_msk (core/operators.py:157-185);
+ * enforce_momentum: the Operators' flag (param.enforce_momentum). */
+int f2d_model_set_noslip(f2d_model *M, const int8_t *msknoslip_host, const int8_t *mskbc_host,
+                         const double *xr0_host, const double *yr0_host, double bcarea,
+                         double x2bc, double y2bc, int enforce_momentum);
+/* Islands (param.isisland, core/island.py:21-43): rhs correction and
+ * boundary psi (host, m*n each) used by the state's inversions. */
+int f2d_model_set_islands(f2d_model *M, const double *rhsp_host, const double *psi_host);
 /* Operators.rhs_adv (core/operators.py:214-236) into dxdt (device) */
 int f2d_model_rhs_adv(f2d_model *M, const double *x_dev, double *dxdt_dev);
 /* cst[3] (umax) used by the next advection (core/fluid2d.py:397) */

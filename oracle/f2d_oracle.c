@@ -30,6 +30,11 @@
  *   models          core/euler.py:103-169, :185-221; core/boussinesq.py:87-153
  *   flux split      core/fluxes.py:100-213 (Fluxes.diag_fluxes, advection,
  *                   rhs_adv), pinned by tests/test_oracle_fluxes.py
+ *   no-slip         core/fortran_operators.f90:221-268
+ *                   (computenoslipsourceterm); core/operators.py:245-290
+ *                   (rhs_noslip); core/euler.py:108-112, :171-179;
+ *                   core/boussinesq.py:92-94, :116-118
+ *   islands         core/operators.py:421-498 (island branch)
  *   NumPy sum       core/grid.py:132-140 (np.sum over the interior slice):
  *                   NumPy 2.x reduces a strided slice in 8192-element
  *                   buffered chunks, each summed pairwise (blocks of 128
@@ -325,6 +330,47 @@ void orc_add_torque(const int8_t *msk, const double *b, double dx, int nh,
 
 /* fortran_multigrid.f90:365-412: every halo cell takes the value of its
  * periodic image in the interior. */
+/* fortran_operators.f90:221-268 (computenoslipsourceterm): the vorticity
+ * source along no-slip walls, from psi at cell corners, in the routine's
+ * own scatter form and loop order.  Writes y on rows 0..m-nh and columns
+ * 0..n-nh (0-based), as the routine does; returns its `total`. */
+double orc_noslip_source(const int8_t *msk, const double *x, double *y, double dx, double dy,
+                         int nh, int m, int n) {
+  const double cff = 1. / (2 * dx * dy);
+  double total = 0.;
+  for (int j = 0; j < nh; j++)
+    for (int i = 0; i < n; i++) AT(y, j, i) = 0.;
+  for (int j = nh; j <= m - nh; j++) {
+    for (int i = 0; i < nh; i++) AT(y, j, i) = 0.;
+    for (int i = nh; i <= n - nh; i++) {
+      AT(y, j, i) = 0.;
+      const int msku = AT(msk, j, i - 1) + AT(msk, j, i);
+      if (msku == 1) {
+        const double v = (AT(x, j, i) + AT(x, j - 1, i) - AT(x, j, i - 2) - AT(x, j - 1, i - 2)) * cff;
+        if (AT(msk, j, i) != 0) {
+          AT(y, j, i) = AT(y, j, i) - v;
+          total = total - v;
+        } else {
+          AT(y, j, i - 1) = AT(y, j, i - 1) + v;
+          total = total + v;
+        }
+      }
+      const int mskv = AT(msk, j - 1, i) + AT(msk, j, i);
+      if (mskv == 1) {
+        const double u = -(AT(x, j, i) + AT(x, j, i - 1) - AT(x, j - 2, i) - AT(x, j - 2, i - 1)) * cff;
+        if (AT(msk, j, i) != 0) {
+          AT(y, j, i) = AT(y, j, i) + u;
+          total = total + u;
+        } else {
+          AT(y, j - 1, i) = AT(y, j - 1, i) - u;
+          total = total - u;
+        }
+      }
+    }
+  }
+  return total;
+}
+
 void orc_fillhalo(double *x, int nh, int m, int n) {
   const int M = m - 2 * nh, N = n - 2 * nh;
   for (int j = 0; j < m; j++) {
@@ -876,6 +922,12 @@ typedef struct {
   /* Fluxes (core/fluxes.py): extended state, its copy, the forward result,
    * the stage state and the three persistent tendencies */
   double *fx_x, *fx_xe, *fx_xw, *fx_xs, *fx_k[3];
+  /* no-slip walls (core/operators.py:157-185, :245-290) and islands
+   * (core/island.py:21-43, core/operators.py:421-498) */
+  int noslip, enforce, isisland;
+  int8_t *msknoslip, *mskbc;
+  double *xr0, *yr0, *source_b, *rhsp, *ipsi;
+  double bcarea, x2bc, y2bc;
 } orc_model;
 
 /* field indices */
@@ -885,6 +937,7 @@ typedef struct {
 #define I_V 3
 #define I_BUOY 4 /* Boussinesq */
 #define I_BANOM 5
+#define I_SOURCE 4 /* Euler */
 
 static double *FLD(orc_model *M, double *base, int k) {
   return base + (size_t)k * M->m * M->n;
@@ -926,11 +979,16 @@ static void rhs_torque(orc_model *M, double *x, double *dxdt) {
   orc_fillhalo(y, M->nh, M->m, M->n);
 }
 
-/* core/operators.py:421-498.  full != 0: gmg.solve + zero mean (perio). */
+/* core/operators.py:421-498.  full & 1: gmg.solve + zero mean (perio);
+ * full & 2: island (rhs corrected by rhsp, psi += the islands' psi). */
 static int invert_vorticity(orc_model *M, double *x, int full) {
   const size_t sz = (size_t)M->m * M->n;
   double *psi = FLD(M, x, I_PSI);
+  const int island = (full & 2) && M->isisland;
+  full &= 1;
   orc_celltocorner(FLD(M, x, I_VORT), M->work, M->m, M->n);
+  if (island)
+    for (size_t q = 0; q < sz; q++) M->work[q] = M->work[q] - M->rhsp[q];
   int st = 0;
   if (!full) {
     orc_gmg_twovcycle(M->gmg, psi, M->work);
@@ -944,8 +1002,33 @@ static int invert_vorticity(orc_model *M, double *x, int full) {
     }
   }
   for (size_t q = 0; q < sz; q++) psi[q] = psi[q] * (double)M->mskp[q];
+  if (island)
+    for (size_t q = 0; q < sz; q++) psi[q] = psi[q] + M->ipsi[q];
   orc_orthogradient(M->msk, psi, M->dx, M->dy, FLD(M, x, I_U), FLD(M, x, I_V), M->m, M->n);
   return st;
+}
+
+/* Operators.rhs_noslip (core/operators.py:245-290) into `source`, then
+ * vorticity -= source; Euler/Boussinesq add_noslip (core/euler.py:171-179,
+ * core/boussinesq.py:116-118) then invert fast (with islands). */
+static int add_noslip(orc_model *M, double *x, double *source) {
+  const int m = M->m, n = M->n, nh = M->nh;
+  const size_t sz = (size_t)m * n;
+  /* fo.cornertocell(psi, work) is overwritten by the source term on every
+   * cell that survives the final halo fill, so it is not restated */
+  orc_noslip_source(M->msknoslip, FLD(M, x, I_PSI), source, M->dx, M->dy, nh, m, n);
+  const double mean = orc_np_sum_interior(source, nh, m, n) / M->bcarea;
+  for (size_t q = 0; q < sz; q++) source[q] = source[q] - mean * (double)M->mskbc[q];
+  if (M->enforce) {
+    const double px = orc_dotprod(M->msk, source, M->xr0, nh, m, n) / M->x2bc;
+    const double py = orc_dotprod(M->msk, source, M->yr0, nh, m, n) / M->y2bc;
+    for (size_t q = 0; q < sz; q++)
+      source[q] = source[q] - (px * M->xr0[q] + py * M->yr0[q]) * (double)M->mskbc[q];
+  }
+  orc_fillhalo(source, nh, m, n);
+  double *w = FLD(M, x, I_VORT);
+  for (size_t q = 0; q < sz; q++) w[q] = w[q] - source[q];
+  return invert_vorticity(M, x, 2);
 }
 
 /* core/euler.py:149-169 and core/boussinesq.py:102-114 */
@@ -1160,6 +1243,8 @@ void orc_model_destroy(orc_model *M) {
   free(M->msk); free(M->mskp); free(M->state); free(M->dx0); free(M->dx1);
   free(M->dx2); free(M->xs); free(M->work);
   free(M->fx_x); free(M->fx_xe); free(M->fx_xw); free(M->fx_xs);
+  free(M->msknoslip); free(M->mskbc); free(M->xr0); free(M->yr0); free(M->source_b);
+  free(M->rhsp); free(M->ipsi);
   for (int k = 0; k < 3; k++) free(M->fx_k[k]);
   free(M);
 }
@@ -1176,12 +1261,55 @@ void orc_model_last_solve(orc_model *M, int *nite, double *res, int *status) {
 }
 
 int orc_model_set_psi_from_vorticity(orc_model *M) {
-  return invert_vorticity(M, M->state, 1);
+  return invert_vorticity(M, M->state, 1 | 2);
 }
 
-/* core/euler.py:103-147 (no forcing, no-slip or sponge), core/boussinesq.py:87-100 */
+/* No-slip walls: msknoslip, mskbc (int8), xr0, yr0 (m x n) and the
+ * scalars of Operators.set_boundary_msk (core/operators.py:157-185). */
+int orc_model_set_noslip(orc_model *M, const int8_t *msknoslip, const int8_t *mskbc,
+                         const double *xr0, const double *yr0, double bcarea, double x2bc,
+                         double y2bc, int enforce) {
+  const size_t sz = (size_t)M->m * M->n;
+  M->msknoslip = (int8_t *)malloc(sz);
+  M->mskbc = (int8_t *)malloc(sz);
+  M->xr0 = (double *)malloc(sz * sizeof(double));
+  M->yr0 = (double *)malloc(sz * sizeof(double));
+  M->source_b = (double *)calloc(sz, sizeof(double));
+  if (!M->msknoslip || !M->mskbc || !M->xr0 || !M->yr0 || !M->source_b) return 2;
+  memcpy(M->msknoslip, msknoslip, sz);
+  memcpy(M->mskbc, mskbc, sz);
+  memcpy(M->xr0, xr0, sz * sizeof(double));
+  memcpy(M->yr0, yr0, sz * sizeof(double));
+  M->bcarea = bcarea; M->x2bc = x2bc; M->y2bc = y2bc;
+  M->enforce = enforce;
+  M->noslip = 1;
+  return 0;
+}
+
+/* Islands (core/island.py:21-43): the rhs correction rhsp and the
+ * boundary values psi, both m x n. */
+int orc_model_set_islands(orc_model *M, const double *rhsp, const double *psi) {
+  const size_t sz = (size_t)M->m * M->n;
+  M->rhsp = (double *)malloc(sz * sizeof(double));
+  M->ipsi = (double *)malloc(sz * sizeof(double));
+  if (!M->rhsp || !M->ipsi) return 2;
+  memcpy(M->rhsp, rhsp, sz * sizeof(double));
+  memcpy(M->ipsi, psi, sz * sizeof(double));
+  M->isisland = 1;
+  return 0;
+}
+
+/* core/euler.py:103-147 (no forcing or sponge), core/boussinesq.py:87-100 */
 int orc_model_step(orc_model *M, double dt) {
   int st = forward(M, dt);
+  if (M->noslip) {
+    double *src = M->kind == MODEL_EULER ? FLD(M, M->state, I_SOURCE) : M->source_b;
+    st |= add_noslip(M, M->state, src);
+    if (M->kind == MODEL_EULER) {
+      const size_t sz = (size_t)M->m * M->n;
+      for (size_t q = 0; q < sz; q++) src[q] = src[q] / dt;
+    }
+  }
   if (M->kind == MODEL_EULER) {
     st |= orc_model_set_psi_from_vorticity(M);
   } else {

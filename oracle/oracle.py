@@ -112,6 +112,9 @@ def lib():
         "orc_model_run": (_i, [_vp, _i, _d, _dp, _dp, _vp]),
         "orc_model_diag_fluxes": (_i, [_vp, _dp, _d, _dp]),
         "orc_model_fx_state": (_vp, [_vp, _i]),
+        "orc_noslip_source": (_d, [_bp, _dp, _dp, _d, _d, _i, _i, _i]),
+        "orc_model_set_noslip": (_i, [_vp, _bp, _bp, _dp, _dp, _d, _d, _d, _i]),
+        "orc_model_set_islands": (_i, [_vp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -151,6 +154,14 @@ def adv_centered(msk, x, y, u, v, cst, nh, order, xflx=None, yflx=None):
 def celltocorner(xr, xp):
     m, n = xr.shape
     lib().orc_celltocorner(_c(xr, np.float64), _c(xp, np.float64), m, n)
+
+
+def noslip_source(msk, x, y, dx, dy, nh):
+    """computenoslipsourceterm (core/fortran_operators.f90:221-268); y in
+    place, returns the routine's total."""
+    m, n = x.shape
+    return lib().orc_noslip_source(_c(msk, np.int8), _c(x, np.float64), _c(y, np.float64), dx, dy,
+                                   nh, m, n)
 
 
 def orthogradient(msk, psi, dx, dy, u, v):
@@ -427,6 +438,21 @@ class OracleModel:
         if st:
             raise RuntimeError("solver is not converging")
         return self.diagnostics()
+
+    def set_noslip(self, msknoslip, mskbc, xr0, yr0, bcarea, x2bc, y2bc, enforce=False):
+        """No-slip walls (core/operators.py:157-185 set-up values)."""
+        self._keep_noslip = [np.ascontiguousarray(a) for a in (msknoslip, mskbc, xr0, yr0)]
+        k = self._keep_noslip
+        if lib().orc_model_set_noslip(self._h, _c(k[0], np.int8), _c(k[1], np.int8),
+                                      _c(k[2], np.float64), _c(k[3], np.float64), bcarea, x2bc,
+                                      y2bc, 1 if enforce else 0):
+            raise MemoryError("orc_model_set_noslip")
+
+    def set_islands(self, rhsp, psi):
+        """Islands (core/island.py:21-43 results)."""
+        r, p = np.ascontiguousarray(rhsp, np.float64), np.ascontiguousarray(psi, np.float64)
+        if lib().orc_model_set_islands(self._h, r, p):
+            raise MemoryError("orc_model_set_islands")
 
     def flux_names(self):
         """fullflx_list of core/fluxes.py:45-50."""

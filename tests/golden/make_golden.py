@@ -130,6 +130,17 @@ def r_add_torque(msk, buoy, dx, nh, g, domega):
     domega[...] = Fd
 
 
+def r_noslip(msk, x, y, dx, dy, nh):
+    """computenoslipsourceterm(msk, x, y, total, dx, dy, nh, m, n)."""
+    m, n = x.shape
+    Fy = F(y)
+    tot = C.c_double(0.0)
+    fcall("fortran_operators", "computenoslipsourceterm_", F(msk, np.int8), F(x), Fy, tot,
+          float(dx), float(dy), nh, m, n)
+    y[...] = Fy
+    return tot.value
+
+
 def r_fillhalo(x, nh):
     m, n = x.shape
     Fx = F(x)
@@ -299,6 +310,27 @@ def kernel_fixtures(nx, ny, seed):
         mskc = (rng.random((m2, n2)) < 0.9).astype(np.int8)
         out[p + "mskc"] = mskc
         out[p + "coarsen"] = r_coarsenmatrix(A9, msk, mskc, nh)
+    return out
+
+
+def noslip_fixtures(nx, ny, seed):
+    """computenoslipsourceterm on each mask (y pre-filled: the routine leaves
+    the last nh-1 rows and columns alone)."""
+    nh = 3
+    rng = np.random.default_rng(seed)
+    m, n = ny + 2 * nh, nx + 2 * nh
+    out = {}
+    for mname, msk in masks(nx, ny, nh, rng).items():
+        x = rng.standard_normal((m, n))
+        y0 = rng.standard_normal((m, n))
+        dx, dy = 1.0 / nx, 1.3 / ny
+        y = y0.copy()
+        tot = r_noslip(msk, x, y, dx, dy, nh)
+        p = "%s_" % mname
+        out[p + "msk"], out[p + "x"], out[p + "y0"] = msk, x, y0
+        out[p + "dxdy"] = np.array([dx, dy])
+        out[p + "noslip"] = y
+        out[p + "total"] = np.array(tot)
     return out
 
 
@@ -567,6 +599,9 @@ def composed_fixture(kind, nx, ny, Lx, Ly, geometry, seed, nsteps):
 def main():
     if not os.path.isdir(REFDIR):
         sys.exit("oracle/_ref missing: run `make -C oracle ref` first")
+    np.savez_compressed(os.path.join(HERE, "noslip_24x16.npz"), **noslip_fixtures(24, 16, 13))
+    if sys.argv[1:] == ["noslip"]:
+        return
     np.savez_compressed(os.path.join(HERE, "kernels_16x16.npz"), **kernel_fixtures(16, 16, 11))
     np.savez_compressed(os.path.join(HERE, "kernels_24x16.npz"), **kernel_fixtures(24, 16, 12))
     np.savez_compressed(os.path.join(HERE, "composed_euler_perio_16.npz"),

@@ -261,3 +261,96 @@ def test_fluid2d_loop_diag_fluxes():
     for t, kt, flx in f.output.flx:
         assert flx.shape == (12, 70, 70) and np.isfinite(flx).all()
         assert np.abs(flx[0]).max() > 0
+
+
+def noslip_pair(kind, nx, ny, Lx, geom, seed, enforce=False, islands=None, **kw):
+    """GPU Fluid2d and oracle with no-slip walls (and islands), from the same
+    host set-up (grid masks, boundary mask, island arrays)."""
+    p = F.Param(modelname=kind, nx=nx, ny=ny, Lx=Lx, Ly=1.0, geometry=geom, noslip=True,
+                enforce_momentum=enforce, isisland=islands is not None, **kw)
+    g = F.Grid(p)
+    if islands is not None:
+        islands(p, g)
+    f = F.Fluid2d(p, g)
+    o = orc.OracleModel(kind, nx, ny, Lx, 1.0, geometry=geom, msk=g.msk,
+                        diffusion=kw.get("diffusion", False), Kdiff=kw.get("Kdiff", 0.0))
+    mskbc, bcarea, x2bc, y2bc = g.boundary_mask()
+    o.set_noslip(g.msknoslip, mskbc, g.xr0, g.yr0, bcarea, x2bc, y2bc, enforce)
+    if islands is not None:
+        o.set_islands(g.island.rhsp, g.island.psi)
+    assert (f.model.mskp == o.mskp).all()
+    return f, o, g
+
+
+@pytest.mark.parametrize("kind,geom,enforce", [("euler", "closed", False),
+                                               ("euler", "closed", True),
+                                               ("boussinesq", "xchannel", False)])
+def test_noslip_matches_oracle(kind, geom, enforce):
+    """No-slip walls (core/operators.py:245-290 after each step): 20 steps
+    against the oracle.  Bitwise expected except with enforce_momentum,
+    whose two dot products are summed in a different order."""
+    nx, Lx = (128, 2.0) if kind == "boussinesq" else (64, 1.0)
+    f, o, g = noslip_pair(kind, nx, 64, Lx, geom, 11, enforce=enforce)
+    w = workloads.turbulence(g, 11) * g.msk
+    f.model.var.get("vorticity")[...] = w
+    o.get("vorticity")[...] = w
+    if kind == "boussinesq":
+        rng = np.random.default_rng(12)
+        b = (rng.standard_normal(g.msk.shape) * 0.1) * g.msk
+        g.fill_halo(b)
+        f.model.var.get("buoyancy")[...] = b
+        o.get("buoyancy")[...] = b
+    f.model.set_psi_from_vorticity()
+    o.set_psi_from_vorticity()
+    d0 = f.model.diagnostics(f.model.var, 0)
+    o.diagnostics()
+    dt = 0.5 * g.dx / d0["maxspeed"]
+    f.model.run(20, dt)
+    o.run(20, dt)
+    compare_states(f, o, "noslip 20 steps")
+    s = f.model.var.state
+    if kind == "euler":
+        src = s[f.model.var.varname_list.index("source")]
+        assert np.abs(src[3:-3, 3:-3]).max() > 0
+        if not enforce:
+            assert (s == o.state).all()
+
+
+def karman(p, g):
+    """An island disc in a channel whose walls carry psi = +-psi0/2 (the
+    pattern of experiments/VonKarman/karman_street.py), no-slip off along
+    the outer walls."""
+    nh = 3
+    r = np.sqrt((g.xr - 0.5) ** 2 + (g.yr - 0.5) ** 2)
+    idx = np.where(r <= 0.12)
+    g.msk[idx] = 0
+    g.msknoslip[idx] = 0
+    g.island.add(idx, 0.0)
+    psi0 = 0.2
+    low = np.zeros_like(g.msk)
+    low[:nh, :] = 1
+    g.island.add(np.where(low == 1), psi0 * 0.5)
+    high = np.zeros_like(g.msk)
+    high[-nh:-1, :] = 1
+    g.island.add(np.where(high == 1), -psi0 * 0.5)
+    g.msknoslip[:nh, :] = 1
+    g.msknoslip[-nh:, :] = 1
+
+
+def test_islands_and_noslip_match_oracle():
+    """Islands (core/island.py, core/operators.py:421-498 island branch) with
+    no-slip walls: the flow past a disc between two walls held at +-psi0/2,
+    from rest, 30 steps against the oracle, bitwise."""
+    f, o, g = noslip_pair("euler", 128, 64, 2.0, "xchannel", 13, islands=karman)
+    f.model.set_psi_from_vorticity()
+    o.set_psi_from_vorticity()
+    compare_states(f, o, "island psi")
+    psi = f.model.var.get("psi")
+    assert abs(psi[3:-3, 3:-3]).max() > 0.05  # the imposed through-flow
+    d0 = f.model.diagnostics(f.model.var, 0)
+    o.diagnostics()
+    dt = 0.5 * g.dx / d0["maxspeed"]
+    f.model.run(30, dt)
+    o.run(30, dt)
+    compare_states(f, o, "islands 30 steps")
+    assert (f.model.var.state == o.state).all()

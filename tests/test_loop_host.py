@@ -137,3 +137,55 @@ def test_enforce_zero_momentum_removes_px_py():
     assert calls == [("device-state", "fast")]
     # only the fluid cells change
     assert (state[0][g.msk == 0] == 0).all()
+
+
+def test_boundary_mask_closed_box():
+    """Operators.set_boundary_msk (core/operators.py:157-185): in a closed
+    n x n box the no-slip boundary cells are the ring of fluid cells along
+    the walls, 4n - 4 of them, with second moments sum(x0^2) over the ring."""
+    n = 16
+    p = F.Param(nx=n, ny=n, geometry="closed")
+    g = F.Grid(p)
+    mskbc, bcarea, x2bc, y2bc = g.boundary_mask()
+    assert bcarea == 4 * n - 4
+    ring = np.zeros((n, n), bool)
+    ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = True
+    assert (mskbc[3:-3, 3:-3] == ring).all()
+    xc = (np.arange(n) + 0.5) / n - 0.5
+    X = np.broadcast_to(xc, (n, n))
+    assert x2bc == pytest.approx(np.sum(X[ring] ** 2), rel=1e-13)
+    assert y2bc == pytest.approx(np.sum(X.T[ring] ** 2), rel=1e-13)
+
+
+def test_island_set_up_by_brute_force():
+    """Island.finalize (core/island.py:21-43): psi0 on every cell corner that
+    touches an island cell (and on the last row and column); rhsp = (number of such corners among the four
+    neighbour corners) * psi0 / (dx dy) wherever that number is positive."""
+    p = F.Param(nx=24, ny=16, Lx=1.5, geometry="closed", isisland=True)
+    g = F.Grid(p)
+    blk = np.zeros_like(g.msk, bool)
+    blk[8:11, 9:12] = True
+    g.msk[blk] = 0
+    g.island.add(np.where(blk), 0.3)
+    g.island.finalize()
+    m, n = g.msk.shape
+    # corner (j, i): cells (j..j+1, i..i+1); celltocorner leaves the last
+    # row and column at 0, which the set-up reads as island corners too
+    corner = np.ones((m, n), bool)
+    for j in range(m - 1):
+        for i in range(n - 1):
+            corner[j, i] = blk[j:j + 2, i:i + 2].any()
+    assert (g.island.psi == np.where(corner, 0.3, 0.0)).all()
+    cnt = np.zeros((m, n), int)
+    for j in range(m):
+        for i in range(n):
+            cnt[j, i] = sum(corner[(j + a) % m, (i + b) % n] for a, b in ((1, 0), (-1, 0), (0, 1), (0, -1)))
+    want = np.where(cnt > 0, cnt * 0.3 / (g.dx * g.dy), 0.0)
+    assert np.allclose(g.island.rhsp, want, rtol=1e-15, atol=0)
+
+
+def test_batch_rejects_noslip_and_islands():
+    from fluid2d_amd.batch import Batch
+    for k in ("noslip", "isisland"):
+        with pytest.raises(ValueError, match=k):
+            Batch([dict(nx=32, ny=32, geometry="closed", **{k: True})])
