@@ -1,6 +1,7 @@
 """Fluid2d driver on the device: core/fluid2d.py ``Fluid2d`` (model factory,
-``loop``, ``set_dt``, ``enforce_zero_momentum``) and the output schedule of
-core/output.py, without files, plotting or logging."""
+``loop``, ``set_dt``, ``enforce_zero_momentum``) and core/output.py's
+history/diagnostic output, without plotting or logging."""
+import os
 import time
 
 import numpy as np
@@ -14,34 +15,93 @@ class Output:
     A diagnostic record is taken whenever ``t >= tnextdiag`` (then
     ``tnextdiag += freq_diag``) and a history record whenever
     ``t >= tnexthis`` (then ``tnexthis += freq_his``); both clocks start at 0.
-    Records stay in host memory: ``diag`` holds ``(t, kt, diags)`` and
-    ``his`` holds ``(t, kt, state)`` (a host copy of the packed state;
-    ``keep_his = False`` keeps only the times); with flux diagnostics on,
-    ``flx`` holds ``(t, kt, fluxes)`` at the history times.  The reference
-    writes them to NetCDF4 files, which this package does not do.
+
+    With ``param.datadir`` set, the records go to NetCDF files in
+    ``<datadir>/<expname>/`` (``<expname>_his.nc`` with ``var_to_save``,
+    ``<expname>_diag.nc`` with ``list_diag``, ``<expname>_flx.nc``), laid out
+    as the reference's (see :mod:`fluid2d_amd.ncio`).  Without it they stay
+    in host memory: ``diag`` holds ``(t, kt, diags)``, ``his`` holds ``(t,
+    kt, state)`` (a host copy of the packed state) and ``flx`` holds ``(t,
+    kt, fluxes)`` with flux diagnostics on.
     """
 
-    def __init__(self, param, keep_his=True):
+    def __init__(self, param, grid=None, varname_list=None, flx_list=None):
+        self.param, self.grid = param, grid
         self.freq_his = float(param.freq_his)
         self.freq_diag = float(param.freq_diag)
         self.tnexthis = 0.0
         self.tnextdiag = 0.0
-        self.keep_his = keep_his
         self.his = []
         self.diag = []
         self.flx = []
+        self.expdir = None
+        self.flx_list = flx_list
+        self.first = True
+        datadir = getattr(param, "datadir", None)
+        self.keep_his = not datadir
+        if datadir and grid is not None:
+            datadir = datadir.replace("~", os.getenv("HOME", "~"))
+            self.expdir = os.path.join(datadir, param.expname)
+            os.makedirs(self.expdir, exist_ok=True)
+            vs = param.var_to_save
+            vs = list(varname_list) if vs == "all" else ([vs] if isinstance(vs, str) else list(vs))
+            for v in vs:  # core/output.py:36-39
+                if v not in varname_list:
+                    raise ValueError("%s is not a model variable => modify param.var_to_save" % v)
+            self.var_to_save, self.varname_list = vs, list(varname_list)
+            self.rename_for_job(None)
+
+    def rename_for_job(self, job):
+        """File names of a run (job None) or of restart job ``job``
+        (core/restart.py:74-89)."""
+        if self.expdir is None:
+            return
+        e = self.param.expname
+        if job is None:
+            his, diag, flx = "%s_his.nc" % e, "%s_diag.nc" % e, "%s_flx.nc" % e
+        else:
+            his, flx = "%s_%02i_his_000.nc" % (e, job), "%s_%02i_flx_000.nc" % (e, job)
+            diag = "%s_%02i_diag.nc" % (e, job)
+        self.hisfile, self.diagfile, self.flxfile = (os.path.join(self.expdir, f)
+                                                     for f in (his, diag, flx))
+
+    def _create(self, data):
+        from .ncio import DiagFile, HisFile
+        ld = self.param.list_diag
+        ld = list(data["diag"].keys()) if ld == "all" else ([ld] if isinstance(ld, str) else ld)
+        self.nchis = HisFile(self.hisfile, self.param, self.grid, self.var_to_save,
+                             self.varname_list)
+        self.nchis.create()
+        self.ncdiag = DiagFile(self.diagfile, ld)
+        self.ncdiag.create()
+        self.ncflx = None
+        if "flx" in data:
+            self.ncflx = HisFile(self.flxfile, self.param, self.grid, self.flx_list, self.flx_list)
+            self.ncflx.create()
 
     def do(self, data, t, kt):
         """``data``: {'diag': dict, 'his': callable returning the state
         [, 'flx': callable returning the fluxes]}."""
+        files = self.expdir is not None
+        if files and self.first:
+            self._create(data)
+        self.first = False
         if t >= self.tnextdiag:
             self.tnextdiag += self.freq_diag
             self.diag.append((t, kt, dict(data["diag"])))
+            if files:
+                self.ncdiag.write(data["diag"], t, kt)
         if t >= self.tnexthis:
             self.tnexthis += self.freq_his
-            self.his.append((t, kt, np.array(data["his"]()) if self.keep_his else None))
+            x = data["his"]() if (files or self.keep_his) else None
+            self.his.append((t, kt, np.array(x) if self.keep_his else None))
+            if files:
+                self.nchis.write(x, t, kt)
             if "flx" in data:
-                self.flx.append((t, kt, data["flx"]()))
+                fl = data["flx"]()
+                self.flx.append((t, kt, fl if self.keep_his else None))
+                if files:
+                    self.ncflx.write(fl, t, kt)
 
 
 class Fluid2d:
@@ -74,7 +134,8 @@ class Fluid2d:
         if self.diag_fluxes:  # core/fluid2d.py:116-118
             from .fluxes import Fluxes
             self.flx = Fluxes(param, grid, self.model.ope)
-        self.output = Output(param)
+        self.output = Output(param, grid, self.model.var.varname_list,
+                             self.flx.fullflx_list if self.diag_fluxes else None)
         self.t = 0.0
         self.kt = 0
         self.stop = False
@@ -117,7 +178,7 @@ class Fluid2d:
             d["flx"] = self.flx.flx.get
         return d
 
-    def loop(self, nsteps=None):
+    def loop(self, nsteps=None, joinhis=True, keepplotalive=False):
         """core/fluid2d.py:188-349: until t >= tend (or ``nsteps`` steps):
         set_dt (shortened to hit history times), step, advance t (by dt, or
         by dt * sqrt(enstrophy) with rescaledtime = 'enstrophy'),

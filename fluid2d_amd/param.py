@@ -44,7 +44,12 @@ DEFAULTS = {
     "isisland": False,
     "enforce_momentum": False,
     "decay": True,
-    # output schedule (core/defaults.json:255-273); files are not written
+    # output (core/defaults.json:229-273, 103-107); files are written only
+    # when datadir is set (the reference's default is ~/data/fluid2d)
+    "datadir": None,
+    "var_to_save": "vorticity",
+    "list_diag": "all",
+    "ninterrestart": 1,
     "freq_his": 1.0,
     "diag_fluxes": False,
     "freq_diag": 1.0,
