@@ -115,18 +115,19 @@ struct Stage {
 // ---------------------------------------------------------------------------
 // k_rhs: tendencies of the tracers for one RK stage.  The stage state of the
 // tracer(s) and of (u, v) is formed once per cell (FP64) into LDS tiles
-// (3-cell halo for the tracer, 1 for the velocities); fluxes, torque and
-// diffusion then read the tiles.
+// (3-cell halo for the tracer, 1 for the velocities); each edge flux of the
+// tile is then computed once into LDS, and each cell takes the divergence,
+// torque and diffusion.
 //
 // FP64 (T = double): the reference arithmetic, operation for operation.
 // Mixed (T = float): the flux arithmetic runs in FP32 on differences
-// q_k - q_c taken in FP64 against the cell's own value q_c, and the part of
-// the tendency carried by q_c -- q_c times the flux divergence of a unit
-// field, D_c -- is added in FP64.  The flux is linear in q, so the sum is the
-// same tendency; rounding then scales with the local variation of q instead
-// of with q itself (a plain FP32 evaluation loses about L/dx digits to the
-// cancellation in f_e - f_w).  Torque and diffusion likewise difference in
-// FP64 and multiply in FP32.
+// q_k - q_L taken in FP64 against the tracer q_L of the edge's left (lower)
+// cell, and q_L times the flux of a unit field is added in FP64.  The flux
+// is linear in q, so the sum is the same flux; rounding then scales with the
+// local variation of q instead of with q itself (a plain FP32 evaluation
+// loses about L/dx digits to the cancellation in f_e - f_w), and the
+// divergence is taken in FP64.  Torque and diffusion difference in FP64 and
+// multiply in FP32.
 // ---------------------------------------------------------------------------
 constexpr int QW = TX + 6, QH = TY + 6;
 
@@ -264,55 +265,52 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   const bool kforce = stage == (in.tscheme == 0 ? 2 : 0);
   T* out = Q + in.k[out_k];
   constexpr bool MIXED = sizeof(T) == 4;
-  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
-    const int J = y0 + r, Ic = x0 + threadIdx.x;
-    if (J >= m - NH || Ic >= n - NH) continue;
-    const int lq = (r + 3) * QW + threadIdx.x + 3;  // cell in the tracer/mask tile
-    const double ue = us[r * (TX + 1) + threadIdx.x + 1], uw = us[r * (TX + 1) + threadIdx.x];
-    const double vn = vs[(r + 1) * TX + threadIdx.x], vsn = vs[r * TX + threadIdx.x];
-    // flux divergence of the unit field (mixed mode only)
-    double Dc = 0.;
-    if (MIXED) {
-      auto one = [](int) { return 1.; };
-      double ce, cw, cn, cs;
-      if (upwind) {
-        ce = up_edge_flux_q(ms + lq, 1, ue, p, one);
-        cw = up_edge_flux_q(ms + lq - 1, 1, uw, p, one);
-        cn = up_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, p, one);
-        cs = up_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, p, one);
-      } else {
-        ce = ce_edge_flux_q(ms + lq, 1, ue, in.order, true, one);
-        cw = ce_edge_flux_q(ms + lq - 1, 1, uw, in.order, true, one);
-        cn = ce_edge_flux_q(ms + lq, (ptrdiff_t)QW, vn, in.order, true, one);
-        cs = ce_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, vsn, in.order, true, one);
-      }
-      Dc = -p.zdx * (ce - cw) - p.zdy * (cn - cs);
-    }
+  // Edge fluxes, each computed once: fx[r][e] on the x-edge between tile
+  // cells (r, e-1) and (r, e), fy[e][c] on the y-edge between (e-1, c) and
+  // (e, c).  FP64: the reference's flux arithmetic.  Mixed: the flux is
+  // linear in q, F(q) = F(q - q_L) + q_L F(1) with q_L the tracer of the
+  // edge's left (lower) cell: the first term in FP32 on differences taken
+  // in FP64, the unit-field flux F(1) and the sum in FP64, so the rounding
+  // scales with the local variation of q.
+  __shared__ double fx[TY][TX + 1];
+  __shared__ double fy[TY + 1][TX];
+  auto edge_flux = [&](const double* qt, int lq, ptrdiff_t st, double vel) -> double {
+    const double ql = MIXED ? qt[lq] : 0.;
+    auto qv = [qt, lq, st, ql](int k) { return T(qt[lq + k * st] - ql); };
+    T f;
+    if (upwind)
+      f = up_edge_flux_q(ms + lq, st, T(vel), pt, qv);
+    else
+      f = ce_edge_flux_q(ms + lq, st, T(vel), in.order, true, qv);
+    if (!MIXED) return (double)f;
+    auto one = [](int) { return 1.; };
+    const double u1 = upwind ? up_edge_flux_q(ms + lq, st, vel, p, one)
+                             : ce_edge_flux_q(ms + lq, st, vel, in.order, true, one);
+    return (double)f + ql * u1;
+  };
 #pragma unroll
-    for (int t = 0; t < NTR; t++) {
-      const double* qt = qs[t];
-      const double qc = MIXED ? qt[lq] : 0.;
-      // tracer at offset k (T-rounded; relative to qc in mixed mode)
-      auto qx = [&](int base) {
-        return [qt, base, qc](int k) { return T(qt[base + k] - qc); };
-      };
-      auto qy = [&](int base) {
-        return [qt, base, qc](int k) { return T(qt[base + k * QW] - qc); };
-      };
-      T fe, fw, fn, fs;
-      if (upwind) {
-        fe = up_edge_flux_q(ms + lq, 1, T(ue), pt, qx(lq));
-        fw = up_edge_flux_q(ms + lq - 1, 1, T(uw), pt, qx(lq - 1));
-        fn = up_edge_flux_q(ms + lq, (ptrdiff_t)QW, T(vn), pt, qy(lq));
-        fs = up_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, T(vsn), pt, qy(lq - QW));
-      } else {
-        fe = ce_edge_flux_q(ms + lq, 1, T(ue), in.order, true, qx(lq));
-        fw = ce_edge_flux_q(ms + lq - 1, 1, T(uw), in.order, true, qx(lq - 1));
-        fn = ce_edge_flux_q(ms + lq, (ptrdiff_t)QW, T(vn), in.order, true, qy(lq));
-        fs = ce_edge_flux_q(ms + lq - QW, (ptrdiff_t)QW, T(vsn), in.order, true, qy(lq - QW));
-      }
-      T val = -pt.zdx * (fe - fw) - pt.zdy * (fn - fs);
-      if (MIXED) val = T((double)val + qc * Dc);
+  for (int t = 0; t < NTR; t++) {
+    const double* qt = qs[t];
+    if (t > 0) __syncthreads();  // the previous tracer's divergence read fx, fy
+    for (int q = tid; q < TY * (TX + 1); q += 256) {
+      const int r = q / (TX + 1), e = q - r * (TX + 1);
+      fx[r][e] = edge_flux(qt, (r + 3) * QW + e + 2, 1, us[r * (TX + 1) + e]);
+    }
+    for (int q = tid; q < (TY + 1) * TX; q += 256) {
+      const int e = q / TX, c = q - e * TX;
+      fy[e][c] = edge_flux(qt, (e + 2) * QW + c + 3, (ptrdiff_t)QW, vs[e * TX + c]);
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+      const int J = y0 + r, Ic = x0 + threadIdx.x;
+      if (J >= m - NH || Ic >= n - NH) continue;
+      const int lq = (r + 3) * QW + threadIdx.x + 3;  // cell in the tracer/mask tile
+      const int c = threadIdx.x;
+      T val;
+      if (MIXED)
+        val = T(-p.zdx * (fx[r][c + 1] - fx[r][c]) - p.zdy * (fy[r + 1][c] - fy[r][c]));
+      else
+        val = T(-pt.zdx * T(fx[r][c + 1] - fx[r][c]) - pt.zdy * T(fy[r + 1][c] - fy[r][c]));
       const int8_t mc = ms[lq];
       if constexpr (NTR > 1) {
         if (t == 0) {  // Boussinesq: y *= msk; add_torque
