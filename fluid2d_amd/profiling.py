@@ -6,6 +6,7 @@ roofline.
     python -m fluid2d_amd.profiling kernels DB [PATTERN]
     python -m fluid2d_amd.profiling pmc DB COUNTER [PATTERN]
     python -m fluid2d_amd.profiling traffic FETCH_DB WRITE_DB KERNEL GRID_X OUT_JSON [TUNING_JSON]
+    python -m fluid2d_amd.profiling sq DB [ROWS]
 
 ``traffic`` scales the per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the
 kernel launches with grid size GRID_X by what the same counters read for
@@ -82,6 +83,29 @@ def traffic(fetch_db, write_db, kernel, grid_x, copy_bytes=5 * (256 << 20)):
     return out
 
 
+def sq_table(path, wave_cycle_rate=None):
+    """Per (kernel, grid) mean per dispatch of the SQ counters of a --pmc
+    run (values summed over the hardware units), with VALU instructions per
+    wavefront, ordered by total VALU instructions."""
+    cur = _db(path).cursor()
+    q = ("select kernel_name, grid_size_x, counter_name, count(*), avg(v) from "
+         "(select dispatch_id, kernel_name, grid_size_x, counter_name, sum(value) as v "
+         " from counters_collection group by dispatch_id, counter_name) "
+         "group by kernel_name, grid_size_x, counter_name")
+    d = {}
+    for k, g, c, n, v in cur.execute(q):
+        e = d.setdefault((k.split("(")[0], g), {"calls": 0})
+        e[c] = v
+        e["calls"] = max(e["calls"], n)
+    rows = []
+    for (k, g), e in d.items():
+        waves = e.get("SQ_WAVES", 0.0)
+        valu = e.get("SQ_INSTS_VALU", 0.0)
+        rows.append((k, g, e["calls"], waves, valu, valu / waves if waves else 0.0, e))
+    rows.sort(key=lambda r: -r[4] * r[2])
+    return rows
+
+
 def main(argv):
     if len(argv) < 2:
         print(__doc__)
@@ -98,6 +122,13 @@ def main(argv):
         print("%-48s %9s %6s %14s" % ("kernel", "grid_x", "calls", argv[2]))
         for r in pmc_table(argv[1], argv[2], pat):
             print("%-48s %9d %6d %14.1f" % (r[0][:48], r[1], r[2], r[3]))
+    elif cmd == "sq":
+        cs = ["SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_BUSY_CYCLES", "SQ_WAIT_INST_ANY"]
+        print("%-48s %8s %6s %9s %12s %9s" % ("kernel", "grid_x", "calls", "waves", "VALU",
+                                              "VALU/wave") + "".join(" %14s" % c for c in cs))
+        for k, g, n, w, v, vw, e in sq_table(argv[1])[:int(argv[2]) if len(argv) > 2 else 30]:
+            print("%-48s %8d %6d %9.0f %12.4g %9.1f" % (k[:48], g, n, w, v, vw)
+                  + "".join(" %14.4g" % e.get(c, 0.0) for c in cs))
     elif cmd == "traffic":
         t = traffic(argv[1], argv[2], argv[3], int(argv[4]))
         if len(argv) > 6:
