@@ -293,6 +293,21 @@ __device__ __forceinline__ T ip_eval(const IpOps<T>& o) {
   return o.m1 <= 0 ? T(0) : r;
 }
 
+// Element i of a per-instance array through a 32-bit byte offset from a
+// uniform base: the load or store then takes the base from scalar registers
+// and one 32-bit vector offset (arrays of one instance stay below 4 GiB).
+__device__ __forceinline__ uint32_t at32(int j, int i, int n) {
+  return (uint32_t)j * (uint32_t)n + (uint32_t)i;
+}
+template <typename T>
+__device__ __forceinline__ T ld32(const T* p, uint32_t i) {
+  return *(const T*)((const char*)p + (size_t)(uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* p, uint32_t i, T v) {
+  *(T*)((char*)p + (size_t)(uint32_t)(i * (uint32_t)sizeof(T))) = v;
+}
+
 // periodic image index of a halo row/column
 __device__ __forceinline__ int img(int j, int m) {
   const int M = m - 2 * NH;
@@ -311,6 +326,29 @@ __device__ __forceinline__ void store_img_mask(PA a, const int8_t* mk, int J, in
   auto put = [&](int j, int i) {
     const size_t c = at(j, i, n);
     a[c] = mk ? v * T(mk[c]) : v;
+  };
+  put(J, I);
+  if (J >= 2 * NH && J < m - 2 * NH && I >= 2 * NH && I < n - 2 * NH) return;  // no image
+  const int M = m - 2 * NH, N = n - 2 * NH;
+  const bool jp = J + M < m && J + M >= m - NH, jm = J - M >= 0 && J - M < NH;
+  const bool ip = I + N < n && I + N >= n - NH, im = I - N >= 0 && I - N < NH;
+  if (jp) put(J + M, I);
+  if (jm) put(J - M, I);
+  if (ip) put(J, I + N);
+  if (im) put(J, I - N);
+  if (jp && ip) put(J + M, I + N);
+  if (jp && im) put(J + M, I - N);
+  if (jm && ip) put(J - M, I + N);
+  if (jm && im) put(J - M, I - N);
+}
+
+// store_img_mask with 32-bit offsets (see ld32)
+template <typename T>
+__device__ __forceinline__ void store_img_mask32(T* a, const int8_t* mk, int J, int I, T v, int m,
+                                                 int n) {
+  auto put = [&](int j, int i) {
+    const uint32_t c = at32(j, i, n);
+    st32(a, c, mk ? v * T(ld32(mk, c)) : v);
   };
   put(J, I);
   if (J >= 2 * NH && J < m - 2 * NH && I >= 2 * NH && I < n - 2 * NH) return;  // no image

@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   // first is used -- the right-hand side of both sweeps, x and the stencil
   // bits on the first sweep's stencil region, and for a prolongation the
   // fine mask there and the coarse tile under it.
-  size_t cc2[IT2];
+  uint32_t cc2[IT2];
   bool ok2[IT2];
   T bv[IT2];
 #pragma unroll
@@ -422,10 +422,10 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
     const int rr = q / JW, cc = q - rr * JW;
     const int j = y0 - 1 + rr, i = x0 - 1 + cc;
     ok2[it] = q < N2 && j <= m - NH && i <= n - NH;
-    cc2[it] = ok2[it] ? at(j, i, n) : 0;
+    cc2[it] = ok2[it] ? at32(j, i, n) : 0u;
   }
 #pragma unroll
-  for (int it = 0; it < IT2; it++) bv[it] = b[cc2[it]];
+  for (int it = 0; it < IT2; it++) bv[it] = ld32(b, cc2[it]);
   const int Ic = x0 + threadIdx.x;
   // x region: rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1], clipped to
   // [1, m-2]; the coarse points its prolongation reads are rows
@@ -447,21 +447,21 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
     const int rr = q / XW, cc = q - rr * XW;
     const int j = y0 - 2 + rr, i = x0 - 2 + cc;
     ok1[it] = q < N1 && j <= m - 2 && i <= n - 2;
-    const size_t c = ok1[it] ? at(j, i, n) : at(1, 1, n);
-    xv[it] = rdx ? xin[c] : T(0);
-    bt[it] = fly ? sb[c] : (uint8_t)0;
+    const uint32_t c = ok1[it] ? at32(j, i, n) : at32(1, 1, n);
+    xv[it] = rdx ? ld32(xin, c) : T(0);
+    bt[it] = fly ? ld32(sb, c) : (uint8_t)0;
     // the fine mask of the prolongation: bit 4 of the stencil bits where
     // they exist (the level's 0/1 mask), else loaded
-    m1v[it] = (rdi && !fly) ? msk[c] : (int8_t)0;
+    m1v[it] = (rdi && !fly) ? ld32(msk, c) : (int8_t)0;
   }
   if (rdi) {
 #pragma unroll
     for (int it = 0; it < ITC; it++) {
       const int q = min(tid + it * NT, NC - 1);
       const int rr = q / CW, cc = q - rr * CW;
-      const size_t c = at(min(cy0 + rr, C.m - 1), min(cx0 + cc, C.n - 1), C.n);
-      cxv[it] = xc[c];
-      cmv[it] = msk2[c];
+      const uint32_t c = at32(min(cy0 + rr, C.m - 1), min(cx0 + cc, C.n - 1), C.n);
+      cxv[it] = ld32(xc, c);
+      cmv[it] = ld32(msk2, c);
     }
   }
   if (skip(dyn, tk.inst, pred)) return;
@@ -565,10 +565,10 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
       const int rr = threadIdx.y + 4 * k, cc = threadIdx.x;
       const int J = y0 + rr;
       if (J >= m - NH || Ic >= n - NH) continue;
-      const size_t c = at(J, Ic, n);
+      const size_t c = at32(J, Ic, n);
       const T v = Jac(c, rr + 2, cc + 2, bs[rr + 1][cc + 1],
                       [&](int dj, int di) { return j1[rr + 1 + dj][cc + 1 + di]; });
-      store_img_mask<T>(xout, mp, J, Ic, v, m, n);
+      store_img_mask32<T>(xout, mp, J, Ic, v, m, n);
     }
   };
   if (full_tile) {
