@@ -175,7 +175,11 @@ template <typename T>
 struct APlanes {
   const T* A;
   size_t P, c;
-  __device__ __forceinline__ T operator()(int k, ptrdiff_t d) const { return A[k * P + c + d]; }
+  __device__ __forceinline__ T operator()(int k, ptrdiff_t d) const {
+    // 32-bit element offset from the uniform base (one instance's operator)
+    const uint32_t o = (uint32_t)k * (uint32_t)P + (uint32_t)c + (uint32_t)(int32_t)d;
+    return *(const T*)((const char*)A + (size_t)(uint32_t)(o * (uint32_t)sizeof(T)));
+  }
 };
 // constant weights w_k where bit k of the cell's stencil byte is set (the
 // finest-level operator: stencil * coef * msk_nb * msk, core/gmg/level.py:292)

@@ -631,10 +631,11 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   // flag 64: the rhs is celltocorner of the T plane at tk.xout
   // (core/fortran_operators.f90:44-64, evaluated as k_c2c does)
   const T* cq = Q + tk.xout;
-  auto bat = [&](size_t c) -> R {
+  auto bat = [&](uint32_t c) -> R {
     if (flags & 64)
-      return R(0.25) * (R(cq[c]) + R(cq[c + 1]) + R(cq[c + n1]) + R(cq[c + n1 + 1]));
-    return bp[c];
+      return R(0.25) * (R(ld32(cq, c)) + R(ld32(cq, c + 1)) + R(ld32(cq, c + n1)) +
+                        R(ld32(cq, c + n1 + 1)));
+    return ld32(bp, c);
   };
   const int fy0 = 2 * tk.y0 - 3, fx0 = 2 * tk.x0 - 3;
   R w[5];
@@ -659,9 +660,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     for (int it = 0; it < ITX; it++) {
       const int q = min(tid + it * NT, NX - 1);
       const int rr = q / XRW, cc = q - rr * XRW;
-      const size_t c = at(min(fy0 - 1 + rr, m1 - 1), min(fx0 - 1 + cc, n1 - 1), n1);
-      xv[it] = x[c];
-      bv[it] = fly ? sb[c] : (uint8_t)0;
+      const uint32_t c = at32(min(fy0 - 1 + rr, m1 - 1), min(fx0 - 1 + cc, n1 - 1), n1);
+      xv[it] = ld32(x, c);
+      bv[it] = fly ? ld32(sb, c) : (uint8_t)0;
     }
   }
   // the coarse mask of the points restricted, then the rhs and mask of the
@@ -672,7 +673,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   for (int it = 0; it < ITR; it++) {
     const int q = min(tid + it * NT, NR - 1);
     const int ry = q / TX2, rx = q - ry * TX2;
-    m2v[it] = msk2[at(min(tk.y0 + ry, m2 - 1), min(tk.x0 + rx, n2 - 1), n2)];
+    m2v[it] = ld32(msk2, at32(min(tk.y0 + ry, m2 - 1), min(tk.x0 + rx, n2 - 1), n2));
   }
   R bq[ITP];
   int8_t mq[ITP];
@@ -681,9 +682,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     const int q = min(tid + it * NT, RY * RX - 1);
     const int rr = q / RX, cc = q - rr * RX;
     const int jf = min(fy0 + rr, m1 - NH), if_ = min(fx0 + cc, n1 - NH);
-    const size_t c = at(img(jf, m1), img(if_, n1), n1);
+    const uint32_t c = at32(img(jf, m1), img(if_, n1), n1);
     bq[it] = bat(c);
-    mq[it] = msk1[c];
+    mq[it] = ld32(msk1, c);
   }
   if (skip(dyn, tk.inst, pred)) {
     if ((flags & 12) && tid == 0) {
@@ -743,7 +744,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
     // coarse interior
     if (rr < 2 * TY2V && cc < 2 * TX2 && !halo && (tk.y0 + rr / 2) < m2 - NH &&
         (tk.x0 + cc / 2) < n2 - NH) {
-      if (flags & 1) store_img_mask<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
+      if (flags & 1) store_img_mask32<T>(Q + tk.xc, nullptr, jf, if_, T(v), m1, n1);
       if (mq[it] != 0) {
         nr += (double)v * (double)v;
         const double bb = (double)bq[it];
@@ -784,7 +785,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       const int cy = 2 * ry + 1, cx = 2 * rx + 1;
       v = restrict_r<R>([&](int dj, int di) { return rs[cy + dj][cx + di]; });
     }
-    store_img_mask<T>(b2, nullptr, J2, I2, T(v), m2, n2);
+    store_img_mask32<T>(b2, nullptr, J2, I2, T(v), m2, n2);
   }
 }
 
