@@ -102,12 +102,13 @@ struct Stage {
     }
   }
   // the same with the kind fixed at compile time (no branch between loads)
+  // and a 32-bit element offset into the instance's planes
   template <int KIND>
-  __device__ __forceinline__ double at(size_t q) const {
-    if constexpr (KIND == 0) return x[q];
-    if constexpr (KIND == 1) return x[q] + c * (double)k0[q];
-    if constexpr (KIND == 2) return x[q] + c * ((double)k0[q] + (double)k1[q]);
-    if constexpr (KIND == 3) return x[q] + c * (double)k1[q];
+  __device__ __forceinline__ double at(uint32_t q) const {
+    if constexpr (KIND == 0) return ld32(x, q);
+    if constexpr (KIND == 1) return ld32(x, q) + c * (double)ld32(k0, q);
+    if constexpr (KIND == 2) return ld32(x, q) + c * ((double)ld32(k0, q) + (double)ld32(k1, q));
+    if constexpr (KIND == 3) return ld32(x, q) + c * (double)ld32(k1, q);
     return 0.;
   }
 };
@@ -195,11 +196,11 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
       const int r = q / QW, c = q - r * QW;
       const int j = y0 - 3 + r, i = x0 - 3 + c;
       const bool ok = j < m && i < n;
-      const size_t e = ok ? at(j, i, n) : 0;
+      const uint32_t e = ok ? at32(j, i, n) : 0u;
       TrVal v;
-      v.m = msk[e];
+      v.m = ld32(msk, e);
       v.q0 = S.template at<K>(e);
-      v.q1 = NTR > 1 ? S.template at<K>(4 * F + e) : 0.;
+      v.q1 = NTR > 1 ? S.template at<K>((uint32_t)(4 * F) + e) : 0.;
       if (!ok) {
         v.m = 0;
         v.q0 = v.q1 = 0.;
@@ -214,14 +215,14 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
       const int r = q / (TX + 1), c = q - r * (TX + 1);  // u on rows y0.., cols x0-1..
       const int j = y0 + r, i = x0 - 1 + c;
       const bool ok = j < m && i < n;
-      const double v = S.template at<K>(2 * F + (ok ? at(j, i, n) : 0));
+      const double v = S.template at<K>((uint32_t)(2 * F) + (ok ? at32(j, i, n) : 0u));
       return ok ? v : 0.;
     }, [&](int q, double v) { us[q] = v; });
     tile_load<((TY + 1) * TX + 255) / 256>(tid, (TY + 1) * TX, [&](int q) {
       const int r = q / TX, c = q - r * TX;  // v on rows y0-1.., cols x0..
       const int j = y0 - 1 + r, i = x0 + c;
       const bool ok = j < m && i < n;
-      const double v = S.template at<K>(3 * F + (ok ? at(j, i, n) : 0));
+      const double v = S.template at<K>((uint32_t)(3 * F) + (ok ? at32(j, i, n) : 0u));
       return ok ? v : 0.;
     }, [&](int q, double v) { vs[q] = v; });
   };
@@ -334,7 +335,7 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
         sm = sm + T(ms[lq + QW]) * T(qt[lq + QW] - tc);
         val = val + coef * sm;
       }
-      store_img_mask<T>(out + (t == 0 ? 0 : 4 * F), nullptr, J, Ic, val, m, n);
+      store_img_mask32<T>(out + (t == 0 ? 0 : 4 * F), nullptr, J, Ic, val, m, n);
     }
   }
 }
