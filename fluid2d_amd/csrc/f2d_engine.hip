@@ -1731,6 +1731,7 @@ struct HostInst {
   int cur[MAXLEV];
   long long buf[MAXLEV][2];  // x buffers per level (level 0 set per phase)
   size_t coarse_lds = 0;
+  std::vector<int8_t> hmsk;  // cell mask (host copy): tile classification
 };
 
 struct Batch {
@@ -1853,6 +1854,55 @@ struct Builder {
     return pick_ty_cells(cells);
   }
 
+  // A finest-level tile whose stencil region may reach a solid cell takes
+  // the smoother's per-cell stencil-bit path, about twice the arithmetic of
+  // a tile with the constant stencil (judged from the cell mask with a
+  // 3-cell margin).
+  bool heavy_tile(const Task& t, int ty, bool rr) const {
+    if (t.lev != 0) return false;
+    const HostInst& h = b.H[t.inst];
+    const int m = h.d.m, n = h.d.n;
+    // the fine rows and columns the task reads (a residual/restriction task
+    // is placed in coarse coordinates, ty coarse rows of TX2 columns)
+    const int j0 = rr ? 2 * t.y0 - 4 : t.y0 - 3, j1 = rr ? 2 * (t.y0 + ty) + 2 : t.y0 + ty + 3;
+    const int i0 = rr ? 2 * t.x0 - 4 : t.x0 - 3, i1 = rr ? 2 * (t.x0 + TX2) + 2 : t.x0 + TX + 3;
+    for (int j = std::max(0, j0); j < std::min(m, j1); j++)
+      for (int i = std::max(0, i0); i < std::min(n, i1); i++)
+        if (h.hmsk[(size_t)j * n + i] == 0) return true;
+    return false;
+  }
+  // Reorder a launch's tasks so that heavy tiles spread over the XCDs and
+  // come first in each XCD's share (task_index gives XCD x a contiguous
+  // share; its first workgroups land on distinct CUs), so that no CU holds
+  // two of them while others hold none.  Only for launches without a
+  // reduction over the task order.  Single-instance launches only: in a
+  // mix, the instance-contiguous order keeps each instance in one XCD's L2.
+  void balance(std::vector<Task>& ts, int ty, bool rr = false) const {
+    const int n = (int)ts.size();
+    if (n < 4 * NXCD) return;
+    for (const Task& t : ts)
+      if (t.inst != ts[0].inst) return;
+    std::vector<Task> heavy, light;
+    for (const Task& t : ts) (heavy_tile(t, ty, rr) ? heavy : light).push_back(t);
+    if (heavy.empty() || light.empty()) return;
+    const int q = n / NXCD, r = n % NXCD;
+    std::vector<std::vector<Task>> chunk(NXCD);
+    for (size_t k = 0; k < heavy.size(); k++) chunk[k % NXCD].push_back(heavy[k]);
+    size_t li = 0;
+    std::vector<Task> out;
+    out.reserve(n);
+    for (int x = 0; x < NXCD; x++) {
+      const size_t want = (size_t)(q + (x < r ? 1 : 0));
+      for (size_t k = 0; k < chunk[x].size() && out.size() < (size_t)n; k++) out.push_back(chunk[x][k]);
+      size_t have = chunk[x].size();
+      while (have < want && li < light.size()) {
+        out.push_back(light[li++]);
+        have++;
+      }
+    }
+    while (li < light.size()) out.push_back(light[li++]);
+    if ((int)out.size() == n) ts.swap(out);
+  }
   // smoother on level l of instance i: input mode, output the other buffer
   Task smooth_task(int i, int l, int mode, int final_mask, std::vector<Task>& out, int ty) {
     HostInst& h = b.H[i];
@@ -1907,6 +1957,8 @@ struct Builder {
         rr_tasks(i, l, h.buf[l][h.cur[l]], -1, -1, rs, ty / 2);
       }
       if (!ts.empty()) {
+        balance(ts, ty);
+        balance(rs, ty / 2, true);
         Launch& Ls = add(L_SMOOTH, ts);
         Ls.pred = pred;
         Ls.ty = ty;
@@ -1926,6 +1978,7 @@ struct Builder {
         smooth_task(i, l, 2, (final_top && dd == D) ? 1 : 0, ts, ty);
       }
       if (!ts.empty()) {
+        balance(ts, ty);
         Launch& Ls = add(L_SMOOTH, ts);
         Ls.pred = pred;
         Ls.ty = ty;
@@ -2596,6 +2649,7 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
     const f2d_model_params& p = params[i];
     HostInst& h = b->H[i];
     h.p = p;
+    h.hmsk.assign(msks[i], msks[i] + (size_t)(p.ny + 2 * NH) * (p.nx + 2 * NH));
     InstD& d = h.d;
     memset(&d, 0, sizeof d);
     if (p.model != 0 && p.model != 1) return bail(efail(F2D_E_ARG, "model"));
