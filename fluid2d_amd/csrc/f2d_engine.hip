@@ -1871,37 +1871,23 @@ struct Builder {
         if (h.hmsk[(size_t)j * n + i] == 0) return true;
     return false;
   }
-  // Reorder a launch's tasks so that heavy tiles spread over the XCDs and
-  // come first in each XCD's share (task_index gives XCD x a contiguous
-  // share; its first workgroups land on distinct CUs), so that no CU holds
-  // two of them while others hold none.  Only for launches without a
-  // reduction over the task order.  Single-instance launches only: in a
-  // mix, the instance-contiguous order keeps each instance in one XCD's L2.
+  // Reorder each XCD's share of a launch's tasks (task_index gives XCD x a
+  // contiguous share; its first workgroups land on distinct CUs) so that
+  // its heavy tiles come first: no CU then holds two of them while others
+  // hold none.  The shares keep their tiles, so neighbouring tiles still
+  // meet in one XCD's L2.  Single-instance launches without a reduction
+  // over the task order only.
   void balance(std::vector<Task>& ts, int ty, bool rr = false) const {
     const int n = (int)ts.size();
     if (n < 4 * NXCD) return;
     for (const Task& t : ts)
       if (t.inst != ts[0].inst) return;
-    std::vector<Task> heavy, light;
-    for (const Task& t : ts) (heavy_tile(t, ty, rr) ? heavy : light).push_back(t);
-    if (heavy.empty() || light.empty()) return;
     const int q = n / NXCD, r = n % NXCD;
-    std::vector<std::vector<Task>> chunk(NXCD);
-    for (size_t k = 0; k < heavy.size(); k++) chunk[k % NXCD].push_back(heavy[k]);
-    size_t li = 0;
-    std::vector<Task> out;
-    out.reserve(n);
     for (int x = 0; x < NXCD; x++) {
-      const size_t want = (size_t)(q + (x < r ? 1 : 0));
-      for (size_t k = 0; k < chunk[x].size() && out.size() < (size_t)n; k++) out.push_back(chunk[x][k]);
-      size_t have = chunk[x].size();
-      while (have < want && li < light.size()) {
-        out.push_back(light[li++]);
-        have++;
-      }
+      const int a = x * q + std::min(x, r), e = a + q + (x < r ? 1 : 0);
+      std::stable_partition(ts.begin() + a, ts.begin() + e,
+                            [&](const Task& t) { return heavy_tile(t, ty, rr); });
     }
-    while (li < light.size()) out.push_back(light[li++]);
-    if ((int)out.size() == n) ts.swap(out);
   }
   // smoother on level l of instance i: input mode, output the other buffer
   Task smooth_task(int i, int l, int mode, int final_mask, std::vector<Task>& out, int ty) {
