@@ -9,7 +9,7 @@ from .device import DeviceArray  # noqa: F401
 from .fluid2d import Fluid2d  # noqa: F401
 from .gmg import Gmg  # noqa: F401
 from .grid import Grid  # noqa: F401
-from .models import Boussinesq, Euler  # noqa: F401
+from .models import Boussinesq, EngineModel, Euler  # noqa: F401
 from .param import Param  # noqa: F401
 
 __version__ = "0.1.0"

@@ -41,6 +41,7 @@ _SIG = {
     "f2d_batch_progress": ([_p, _i, C.POINTER(_i), C.POINTER(C.c_longlong)], _i),
     "f2d_batch_use_graph": ([_p, _i], _i),
     "f2d_batch_set_psi_from_vorticity": ([_p], _i),
+    "f2d_batch_refresh_diagnostics": ([_p], _i),
     "f2d_batch_step": ([_p, _i], _i),
     "f2d_batch_step_async": ([_p, _i], _i),
     "f2d_batch_synchronize": ([_p], _i),
@@ -201,6 +202,11 @@ class Batch:
 
     def set_psi_from_vorticity(self):
         _call("f2d_batch_set_psi_from_vorticity", self._h)
+
+    def refresh_diagnostics(self):
+        """Recompute every instance's diagnostics (and next umax) from its
+        current state."""
+        _call("f2d_batch_refresh_diagnostics", self._h)
 
     def step(self, nsteps=1):
         _call("f2d_batch_step", self._h, int(nsteps))

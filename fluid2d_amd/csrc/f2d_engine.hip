@@ -1753,7 +1753,7 @@ struct Batch {
   std::vector<Task> tasks;
   std::vector<FoldSpan> spans;
   std::vector<int> slots;
-  std::vector<Launch> prog_init, prog_step;
+  std::vector<Launch> prog_init, prog_step, prog_diag;
   hipStream_t s = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -2480,6 +2480,10 @@ static int build_programs(Batch& b) {
   // Boussinesq both start from set_psi_from_vorticity (a full solve).
   bd.solve_full(bd.all());
   bd.diag();
+  // diagnostics of the current states alone (after a host write)
+  b.prog_diag.clear();
+  bd.prog = &b.prog_diag;
+  bd.diag();
   bd.prog = &b.prog_step;
   {
     Launch g;
@@ -2488,7 +2492,7 @@ static int build_programs(Batch& b) {
   }
   bd.rk_step();
   size_t np = 0;
-  for (auto* pr : {&b.prog_init, &b.prog_step})
+  for (auto* pr : {&b.prog_init, &b.prog_step, &b.prog_diag})
     for (const Launch& L : *pr)
       if (L.kind == L_RR || L.kind == L_DIAG) np = std::max(np, (size_t)L.ntask * 8);
   np = std::max(np, (size_t)b.slots.size() * 256 + 256);
@@ -2913,14 +2917,17 @@ static int sync_dyn_to_host(f2d_batch* b) {
 }
 
 int f2d_batch_set_dt(f2d_batch* b, int inst, double dt) {
-  ETRY(sync_dyn_to_host(b));
-  if (inst < 0) {
-    for (int i = 0; i < b->ninst; i++) b->hD[i].dt = dt;
-  } else {
-    if (inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
-    b->hD[inst].dt = dt;
+  if (inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  // only the dt fields are written (the rest of the device record is the
+  // engine's), through the pinned mirror
+  const int i0 = inst < 0 ? 0 : inst, i1 = inst < 0 ? b->ninst : inst + 1;
+  ECHK(hipStreamSynchronize(b->s));
+  for (int i = i0; i < i1; i++) {
+    b->hD[i].dt = dt;
+    ECHK(hipMemcpyAsync(&b->dD[i].dt, &b->hD[i].dt, sizeof(double), hipMemcpyHostToDevice, b->s));
   }
-  return sync_dyn_to_device(b);
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
 }
 
 int f2d_batch_set_steps(f2d_batch* b, int inst, int nsteps) {
@@ -2961,6 +2968,12 @@ int f2d_batch_use_graph(f2d_batch* b, int on) {
 
 int f2d_batch_set_psi_from_vorticity(f2d_batch* b) {
   ETRY(run_prog(*b, b->prog_init));
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+
+int f2d_batch_refresh_diagnostics(f2d_batch* b) {
+  ETRY(run_prog(*b, b->prog_diag));
   ECHK(hipStreamSynchronize(b->s));
   return F2D_OK;
 }

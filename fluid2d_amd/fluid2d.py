@@ -6,7 +6,7 @@ import time
 
 import numpy as np
 
-from .models import Boussinesq, Euler
+from .models import Boussinesq, EngineModel, Euler
 
 
 class Output:
@@ -121,12 +121,18 @@ class Fluid2d:
         if param.modelname == "euler":
             if self.geometry not in ("closed", "disc"):
                 self.enforce_momentum = False
-            self.model = Euler(param, grid)
         elif param.modelname == "boussinesq":
             self.enforce_momentum = False
-            self.model = Boussinesq(param, grid)
         else:
             raise ValueError("model %s is outside the device path" % param.modelname)
+        operator_level = (self.enforce_momentum or self.diag_fluxes
+                          or getattr(param, "noslip", False) or getattr(param, "isisland", False))
+        if getattr(param, "engine", True) and not operator_level:
+            self.model = EngineModel(param, grid)
+        elif param.modelname == "euler":
+            self.model = Euler(param, grid)
+        else:
+            self.model = Boussinesq(param, grid)
         if getattr(param, "isisland", False):  # core/fluid2d.py:110-113
             grid.island.finalize(self.model.mskp)
             self.model.set_islands(grid.island.rhsp, grid.island.psi)

@@ -171,7 +171,8 @@ class Restart:
             f2d.tend += round(t)  # core/restart.py:61
             f2d.t, f2d.dt, f2d.kt = t, dt, kt
             f2d.output.tnextdiag, f2d.output.tnexthis = tnextdiag, tnexthis
-            f2d.model.var.sync_to_device()
+            # the host copy is now the state; the model uploads it (and
+            # recomputes its diagnostics) before its next device operation
             self.nextrestart = self.lastrestart + 1
         else:
             self.nextrestart = 0

@@ -54,6 +54,10 @@ DEFAULTS = {
     "diag_fluxes": False,
     "freq_diag": 1.0,
     "nprint": 20,
+    # this package: Fluid2d runs the step on the batched engine (one
+    # instance, graph-replayed) unless an option needs the operator-level
+    # model (noslip, isisland, enforce_momentum, diag_fluxes)
+    "engine": True,
     "plot_interactive": False,
 }
 

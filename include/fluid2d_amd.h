@@ -283,6 +283,10 @@ int f2d_batch_set_adaptive(f2d_batch *b, int on, double cfl, double dtmax);
 int f2d_batch_use_graph(f2d_batch *b, int on);
 /* set_psi_from_vorticity of every instance, then diagnostics (maxspeed) */
 int f2d_batch_set_psi_from_vorticity(f2d_batch *b);
+/* diagnostics (and the next step's umax, core/fluid2d.py:397) of every
+ * instance's current state, e.g. after a host write of the state
+ * (Euler.diagnostics, core/euler.py:185-221) */
+int f2d_batch_refresh_diagnostics(f2d_batch *b);
 /* nsteps x (RK step + diagnostics), synchronous / enqueue only */
 int f2d_batch_step(f2d_batch *b, int nsteps);
 int f2d_batch_step_async(f2d_batch *b, int nsteps);

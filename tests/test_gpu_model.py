@@ -61,7 +61,7 @@ def compare_states(f, o, what):
 def test_gmg_setup_matches_oracle():
     nx = ny = 64
     o = orc.OracleModel("euler", nx, ny, geometry="closed")
-    p = F.Param(modelname="euler", nx=nx, ny=ny, geometry="closed")
+    p = F.Param(modelname="euler", nx=nx, ny=ny, geometry="closed", engine=False)
     f = F.Fluid2d(p, F.Grid(p))
     og = orc.OracleGmg(nx, ny, 1.0 / nx, 1.0 / ny, o.mskp)
     assert (f.model.mskp == o.mskp).all()
@@ -96,10 +96,13 @@ def test_twovcycle_and_solve_bitwise(geom):
     assert (xd.get() == xo).all()
 
 
-def test_euler_perio_128_200_steps():
+@pytest.mark.parametrize("engine", [True, False])
+def test_euler_perio_128_200_steps(engine):
     """Config: Euler 128x128 doubly periodic, RK3_SSP, UP5 parabolic, 200
-    steps at CFL 0.8 of the initial maxspeed (k^-3 initial field)."""
-    f, o = setup_pair("euler", 128, 128, 1.0, 1.0, "perio", 1)
+    steps at CFL 0.8 of the initial maxspeed (k^-3 initial field), on the
+    engine-backed model and on the operator-level one."""
+    f, o = setup_pair("euler", 128, 128, 1.0, 1.0, "perio", 1, engine=engine)
+    assert isinstance(f.model, F.EngineModel) == engine
     compare_states(f, o, "init")
     d0 = f.model.diagnostics(f.model.var, 0)
     do = o.diagnostics()
@@ -223,7 +226,7 @@ def test_diag_fluxes_matches_oracle(kind, geom, order):
     from fluid2d_amd.fluxes import Fluxes
 
     nx, ny, Lx = (128, 64, 2.0) if kind == "boussinesq" else (64, 64, 1.0)
-    f, o = setup_pair(kind, nx, ny, Lx, 1.0, geom, 8, order=order)
+    f, o = setup_pair(kind, nx, ny, Lx, 1.0, geom, 8, order=order, engine=False)
     d0 = f.model.diagnostics(f.model.var, 0)
     o.diagnostics()
     dt = 0.5 * f.model.grid.dx / max(d0["maxspeed"], 1.0)
