@@ -20,6 +20,9 @@ from .device import DeviceArray, ptr
 class Fluxes:
     def __init__(self, param, grid, ope):
         self._model = ope._m
+        if not hasattr(self._model, "_h"):
+            raise ValueError("flux diagnostics run on the operator-level model: build Fluid2d "
+                             "with param.diag_fluxes = True or param.engine = False")
         kind = self._model.kind
         self.tracer_list = ["vorticity"] + (["buoyancy"] if kind == "boussinesq" else [])
         ext = self.tracer_list + ["uc", "vc"]
