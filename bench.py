@@ -170,10 +170,17 @@ def roofline(b, nsteps=2):
     return dom, shares, pr, achieved
 
 
-def cpu_baseline(name, seconds):
+def cpu_baseline(name, seconds, variant="serial"):
+    """The C oracle on the host cores: "serial" (1 thread) or "omp" (its row
+    loops under OpenMP with OMP_NUM_THREADS threads, the same results)."""
     from fluid2d_amd import workloads
     from oracle import oracle as orc
 
+    orc.use(variant)
+    cores = 1
+    if variant == "omp":
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(cores)
     kw, _, _ = workloads.CONFIGS[name]
     g = workloads.build_grid(name)
     o = orc.OracleModel(kw["modelname"], g.nx, g.ny, g.Lx, g.Ly, geometry=g.geometry,
@@ -191,10 +198,13 @@ def cpu_baseline(name, seconds):
     t0 = time.perf_counter()
     o.run(steps, dt)
     el = time.perf_counter() - t0
-    return {"value": g.nx * g.ny * steps / el, "unit": "cell-updates/s", "cores": 1,
+    orc.use("serial")
+    return {"value": g.nx * g.ny * steps / el, "unit": "cell-updates/s", "cores": cores,
             "kind": "port",
-            "sample": "%s: %d RK3_SSP steps (after 1 untimed) of the C oracle, FP64, 1 thread, "
-                      "gcc -O2 -ffp-contract=off, %.1f s" % (name, steps, el)}
+            "sample": "%s: %d RK3_SSP steps (after 1 untimed) of the C oracle, FP64, %s, "
+                      "gcc -O2 -ffp-contract=off, %.1f s" % (
+                          name, steps, "1 thread" if variant == "serial" else
+                          "%d OpenMP threads over grid rows" % cores, el)}
 
 
 def finest_tile_height(nx, ny, min_tiles):
@@ -434,8 +444,10 @@ def main():
         out["serving"] = summary
         log("serving: p50 %.3f s p99 %.3f s" % (summary["p50_s"], summary["p99_s"]))
     if "cpu" in legs:
-        log("cpu baseline (~%.0f s)" % a.cpu_seconds)
-        out["cpu_baseline"] = cpu_baseline(name, a.cpu_seconds)
+        log("cpu baseline, all cores (~%.0f s)" % a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(name, a.cpu_seconds, "omp")
+        log("cpu baseline, 1 thread (~%.0f s)" % a.cpu_seconds)
+        out["cpu_baseline_1thread"] = cpu_baseline(name, a.cpu_seconds, "serial")
     print(json.dumps(out), flush=True)
 
 

@@ -51,6 +51,15 @@
 
 #define AT(a, j, i) (a)[(size_t)(j) * (size_t)n + (size_t)(i)]
 
+/* Built with -fopenmp (liboracle_omp.so, the multi-core CPU baseline), the
+ * row loops of the stencil kernels run in parallel; each cell's arithmetic
+ * is unchanged, so the results are the same bits.  Reductions stay serial. */
+#ifdef _OPENMP
+#define OMP_ROWS _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_ROWS
+#endif
+
 /* ------------------------------------------------------------------------ */
 /* Advection: upwind orders 1/3/5                                            */
 /* ------------------------------------------------------------------------ */
@@ -129,8 +138,9 @@ int orc_adv_upwind(const int8_t *msk, const double *x, double *y,
   double *fx = (double *)calloc((size_t)nr * nc, sizeof(double));
   double *fy = (double *)calloc((size_t)nr * nc, sizeof(double));
   if (!fx || !fy) { free(fx); free(fy); return 2; }
-  double p[6];
+  OMP_ROWS
   for (int J = j0; J <= j1; J++) {
+    double p[6];
     const int8_t *mr = msk + (size_t)J * n;
     for (int I = i0; I <= i1; I++) {
       double f = 0.;
@@ -164,6 +174,7 @@ int orc_adv_upwind(const int8_t *msk, const double *x, double *y,
   }
 #define FX(J, I) fx[(size_t)((J) - j0) * nc + ((I) - i0)]
 #define FY(J, I) fy[(size_t)((J) - j0) * nc + ((I) - i0)]
+  OMP_ROWS
   for (int J = j0 + 1; J <= j1; J++)
     for (int I = i0 + 1; I <= i1; I++)
       AT(y, J, I) = -zdx * (FX(J, I) - FX(J, I - 1)) - zdy * (FY(J, I) - FY(J - 1, I));
@@ -219,8 +230,9 @@ int orc_adv_centered(const int8_t *msk, const double *x, double *y,
   double *fx = (double *)calloc((size_t)nr * nc, sizeof(double));
   double *fy = (double *)calloc((size_t)nr * nc, sizeof(double));
   if (!fx || !fy) { free(fx); free(fy); return 2; }
-  double p[6];
+  OMP_ROWS
   for (int J = j0; J <= j1; J++) {
+    double p[6];
     const int8_t *mr = msk + (size_t)J * n;
     for (int I = i0; I <= i1; I++) {
       double f = 0.;
@@ -246,6 +258,7 @@ int orc_adv_centered(const int8_t *msk, const double *x, double *y,
   }
 #define FX(J, I) fx[(size_t)((J) - j0) * nc + ((I) - i0)]
 #define FY(J, I) fy[(size_t)((J) - j0) * nc + ((I) - i0)]
+  OMP_ROWS
   for (int J = j0 + 1; J <= j1; J++)
     for (int I = i0 + 1; I <= i1; I++)
       AT(y, J, I) = -zdx * (FX(J, I) - FX(J, I - 1)) - zdy * (FY(J, I) - FY(J - 1, I));
@@ -273,6 +286,7 @@ int orc_adv_centered(const int8_t *msk, const double *x, double *y,
 /* fortran_operators.f90:44-64: value at the upper-right corner; the last
  * row and column are not written. */
 void orc_celltocorner(const double *xr, double *xp, int m, int n) {
+  OMP_ROWS
   for (int j = 0; j < m - 1; j++)
     for (int i = 0; i < n - 1; i++)
       AT(xp, j, i) = 0.25 * (AT(xr, j, i) + AT(xr, j, i + 1) + AT(xr, j + 1, i) +
@@ -284,6 +298,7 @@ void orc_celltocorner(const double *xr, double *xp, int m, int n) {
 void orc_orthogradient(const int8_t *msk, const double *psi, double dx,
                        double dy, double *u, double *v, int m, int n) {
   const double zdx = 1. / dx, zdy = 1. / dy;
+  OMP_ROWS
   for (int j = 1; j < m - 1; j++)
     for (int i = 1; i < n - 1; i++) {
       AT(u, j, i) = (AT(msk, j, i) + AT(msk, j, i + 1) == 2)
@@ -299,6 +314,7 @@ void orc_orthogradient(const int8_t *msk, const double *psi, double dx,
 void orc_add_diffusion(const int8_t *msk, const double *t, double dx,
                        double kdiff, double *dt, int m, int n) {
   const double coef = kdiff / (dx * dx);
+  OMP_ROWS
   for (int j = 1; j < m - 1; j++)
     for (int i = 1; i < n - 1; i++) {
       if (AT(msk, j, i) != 1) continue;
@@ -316,6 +332,7 @@ void orc_add_diffusion(const int8_t *msk, const double *t, double dx,
 void orc_add_torque(const int8_t *msk, const double *b, double dx, int nh,
                     double g, double *dw, int m, int n) {
   const double coef = 0.5 * g / dx;
+  OMP_ROWS
   for (int j = nh; j < m - nh; j++)
     for (int i = nh; i < n - nh; i++) {
       int ml = AT(msk, j, i) + AT(msk, j, i - 1);
@@ -373,6 +390,7 @@ double orc_noslip_source(const int8_t *msk, const double *x, double *y, double d
 
 void orc_fillhalo(double *x, int nh, int m, int n) {
   const int M = m - 2 * nh, N = n - 2 * nh;
+  OMP_ROWS
   for (int j = 0; j < m; j++) {
     int sj = j < nh ? j + M : (j >= m - nh ? j - M : j);
     for (int i = 0; i < n; i++) {
@@ -421,9 +439,11 @@ static double jacobi_at(const int8_t *msk, const double *A, const double *x,
 void orc_smoothtwice(const int8_t *msk, const double *A, double *x,
                      const double *b, double omega, int m, int n, double *work) {
   const double c1 = omega, c2 = 1. - c1;
+  OMP_ROWS
   for (int j = 1; j < m - 1; j++)
     for (int i = 1; i < n - 1; i++)
       AT(work, j, i) = jacobi_at(msk, A, x, b, c1, c2, j, i, m, n);
+  OMP_ROWS
   for (int j = 2; j < m - 2; j++)
     for (int i = 2; i < n - 2; i++)
       AT(x, j, i) = jacobi_at(msk, A, work, b, c1, c2, j, i, m, n);
@@ -436,6 +456,7 @@ void orc_residual(const int8_t *msk, const double *A, const double *x,
                   const double *b, double *r, int m, int n) {
   const double *A1 = APL(A, 0), *A2 = APL(A, 1), *A3 = APL(A, 2),
                *A4 = APL(A, 3), *A5 = APL(A, 4);
+  OMP_ROWS
   for (int j = 1; j < m - 1; j++)
     for (int i = 1; i < n - 1; i++) {
       if (AT(msk, j, i) == 0) { AT(r, j, i) = 0.; continue; }
@@ -457,6 +478,7 @@ void orc_residual(const int8_t *msk, const double *A, const double *x,
 void orc_restrict(const int8_t *msk2, const double *x1, int nh, double *x2,
                   int m2, int n2, int m1, int n1) {
   (void)m1;
+  OMP_ROWS
   for (int J2 = nh - 1; J2 <= m2 - nh - 1; J2++)
     for (int I2 = nh - 1; I2 <= n2 - nh - 1; I2++) {
       double val = 0.;
@@ -487,6 +509,7 @@ int orc_interpolate(const int8_t *msk1, const int8_t *msk2, const double *x2,
 #define MC(jj, ii) msk2[(size_t)(jj) * n2 + (ii)]
 #define MF(jj, ii) msk1[(size_t)(jj) * n1 + (ii)]
 #define FW(jj, ii) x1[(size_t)(jj) * n1 + (ii)]
+  OMP_ROWS
   for (int J2 = 1; J2 <= m2 - 3; J2++)
     for (int I2 = 1; I2 <= n2 - 3; I2++) {
       const int j = 2 * J2 - 2, i = 2 * I2 - 2;
@@ -1060,21 +1083,27 @@ static int rk_forward(orc_model *M, double *x, double *xs, double **k, size_t to
   int st = 0;
   if (M->tscheme == TS_RK3_SSP) {
     M->kstage = 0; st |= rhs(M, x, k0);
+    OMP_ROWS
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + dt * k0[q];
     M->kstage = 1; st |= rhs(M, xs, k1);
     const double c = 0.25 * dt;
+    OMP_ROWS
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + c * (k0[q] + k1[q]);
     M->kstage = 2; st |= rhs(M, xs, k2);
     const double c6 = dt / 6.;
+    OMP_ROWS
     for (size_t q = 0; q < tot; q++) x[q] = x[q] + c6 * ((k0[q] + k1[q]) + 4. * k2[q]);
   } else {
     M->kstage = 0; st |= rhs(M, x, k0);
     const double c3 = dt / 3.;
+    OMP_ROWS
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + c3 * k0[q];
     M->kstage = 1; st |= rhs(M, xs, k1);
     const double c2 = 0.5 * dt;
+    OMP_ROWS
     for (size_t q = 0; q < tot; q++) xs[q] = x[q] + c2 * k1[q];
     M->kstage = 2; st |= rhs(M, xs, k2);
+    OMP_ROWS
     for (size_t q = 0; q < tot; q++) x[q] = x[q] + dt * k2[q];
   }
   return st;

@@ -50,13 +50,30 @@ _vp = C.c_void_p
 _i = C.c_int
 _d = C.c_double
 
+_OMP_PATH = os.path.join(_HERE, "liboracle_omp.so")
+
 _lib = None
+_libs = {}
+_variant = "serial"
 
 
-def build(force=False):
-    """Compile liboracle.so with the committed recipe (oracle/Makefile)."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+def build(force=False, target="liboracle.so"):
+    """Compile liboracle.so (or the OpenMP build, liboracle_omp.so) with the
+    committed recipe (oracle/Makefile)."""
+    path = os.path.join(_HERE, target)
+    if force or not os.path.exists(path):
+        subprocess.check_call(["make", "-s", "-C", _HERE, target])
+
+
+def use(variant):
+    """Select the library later OracleModel objects and calls use: "serial"
+    (liboracle.so) or "omp" (the same code with its row loops under OpenMP,
+    OMP_NUM_THREADS threads; the same results)."""
+    global _lib, _variant
+    if variant not in ("serial", "omp"):
+        raise ValueError(variant)
+    _variant = variant
+    _lib = _libs.get(variant)
 
 
 def _opt(a):
@@ -67,9 +84,10 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(_LIB_PATH):
-        build()
-    L = C.CDLL(_LIB_PATH)
+    path = _OMP_PATH if _variant == "omp" else _LIB_PATH
+    if not os.path.exists(path):
+        build(target=os.path.basename(path))
+    L = C.CDLL(path)
     sig = {
         "orc_adv_upwind": (_i, [_bp, _dp, _dp, _dp, _dp, _vp, _vp, _dp, _i, _i, _i, _i, _i]),
         "orc_adv_centered": (_i, [_bp, _dp, _dp, _dp, _dp, _vp, _vp, _dp, _i, _i, _i, _i]),
@@ -121,6 +139,7 @@ def lib():
         f.restype = res
         f.argtypes = args
     _lib = L
+    _libs[_variant] = L
     return L
 
 
@@ -367,6 +386,7 @@ class OracleModel:
     def __init__(self, modelname, nx, ny, Lx=1.0, Ly=1.0, geometry="perio", order=5,
                  flux_splitting_method="parabolic", aparab=0.05, timestepping="RK3_SSP",
                  diffusion=False, Kdiff=0.0, gravity=1.0, hydroepsilon=1.0, msk=None):
+        self._L = lib()
         self.modelname = modelname
         self.grid = make_grid(nx, ny, Lx, Ly, geometry)
         if msk is not None:
@@ -375,52 +395,52 @@ class OracleModel:
         self.msk = self.grid["msk"]
         kd = Kdiff if isinstance(Kdiff, dict) else {"vorticity": Kdiff, "buoyancy": Kdiff}
         kdiff = np.array([kd.get("vorticity", 0.0), kd.get("buoyancy", 0.0)])
-        self._h = lib().orc_model_create(
+        self._h = self._L.orc_model_create(
             MODELS[modelname], nx, ny, Lx, Ly, self.msk, order, FS_METHODS[flux_splitting_method],
             aparab, SCHEMES[timestepping], 1 if geometry == "perio" else 0,
             1 if diffusion else 0, kdiff, gravity, hydroepsilon)
         self.varname_list = self.VARS[modelname]
         self.nvar = len(self.varname_list)
         self.m, self.n = ny + 6, nx + 6
-        p = lib().orc_model_state(self._h)
+        p = self._L.orc_model_state(self._h)
         buf = (C.c_byte * (self.nvar * self.m * self.n * 8)).from_address(p)
         self.state = np.frombuffer(buf, dtype=np.float64).reshape(self.nvar, self.m, self.n)
-        self.gmg_handle = lib().orc_model_gmg(self._h)
+        self.gmg_handle = self._L.orc_model_gmg(self._h)
         self.diags = {"maxspeed": 0.0}
         self.area = self.grid["area"]
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_model_destroy(self._h)
+            self._L.orc_model_destroy(self._h)
             self._h = None
 
     def get(self, name):
         return self.state[self.varname_list.index(name)]
 
     def tendency(self, k):
-        p = lib().orc_model_tendency(self._h, k)
+        p = self._L.orc_model_tendency(self._h, k)
         buf = (C.c_byte * (self.nvar * self.m * self.n * 8)).from_address(p)
         return np.frombuffer(buf, dtype=np.float64).reshape(self.nvar, self.m, self.n)
 
     @property
     def mskp(self):
-        p = lib().orc_model_mskp(self._h)
+        p = self._L.orc_model_mskp(self._h)
         buf = (C.c_byte * (self.m * self.n)).from_address(p)
         return np.frombuffer(buf, dtype=np.int8).reshape(self.m, self.n)
 
     def set_psi_from_vorticity(self):
-        st = lib().orc_model_set_psi_from_vorticity(self._h)
+        st = self._L.orc_model_set_psi_from_vorticity(self._h)
         if st:
             raise RuntimeError("solver is not converging")
 
     def last_solve(self):
         nite, res, st = C.c_int(), C.c_double(), C.c_int()
-        lib().orc_model_last_solve(self._h, C.byref(nite), C.byref(res), C.byref(st))
+        self._L.orc_model_last_solve(self._h, C.byref(nite), C.byref(res), C.byref(st))
         return nite.value, res.value
 
     def diagnostics(self):
         out = np.zeros(8)
-        lib().orc_model_diagnostics(self._h, self.grid["xr"], self.grid["yr"], out)
+        self._L.orc_model_diagnostics(self._h, self.grid["xr"], self.grid["yr"], out)
         a = self.area
         d = {"maxspeed": out[0], "ke": out[1] / a, "vorticity": out[2] / a,
              "enstrophy": 0.5 * out[3] / a, "raw": out.copy()}
@@ -433,8 +453,8 @@ class OracleModel:
         return d
 
     def step(self, dt):
-        lib().orc_model_set_umax(self._h, self.diags["maxspeed"])
-        st = lib().orc_model_step(self._h, dt)
+        self._L.orc_model_set_umax(self._h, self.diags["maxspeed"])
+        st = self._L.orc_model_step(self._h, dt)
         if st:
             raise RuntimeError("solver is not converging")
         return self.diagnostics()
@@ -443,7 +463,7 @@ class OracleModel:
         """No-slip walls (core/operators.py:157-185 set-up values)."""
         self._keep_noslip = [np.ascontiguousarray(a) for a in (msknoslip, mskbc, xr0, yr0)]
         k = self._keep_noslip
-        if lib().orc_model_set_noslip(self._h, _c(k[0], np.int8), _c(k[1], np.int8),
+        if self._L.orc_model_set_noslip(self._h, _c(k[0], np.int8), _c(k[1], np.int8),
                                       _c(k[2], np.float64), _c(k[3], np.float64), bcarea, x2bc,
                                       y2bc, 1 if enforce else 0):
             raise MemoryError("orc_model_set_noslip")
@@ -451,7 +471,7 @@ class OracleModel:
     def set_islands(self, rhsp, psi):
         """Islands (core/island.py:21-43 results)."""
         r, p = np.ascontiguousarray(rhsp, np.float64), np.ascontiguousarray(psi, np.float64)
-        if lib().orc_model_set_islands(self._h, r, p):
+        if self._L.orc_model_set_islands(self._h, r, p):
             raise MemoryError("orc_model_set_islands")
 
     def flux_names(self):
@@ -464,7 +484,7 @@ class OracleModel:
         the reversible and irreversible edge fluxes, (len(flux_names()), m, n)."""
         flx = np.zeros((len(self.flux_names()), self.m, self.n))
         x = np.ascontiguousarray(self.state.copy())
-        if lib().orc_model_diag_fluxes(self._h, x, dt, flx):
+        if self._L.orc_model_diag_fluxes(self._h, x, dt, flx):
             raise RuntimeError("diag_fluxes failed")
         return flx
 
@@ -476,7 +496,7 @@ class OracleModel:
         nvs, ne = self.nvar + 2, self.nvar + 2 + 2 * nt
         out = []
         for which, planes in ((0, nvs), (1, ne), (2, ne)):
-            p = lib().orc_model_fx_state(self._h, which)
+            p = self._L.orc_model_fx_state(self._h, which)
             buf = (C.c_byte * (planes * self.m * self.n * 8)).from_address(p)
             out.append(np.frombuffer(buf, dtype=np.float64).reshape(planes, self.m, self.n).copy())
         return out
@@ -484,8 +504,8 @@ class OracleModel:
     def run(self, nsteps, dt):
         """Fluid2d.loop body with fixed dt; returns the F-cycle count per step."""
         log = np.zeros(nsteps, dtype=np.int32)
-        lib().orc_model_set_umax(self._h, self.diags["maxspeed"])
-        st = lib().orc_model_run(self._h, nsteps, dt, self.grid["xr"], self.grid["yr"],
+        self._L.orc_model_set_umax(self._h, self.diags["maxspeed"])
+        st = self._L.orc_model_run(self._h, nsteps, dt, self.grid["xr"], self.grid["yr"],
                                  log.ctypes.data_as(_vp))
         if st:
             raise RuntimeError("solver is not converging")
