@@ -357,3 +357,24 @@ def test_islands_and_noslip_match_oracle():
     o.run(30, dt)
     compare_states(f, o, "islands 30 steps")
     assert (f.model.var.state == o.state).all()
+
+
+@pytest.mark.parametrize("kind,geom", [("euler", "disc"), ("euler", "ychannel"),
+                                       ("euler", "xperio"), ("boussinesq", "closed")])
+def test_engine_model_equals_operator_model(kind, geom):
+    """Fluid2d's engine-backed model and the operator-level model take the
+    same 10 steps to the same bits, for the geometries not covered above."""
+    states = []
+    for engine in (True, False):
+        p = F.Param(modelname=kind, nx=64, ny=64, geometry=geom, engine=engine)
+        g = F.Grid(p)
+        f = F.Fluid2d(p, g)
+        assert isinstance(f.model, F.EngineModel) == engine
+        f.model.var.get("vorticity")[...] = workloads.turbulence(g, 21) * g.msk
+        f.model.set_psi_from_vorticity()
+        d = f.model.diagnostics(f.model.var, 0)
+        log = f.model.run(10, 0.5 * g.dx / d["maxspeed"])
+        states.append((f.model.var.state.copy(), list(log)))
+    (a, la), (b, lb) = states
+    assert la == lb
+    assert (a == b).all()
