@@ -294,7 +294,15 @@ class EngineModel:
         self.area = grid.area
         self.order = int(param.order)
         self.aparab = float(param.aparab)
-        self.batch = Batch([(param, grid)], precision="fp64")
+        # launch shapes picked by the autotuner (timed once per grid and
+        # model on a scratch batch, then cached)
+        from . import autotune
+        cands = [{"coarse_lds_budget": 0, "min_tiles": mt, "coarse_wave_cells": wc}
+                 for mt in (0, 1024, 4096) for wc in (256, 0)]
+        tuning = None
+        if getattr(param, "autotune", True):
+            tuning, _ = autotune.tune([(param, grid)], "fp64", cands=cands)
+        self.batch = Batch([(param, grid)], precision="fp64", tuning=tuning)
         p = C.c_void_p()
         bcall("f2d_batch_state_ptr", self.batch._h, 0, C.byref(p))
         nv = len(VARNAMES[self.kind])

@@ -58,6 +58,7 @@ DEFAULTS = {
     # instance, graph-replayed) unless an option needs the operator-level
     # model (noslip, isisland, enforce_momentum, diag_fluxes)
     "engine": True,
+    "autotune": True,  # the engine's launch shapes from the autotuner (cached)
     "plot_interactive": False,
 }
 
