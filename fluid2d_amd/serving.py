@@ -81,14 +81,14 @@ def time_step(req, grid, maxspeed0):
 
 
 class Cohort:
-    def __init__(self, reqs, precision, use_graph=True):
+    def __init__(self, reqs, precision, use_graph=True, tuning=None):
         self.reqs = list(reqs)
         pg = []
         for r in self.reqs:
             p = r.param()
             pg.append((p, Grid(p)))
         self.grids = [g for _, g in pg]
-        self.batch = Batch(pg, precision=precision, use_graph=use_graph)
+        self.batch = Batch(pg, precision=precision, use_graph=use_graph, tuning=tuning)
         for i, r in enumerate(self.reqs):
             for k, v in initial_fields(r, self.grids[i]).items():
                 self.batch.set_field(i, k, v)
@@ -128,7 +128,8 @@ class Server:
     """Continuous-batching request loop on one GPU."""
 
     def __init__(self, hbm_budget=64 << 30, precision="mixed", chunk=20, max_cohort=64,
-                 use_graph=True, keep_state=False):
+                 use_graph=True, keep_state=False, tuning=None):
+        self.tuning = tuning  # launch shapes of every cohort's batch (None: defaults)
         self.hbm_budget = int(hbm_budget)
         self.precision = precision
         self.chunk = int(chunk)
@@ -156,7 +157,7 @@ class Server:
                 admit.append(pending.popleft())
                 used += need
             if admit:
-                c = Cohort(admit, self.precision, self.use_graph)
+                c = Cohort(admit, self.precision, self.use_graph, self.tuning)
                 cohorts.append(c)
                 c.advance(self.chunk)
                 peak = max(peak, used)
