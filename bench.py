@@ -438,12 +438,7 @@ def main():
         from fluid2d_amd import serving
         log("serving %d requests" % a.serve_requests)
         reqs = serving.stream_requests(a.serve_requests)
-        # cohort launch shapes: fine phases spread over >= 1024 workgroups,
-        # coarse levels on the whole workgroup (the best of three tunings
-        # compared on this request stream)
-        srv = serving.Server(hbm_budget=64 << 30, precision="mixed", chunk=20,
-                             tuning={"coarse_lds_budget": 0, "min_tiles": 1024,
-                                     "coarse_wave_cells": 0})
+        srv = serving.Server(hbm_budget=64 << 30, precision="mixed", chunk=20)
         summary, _ = srv.run(reqs)
         summary["arrival_rate_per_s"] = 50.0
         out["serving"] = summary
