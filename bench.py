@@ -20,8 +20,9 @@ Further legs, reported as extra keys of the same JSON line:
   Rayleigh-Taylor run and the ensemble;
 * ``serving``   Poisson arrivals (50/s) of Euler/Boussinesq runs of uneven
   size, continuous batching, p50/p99 time-to-result;
-* ``cpu_baseline`` the C oracle (FP64, 1 thread) on a bounded sample of the
-  headline workload.
+* ``cpu_baseline`` the C oracle (FP64) with its row loops under OpenMP on
+  the host's cores, and ``cpu_baseline_1thread`` the same on one thread,
+  each on a bounded sample of the headline workload.
 
 --gpus N > 1 is refused: this build runs on one GPU.
 """

@@ -7,8 +7,8 @@
 #
 # Flags: -O2 -ffp-contract=off so that the arithmetic follows the source
 # text without fused multiply-adds (the reference's own -march=native build
-# may contract).  fortran_diag.f90 and gmg/fortran_multigrid.f90
-# are fixed-form sources.
+# may contract them, which these checks do not follow).  fortran_diag.f90 and
+# gmg/fortran_multigrid.f90 are fixed-form sources.
 set -euo pipefail
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
