@@ -24,7 +24,7 @@ Further legs, reported as extra keys of the same JSON line:
   the host's cores, and ``cpu_baseline_1thread`` the same on one thread,
   each on a bounded sample of the headline workload.
 
---gpus N > 1 is refused: this build runs on one GPU.
+--gpus N > 1 is refused: the library runs on one GPU.
 """
 import argparse
 import json
@@ -270,7 +270,7 @@ def sequential_plain(n, tune, steps=20):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+        print("bench.py: the library runs on one GPU; --gpus %d is not supported "
               "(instances shard across GPUs with no exchange; not built here)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
