@@ -424,9 +424,10 @@ class OracleModel:
 
     @property
     def mskp(self):
+        """The corner mask, copied out (it outlives the model)."""
         p = self._L.orc_model_mskp(self._h)
         buf = (C.c_byte * (self.m * self.n)).from_address(p)
-        return np.frombuffer(buf, dtype=np.int8).reshape(self.m, self.n)
+        return np.frombuffer(buf, dtype=np.int8).reshape(self.m, self.n).copy()
 
     def set_psi_from_vorticity(self):
         st = self._L.orc_model_set_psi_from_vorticity(self._h)
