@@ -60,6 +60,9 @@ def parse():
                     help="default launch configuration instead of the autotuned one")
     ap.add_argument("--tuning", default=None,
                     help="JSON launch configuration to use instead of running the autotuner")
+    ap.add_argument("--workload", default=HEADLINE,
+                    help="workload of the plain leg; other than %s only with --legs plain "
+                         "(kernel traces of the other configurations)" % HEADLINE)
     ap.add_argument("--eager", action="store_true",
                     help="launch the step kernel by kernel instead of replaying a hipGraph "
                          "(for kernel-trace profilers)")
@@ -309,7 +312,14 @@ def main():
     USE_GRAPH = not a.eager
     if a.tuning:
         FIXED_TUNING = json.loads(a.tuning)
-    name = HEADLINE
+    from fluid2d_amd import workloads
+    name = a.workload
+    if name not in workloads.CONFIGS:
+        raise SystemExit("bench.py: unknown workload %s (one of %s)"
+                         % (name, ", ".join(sorted(workloads.CONFIGS))))
+    if name != HEADLINE and legs != {"plain"}:
+        raise SystemExit("bench.py: --workload %s runs with --legs plain only" % name)
+    wkw = workloads.CONFIGS[name][0]
     out = {
         "metric": "cell-updates/s incl. MG Poisson, batched vs plain @ rel-L2<=1e-6; p99 latency",
         "value": None, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
@@ -317,9 +327,11 @@ def main():
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded k^-3 turbulence, vortex pair, Rayleigh-Taylor inits)",
         "config": {"workload": name, "path": "plain FP64, one instance, graph-replayed step",
-                   "nx": 1024, "ny": 1024, "model": "euler", "geometry": "closed",
+                   "nx": wkw["nx"], "ny": wkw["ny"], "model": wkw["modelname"],
+                   "geometry": wkw["geometry"],
                    "scheme": "RK3_SSP", "advection": "UP5 parabolic",
-                   "mg": "2 V-cycles per inverted stage + F-cycle solve (maxite 4, tol 1e-11)"},
+                   "mg": ("2 V-cycles per inverted stage + F-cycle solve (maxite 4, tol 1e-11)"
+                          if wkw["modelname"] == "euler" else "2 V-cycles per inverted stage")},
         "roofline": None, "cpu_baseline": None,
     }
     plain_state = None
@@ -333,10 +345,13 @@ def main():
         cells = g.nx * g.ny
         out["value"] = cells * a.steps / el
         out["ms_per_step"] = el / a.steps * 1e3
-        nite = b.solve_info(0)[0]
-        out["config"]["F_cycles_last_step"] = nite
-        nf = float(nite)
-        wpc = 109 + 28 * nf
+        if wkw["modelname"] == "euler":
+            # words per cell-update: 109 + 28 per F-cycle of the full solve
+            nite = b.solve_info(0)[0]
+            out["config"]["F_cycles_last_step"] = nite
+            wpc = 109 + 28 * float(nite)
+        else:
+            wpc = 162.0  # Boussinesq: fast inversions only, no full solve
         out["step_roofline"] = {
             "words_per_cell_update": wpc,
             "achieved": out["value"] * wpc * 8 / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
