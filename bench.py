@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--tuning", default=None,
                     help="JSON launch configuration to use instead of running the autotuner")
     ap.add_argument("--workload", default=HEADLINE,
-                    help="workload of the plain leg; other than %s only with --legs plain "
+                    help="workload of the plain leg; other than %s only with --legs plain or copy,plain "
                          "(kernel traces of the other configurations)" % HEADLINE)
     ap.add_argument("--eager", action="store_true",
                     help="launch the step kernel by kernel instead of replaying a hipGraph "
@@ -317,8 +317,8 @@ def main():
     if name not in workloads.CONFIGS:
         raise SystemExit("bench.py: unknown workload %s (one of %s)"
                          % (name, ", ".join(sorted(workloads.CONFIGS))))
-    if name != HEADLINE and legs != {"plain"}:
-        raise SystemExit("bench.py: --workload %s runs with --legs plain only" % name)
+    if name != HEADLINE and not legs <= {"plain", "copy"}:
+        raise SystemExit("bench.py: --workload %s runs with --legs plain (and copy) only" % name)
     wkw = workloads.CONFIGS[name][0]
     out = {
         "metric": "cell-updates/s incl. MG Poisson, batched vs plain @ rel-L2<=1e-6; p99 latency",
