@@ -2450,8 +2450,21 @@ static void launch_bytes(const Batch& b, const Launch& L, double* alg, double* i
 namespace f2d {
 namespace eng {
 
+// Destroys the step graph once the stream has drained: an executable graph
+// must not be destroyed while a launch of it may still be in flight.
+static void drop_graph(Batch& b) {
+  if (!b.gexec && !b.graph) return;
+  if (b.s) (void)hipStreamSynchronize(b.s);
+  if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.gexec = nullptr;
+  b.graph = nullptr;
+}
+
 static void batch_free(Batch* b) {
   if (!b) return;
+  // everything enqueued on the batch's stream reads the buffers freed below
+  if (b->s) (void)hipStreamSynchronize(b->s);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->P) (void)hipFree(b->P);
@@ -2561,6 +2574,28 @@ static int first_coarse_level(const InstD& d, size_t budget, size_t tsz, int* k0
   return F2D_OK;
 }
 
+// The kernels address each array of one instance through 32-bit byte
+// offsets from a uniform base (at32/ld32/st32, APlanes, Stage::at): every
+// per-instance array must stay below 4 GiB.  The largest are the state
+// (nvar planes, FP64), a tendency (nvar planes of the pool type), the FP64
+// level-0 operator A0d (5 planes) and the level operators (5 planes of the
+// pool type, the finest the largest).  Host-only: no device call.
+static int extent_check(const f2d_model_params& p, size_t tsz) {
+  const double lim = 4294967296.0;  // 2^32 bytes
+  const double F = (double)(p.ny + 2 * NH) * (double)(p.nx + 2 * NH);
+  const double nvar = p.model == 0 ? 5. : 6.;
+  struct {
+    const char* what;
+    double bytes;
+  } ext[] = {{"state", nvar * F * 8.}, {"tendency", nvar * F * (double)tsz},
+             {"FP64 level-0 operator", 5. * F * 8.}, {"level-0 operator", 5. * F * (double)tsz}};
+  for (const auto& e : ext)
+    if (e.bytes >= lim)
+      return efail(F2D_E_ARG, "grid %dx%d: the %s takes %.0f bytes; each array of one instance "
+                   "must stay below 2^32 bytes", p.nx, p.ny, e.what, e.bytes);
+  return F2D_OK;
+}
+
 // apply a tuning: coarse split per instance, tile policy, programs rebuilt
 static int apply_tuning(Batch& b, const f2d_batch_tuning* t) {
   const size_t budget = lds_budget(t ? t->coarse_lds_budget : 0);
@@ -2582,11 +2617,8 @@ static int apply_tuning(Batch& b, const f2d_batch_tuning* t) {
   }
   std::vector<InstD> di(b.ninst);
   for (int i = 0; i < b.ninst; i++) di[i] = b.H[i].d;
+  drop_graph(b);
   ECHK(hipMemcpy(b.dI, di.data(), b.ninst * sizeof(InstD), hipMemcpyHostToDevice));
-  if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
-  if (b.graph) (void)hipGraphDestroy(b.graph);
-  b.gexec = nullptr;
-  b.graph = nullptr;
   return build_programs(b);
 }
 
@@ -2606,6 +2638,13 @@ int f2d_batch_create(int ninst, const f2d_model_params* params, const int8_t* co
                      int precision, const f2d_batch_tuning* tuning, f2d_batch** out) {
   if (ninst < 1) return efail(F2D_E_ARG, "ninst must be >= 1");
   if (precision != 0 && precision != 1) return efail(F2D_E_ARG, "precision must be 0 or 1");
+  // host-only checks first: nothing below runs for a rejected batch
+  for (int i = 0; i < ninst; i++) {
+    if (params[i].nx < 1 || params[i].ny < 1)
+      return efail(F2D_E_ARG, "instance %d: grid %dx%d", i, params[i].nx, params[i].ny);
+    const int rc = extent_check(params[i], precision == 0 ? sizeof(double) : sizeof(float));
+    if (rc) return efail(rc, "instance %d: %s", i, std::string(e_err).c_str());
+  }
   f2d_batch* b = new f2d_batch();
   b->prec = precision;
   const size_t tsz = precision == 0 ? sizeof(double) : sizeof(float);
@@ -2887,7 +2926,10 @@ int f2d_batch_set_field(f2d_batch* b, int inst, int field, const double* host) {
   const InstD& d = b->H[inst].d;
   if (field < 0 || field >= d.nvar) return efail(F2D_E_ARG, "field out of range");
   const size_t F = (size_t)d.m * d.n;
-  ECHK(hipMemcpy(b->P + d.state + field * F, host, F * 8, hipMemcpyHostToDevice));
+  // the batch stream is non-blocking: steps still in flight read the state
+  ECHK(hipStreamSynchronize(b->s));
+  ECHK(hipMemcpyAsync(b->P + d.state + field * F, host, F * 8, hipMemcpyHostToDevice, b->s));
+  ECHK(hipStreamSynchronize(b->s));
   return F2D_OK;
 }
 
@@ -2949,15 +2991,11 @@ int f2d_batch_progress(f2d_batch* b, int inst, int* steps_left, long long* steps
 }
 
 int f2d_batch_set_adaptive(f2d_batch* b, int on, double cfl, double dtmax) {
+  // the flags are kernel arguments of the captured step: recapture
+  drop_graph(*b);
   b->adapt = on;
   b->cfl = cfl;
   b->dtmax = dtmax;
-  if (b->gexec) {
-    (void)hipGraphExecDestroy(b->gexec);
-    (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
-  }
   return F2D_OK;
 }
 

@@ -93,6 +93,11 @@ void launch_kemaxu(const int8_t* msk, const double* u, const double* v, int m, i
                    double* partial, double* out, hipStream_t s);
 void launch_diag(const int8_t* msk, const double* state, int kind, const double* xr,
                  const double* yr, int m, int n, double* partial, double* out, hipStream_t s);
+// masked interior sums of core/fortran_diag.f90: kind 0 computedotprod
+// (out[0] = sum x*y), 1 computesum (out[0] = sum x), 2 computesumandnorm
+// (out[0..1] = sum x, sum x^2); y is read by kind 0 only
+void launch_msum(int kind, const int8_t* msk, const double* x, const double* y, int m, int n,
+                 double* partial, double* out, hipStream_t s);
 // NumPy-order interior sum; out[0] = sum * 1. / div (div == 0: the sum)
 void launch_domain_sum(const double* x, int m, int n, double div, double* partial, double* out,
                        hipStream_t s);

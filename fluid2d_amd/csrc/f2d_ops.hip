@@ -734,6 +734,40 @@ __global__ void __launch_bounds__(256) k_dot2_partial(const int8_t* __restrict__
   block_reduce_store<2>(acc, mx, partial);
 }
 
+// The three masked sums of core/fortran_diag.f90 over the interior:
+//   KIND 0 computedotprod (:3-30)     {sum x*y}        where msk != 0
+//   KIND 1 computesum (:63-90)        {sum x}          where msk != 0
+//   KIND 2 computesumandnorm (:93-120){sum x, sum x^2} where msk == 1
+template <int KIND>
+__global__ void __launch_bounds__(256) k_msum_partial(const int8_t* __restrict__ msk,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y, int m, int n,
+                                                      double* partial) {
+  const int N = n - 2 * NH;
+  const size_t tot = (size_t)(m - 2 * NH) * N;
+  constexpr int NV = KIND == 2 ? 2 : 1;
+  double acc[NV];
+  for (int k = 0; k < NV; k++) acc[k] = 0.;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = at(NH + (int)(q / N), NH + (int)(q % N), n);
+    const int8_t mk = msk[c];
+    if constexpr (KIND == 0) {
+      if (mk != 0) acc[0] += x[c] * y[c];
+    } else if constexpr (KIND == 1) {
+      if (mk != 0) acc[0] += x[c];
+    } else {
+      if (mk == 1) {
+        acc[0] += x[c];
+        acc[NV - 1] += x[c] * x[c];
+      }
+    }
+  }
+  bool mx[NV];
+  for (int k = 0; k < NV; k++) mx[k] = false;
+  block_reduce_store<NV>(acc, mx, partial);
+}
+
 // fold nblk partial rows of NV values in block order; scale[k] multiplies
 // result k (diagnostic constants 0.25 / 0.5).
 __global__ void k_fold(const double* __restrict__ partial, int nblk, int nv, MaxMask mm,
@@ -1036,6 +1070,20 @@ void launch_dot2(const int8_t* msk, const double* x, const double* a, const doub
   Scale8 sc;
   for (int k = 0; k < 8; k++) sc.v[k] = 1.;
   hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, s, partial, nb, 2, MaxMask{0}, sc, out);
+}
+void launch_msum(int kind, const int8_t* msk, const double* x, const double* y, int m, int n,
+                 double* partial, double* out, hipStream_t s) {
+  const int nb = red_blocks(m, n);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_msum_partial<0>, dim3(nb), B1, 0, s, msk, x, y, m, n, partial);
+  else if (kind == 1)
+    hipLaunchKernelGGL(k_msum_partial<1>, dim3(nb), B1, 0, s, msk, x, y, m, n, partial);
+  else
+    hipLaunchKernelGGL(k_msum_partial<2>, dim3(nb), B1, 0, s, msk, x, y, m, n, partial);
+  Scale8 sc;
+  for (int k = 0; k < 8; k++) sc.v[k] = 1.;
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, s, partial, nb, kind == 2 ? 2 : 1, MaxMask{0},
+                     sc, out);
 }
 void launch_domain_sum(const double* x, int m, int n, double div, double* partial, double* out,
                        hipStream_t s) {

@@ -845,6 +845,36 @@ int f2d_kemaxu_f64(const int8_t* msk, const double* u, const double* v, int nh, 
   return F2D_OK;
 }
 
+int f2d_dotprod_f64(const int8_t* msk, const double* x, const double* y, int nh, int m, int n,
+                    double* out, f2d_stream_t stream) {
+  TRY(check_field(nh, m, n));
+  double* partial;
+  TRY(red_scratch(&partial));
+  launch_msum(0, msk, x, y, m, n, partial, out, S(stream));
+  LAUNCHCHK();
+  return F2D_OK;
+}
+
+int f2d_sum_f64(const int8_t* msk, const double* x, int nh, int m, int n, double* out,
+                f2d_stream_t stream) {
+  TRY(check_field(nh, m, n));
+  double* partial;
+  TRY(red_scratch(&partial));
+  launch_msum(1, msk, x, nullptr, m, n, partial, out, S(stream));
+  LAUNCHCHK();
+  return F2D_OK;
+}
+
+int f2d_sumandnorm_f64(const int8_t* msk, const double* x, int nh, int m, int n, double* out,
+                       f2d_stream_t stream) {
+  TRY(check_field(nh, m, n));
+  double* partial;
+  TRY(red_scratch(&partial));
+  launch_msum(2, msk, x, nullptr, m, n, partial, out, S(stream));
+  LAUNCHCHK();
+  return F2D_OK;
+}
+
 int f2d_domain_sum_f64(const double* x, int nh, int m, int n, double* out, f2d_stream_t stream) {
   TRY(check_field(nh, m, n));
   double* partial;

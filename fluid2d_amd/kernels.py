@@ -27,6 +27,9 @@ interpolate                       core/gmg/fortran_multigrid.f90:415-498
 coarsenmatrix                     core/gmg/fortran_multigrid.f90:706-811 + fill
 computenorm                       core/gmg/fortran_multigrid.f90:813-839
 computekemaxu                     core/fortran_diag.f90:155-195
+computedotprod                    core/fortran_diag.f90:3-30
+computesum                        core/fortran_diag.f90:63-90
+computesumandnorm                 core/fortran_diag.f90:93-120
 domain_integration                core/grid.py:132-140 (NumPy summation order)
 ================================  =============================================
 """
@@ -131,6 +134,25 @@ def computenorm(msk, x, nh):
 def computekemaxu(msk, u, v, nh):
     m, n = _shape(u)
     r = _scalar_call("f2d_kemaxu_f64", 2, ptr(msk), ptr(u), ptr(v), nh, m, n)
+    return float(r[0]), float(r[1])
+
+
+def computedotprod(msk, x, y, nh):
+    """Sum of x*y over the interior where msk != 0."""
+    m, n = _shape(x)
+    return float(_scalar_call("f2d_dotprod_f64", 1, ptr(msk), ptr(x), ptr(y), nh, m, n)[0])
+
+
+def computesum(msk, x, nh):
+    """Sum of x over the interior where msk != 0."""
+    m, n = _shape(x)
+    return float(_scalar_call("f2d_sum_f64", 1, ptr(msk), ptr(x), nh, m, n)[0])
+
+
+def computesumandnorm(msk, x, nh):
+    """(sum of x, sum of x^2) over the interior where msk == 1."""
+    m, n = _shape(x)
+    r = _scalar_call("f2d_sumandnorm_f64", 2, ptr(msk), ptr(x), nh, m, n)
     return float(r[0]), float(r[1])
 
 

@@ -130,6 +130,25 @@ int f2d_norm2_f64(const int8_t *msk, const double *x, int nh, int m, int n, doub
 int f2d_kemaxu_f64(const int8_t *msk, const double *u, const double *v, int nh, int m, int n,
                    double *out, f2d_stream_t stream);
 
+/* core/fortran_diag.f90:3-30 computedotprod(msk,x,y,nh) -> z: out[0] = sum of
+ * x*y over the interior where msk != 0 (device scalar; tree order, rounding
+ * differs from the serial reference by rounding only).  Called by
+ * core/euler.py:197-206 (momentum, enstrophy source terms). */
+int f2d_dotprod_f64(const int8_t *msk, const double *x, const double *y, int nh, int m, int n,
+                    double *out, f2d_stream_t stream);
+
+/* core/fortran_diag.f90:63-90 computesum(msk,x,nh) -> y: out[0] = sum of x
+ * over the interior where msk != 0 (device scalar, tree order). */
+int f2d_sum_f64(const int8_t *msk, const double *x, int nh, int m, int n, double *out,
+                f2d_stream_t stream);
+
+/* core/fortran_diag.f90:93-120 computesumandnorm(msk,x,nh) -> (y, y2):
+ * out[0] = sum of x, out[1] = sum of x^2 over the interior where msk == 1
+ * (device scalars, tree order).  Called by core/euler.py:194-196 and
+ * core/boussinesq.py:132-139. */
+int f2d_sumandnorm_f64(const int8_t *msk, const double *x, int nh, int m, int n, double *out,
+                       f2d_stream_t stream);
+
 /* core/grid.py:132-140 domain_integration: np.sum over the interior in
  * NumPy's own summation order (bit-identical to NumPy 2.x); out[0] (device). */
 int f2d_domain_sum_f64(const double *x, int nh, int m, int n, double *out, f2d_stream_t stream);

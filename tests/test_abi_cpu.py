@@ -71,3 +71,43 @@ def test_grid_mask_matches_oracle_grid():
         assert (g.msk == o["msk"]).all()
         assert g.area == o["area"]
         assert (g.xr == o["xr"]).all() and (g.yr == o["yr"]).all()
+
+
+def _params(nx, ny, model=0):
+    from fluid2d_amd import batch
+    p = _lib.f2d_model_params()
+    p.model, p.nx, p.ny, p.Lx, p.Ly = model, nx, ny, 1.0, 1.0
+    p.order, p.flux_splitting, p.aparab = 5, 1, 0.05
+    p.hydroepsilon = 1.0
+    assert batch  # the engine's ctypes table is bound on first use
+    return p
+
+
+@pytest.mark.parametrize("nx,ny,model,prec,ok", [
+    # FP64 level-0 operator, 5 planes of (ny+6)(nx+6) doubles: 2^32 bytes at
+    # about 10,360 cells a side; the state (5 or 6 planes) comes just below
+    (16384, 16384, 0, 0, False),
+    (10368, 10368, 0, 1, False),
+    (16384, 8, 1, 0, True),       # long and thin: small arrays, passes the guard
+    (9472, 9472, 1, 0, False),    # Boussinesq state: 6 planes of FP64
+])
+def test_batch_rejects_arrays_beyond_32bit_offsets(nx, ny, model, prec, ok):
+    """Kernels address one instance's arrays through 32-bit byte offsets;
+    f2d_batch_create rejects any instance whose state, tendency or operator
+    array reaches 2^32 bytes, before any device call (no GPU here)."""
+    import ctypes as C
+    from fluid2d_amd import batch
+    lib = batch._L()
+    F = (ny + 6) * (nx + 6)
+    largest = max((5 if model == 0 else 6) * F * 8, 5 * F * 8)
+    assert (largest < 2 ** 32) == ok
+    if ok:
+        return  # an accepted batch would go on to allocate on the device
+    arr = (_lib.f2d_model_params * 1)(_params(nx, ny, model))
+    msks = (C.c_void_p * 1)(None)  # never read: the guard runs first
+    h = C.c_void_p()
+    rc = lib.f2d_batch_create(1, arr, msks, prec, None, C.byref(h))
+    assert rc == _lib.F2D_E_ARG
+    msg = lib.f2d_batch_last_error().decode()
+    assert "2^32" in msg and "instance 0" in msg
+    assert h.value is None

@@ -143,6 +143,22 @@ def test_diag_vs_fortran(kf, mname):
     gke, gmx = kf[p + "diag"][0], kf[p + "diag"][1]
     assert mx == gmx
     assert abs(ke - gke) <= 1e-13 * abs(gke)
+    # computesumandnorm / computedotprod / computesum: the reference sums
+    # serially, the GPU in a tree; the two differ by rounding only, bounded
+    # by 1e-13 of the sum of the terms' magnitudes
+    x, y0 = kf[p + "x"], kf[p + "y0"]
+    inner = (slice(NH, -NH), slice(NH, -NH))
+    fl1 = msk[inner] == 1
+    fl = msk[inner] != 0
+    xi, yi = x[inner], y0[inner]
+    s, s2 = K.computesumandnorm(D(msk), D(x), NH)
+    g = kf[p + "diag"]
+    assert abs(s - g[2]) <= 1e-13 * np.abs(xi[fl1]).sum()
+    assert abs(s2 - g[3]) <= 1e-13 * (xi[fl1] ** 2).sum()
+    z = K.computedotprod(D(msk), D(x), D(y0), NH)
+    assert abs(z - g[4]) <= 1e-13 * np.abs(xi[fl] * yi[fl]).sum()
+    sm = K.computesum(D(msk), D(x), NH)
+    assert abs(sm - g[5]) <= 1e-13 * np.abs(xi[fl]).sum()
 
 
 @pytest.mark.parametrize("shape", [(22, 22), (70, 70), (134, 134), (262, 518), (1030, 1030)])
