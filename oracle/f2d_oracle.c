@@ -951,6 +951,8 @@ typedef struct {
   int8_t *msknoslip, *mskbc;
   double *xr0, *yr0, *source_b, *rhsp, *ipsi;
   double bcarea, x2bc, y2bc;
+  /* test-only: every inversion solved to convergence (temporal-order checks) */
+  int exact_inv;
 } orc_model;
 
 /* field indices */
@@ -1013,7 +1015,18 @@ static int invert_vorticity(orc_model *M, double *x, int full) {
   if (island)
     for (size_t q = 0; q < sz; q++) M->work[q] = M->work[q] - M->rhsp[q];
   int st = 0;
-  if (!full) {
+  if (M->exact_inv) {
+    /* test-only: the fast and the full inversion both solved to the
+     * round-off floor, so that a step integrates the semi-discrete ODE */
+    int nite; double res;
+    st = orc_gmg_solve(M->gmg, psi, M->work, 60, 1e-14, &nite, &res);
+    if (st == 3) st = 0; /* stalls at round-off count as converged here */
+    M->last_nite = nite; M->last_res = res; M->last_status = st;
+    if (full && M->perio) {
+      const double psim = orc_np_sum_interior(psi, M->nh, M->m, M->n) * 1. / M->area;
+      for (size_t q = 0; q < sz; q++) psi[q] -= psim;
+    }
+  } else if (!full) {
     orc_gmg_twovcycle(M->gmg, psi, M->work);
   } else {
     int nite; double res;
@@ -1106,6 +1119,28 @@ static int rk_forward(orc_model *M, double *x, double *xs, double **k, size_t to
     OMP_ROWS
     for (size_t q = 0; q < tot; q++) x[q] = x[q] + dt * k2[q];
   }
+  return st;
+}
+
+/* Test-only: one step of rk_forward on dx/dt = lam * x (elementwise), so
+ * that the stage algebra can be checked against the stability polynomial
+ * fixed by the scheme's order conditions. */
+static const double *g_lam;
+static int linear_rhs(orc_model *M, double *x, double *dxdt) {
+  for (size_t q = 0; q < (size_t)M->n; q++) dxdt[q] = g_lam[q] * x[q];
+  return 0;
+}
+int orc_rk_linear(int tscheme, int cnt, const double *lam, double *x, double dt) {
+  orc_model M;
+  memset(&M, 0, sizeof M);
+  M.tscheme = tscheme;
+  M.n = cnt;
+  double *buf = calloc((size_t)4 * cnt, sizeof(double));
+  if (!buf) return 1;
+  double *k[3] = {buf, buf + cnt, buf + 2 * cnt};
+  g_lam = lam;
+  const int st = rk_forward(&M, x, buf + 3 * cnt, k, (size_t)cnt, dt, linear_rhs);
+  free(buf);
   return st;
 }
 
@@ -1285,6 +1320,8 @@ double *orc_model_tendency(orc_model *M, int k) {
 int8_t *orc_model_mskp(orc_model *M) { return M->mskp; }
 orc_gmg *orc_model_gmg(orc_model *M) { return M->gmg; }
 void orc_model_set_umax(orc_model *M, double umax) { M->cst[3] = umax; }
+/* test-only knob: solve every inversion to convergence */
+void orc_model_set_exact_inversions(orc_model *M, int on) { M->exact_inv = on; }
 void orc_model_last_solve(orc_model *M, int *nite, double *res, int *status) {
   *nite = M->last_nite; *res = M->last_res; *status = M->last_status;
 }

@@ -133,6 +133,8 @@ def lib():
         "orc_noslip_source": (_d, [_bp, _dp, _dp, _d, _d, _i, _i, _i]),
         "orc_model_set_noslip": (_i, [_vp, _bp, _bp, _dp, _dp, _d, _d, _d, _i]),
         "orc_model_set_islands": (_i, [_vp, _dp, _dp]),
+        "orc_model_set_exact_inversions": (None, [_vp, _i]),
+        "orc_rk_linear": (_i, [_i, _i, _dp, _dp, _d]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -272,6 +274,15 @@ def dotprod(msk, x, y, nh):
 def sum_masked(msk, x, nh):
     m, n = x.shape
     return lib().orc_sum(_c(msk, np.int8), _c(x, np.float64), nh, m, n)
+
+
+def rk_linear(timestepping, lam, x, dt):
+    """Test-only: one step of the oracle's RK3_SSP / RK3 stage algebra on
+    dx/dt = lam * x (elementwise); x is updated in place."""
+    lam = np.ascontiguousarray(lam, np.float64)
+    assert x.dtype == np.float64 and x.flags.c_contiguous and x.shape == lam.shape
+    if lib().orc_rk_linear(SCHEMES[timestepping], x.size, lam, x, dt):
+        raise MemoryError("orc_rk_linear")
 
 
 def np_sum_interior(x, nh):
@@ -501,6 +512,11 @@ class OracleModel:
             buf = (C.c_byte * (planes * self.m * self.n * 8)).from_address(p)
             out.append(np.frombuffer(buf, dtype=np.float64).reshape(planes, self.m, self.n).copy())
         return out
+
+    def set_exact_inversions(self, on=True):
+        """Test-only: solve every inversion (fast and full) to the round-off
+        floor, so that a step integrates the semi-discrete ODE."""
+        self._L.orc_model_set_exact_inversions(self._h, 1 if on else 0)
 
     def run(self, nsteps, dt):
         """Fluid2d.loop body with fixed dt; returns the F-cycle count per step."""
