@@ -467,8 +467,14 @@ class RefModel:
     """Euler / Boussinesq RK3_SSP step: core/euler.py, core/boussinesq.py,
     core/operators.py, core/timescheme.py:167-180."""
 
-    def __init__(self, kind, nx, ny, Lx, Ly, msk, Kdiff=0.0, diffusion=False, gravity=1.0):
+    def __init__(self, kind, nx, ny, Lx, Ly, msk, Kdiff=0.0, diffusion=False, gravity=1.0,
+                 scheme="RK3_SSP"):
         self.kind, self.nh = kind, 3
+        # core/timescheme.py:30-33, 61-75: the stage of the single-stage
+        # terms and their coefficient
+        self.scheme = scheme
+        self.kforcing = 2 if scheme == "RK3_SSP" else 0
+        self.dtcoef = 1.5 if scheme == "RK3_SSP" else 1.0
         self.dx, self.dy = Lx / nx, Ly / ny
         self.msk = msk
         self.nvar = 5 if kind == "euler" else 6
@@ -512,7 +518,7 @@ class RefModel:
     def dynamics(self, x, dxdt, kstage):
         self.rhs_adv(x, dxdt)
         if self.kind == "euler":
-            if kstage != 2:
+            if kstage != self.kforcing:
                 self.invert(dxdt, False)
             return
         y = dxdt[0].copy()
@@ -520,10 +526,10 @@ class RefModel:
         r_add_torque(self.msk, x[4], self.dx, 3, self.gravity, y)
         r_fillhalo(y, 3)
         dxdt[0][:, :] = y
-        if kstage == 2 and self.diffusion:
+        if kstage == self.kforcing and self.diffusion:
             for k in (0, 4):
                 y = dxdt[k].copy()
-                r_add_diffusion(self.msk, x[k], self.dx, 3, 1.5 * self.Kdiff, y)
+                r_add_diffusion(self.msk, x[k], self.dx, 3, self.dtcoef * self.Kdiff, y)
                 r_fillhalo(y, 3)
                 dxdt[k] = y
         self.invert(dxdt, False)
@@ -531,12 +537,20 @@ class RefModel:
     def step(self, dt, maxspeed):
         self.cst[3] = maxspeed
         x = self.state
-        self.dynamics(x, self.dx0, 0)
-        xs = x + dt * self.dx0
-        self.dynamics(xs, self.dx1, 1)
-        xs = x + (0.25 * dt) * (self.dx0 + self.dx1)
-        self.dynamics(xs, self.dx2, 2)
-        x += (dt / 6.0) * (self.dx0 + self.dx1 + 4 * self.dx2)
+        if self.scheme == "RK3_SSP":  # core/timescheme.py:167-180
+            self.dynamics(x, self.dx0, 0)
+            xs = x + dt * self.dx0
+            self.dynamics(xs, self.dx1, 1)
+            xs = x + (0.25 * dt) * (self.dx0 + self.dx1)
+            self.dynamics(xs, self.dx2, 2)
+            x += (dt / 6.0) * (self.dx0 + self.dx1 + 4 * self.dx2)
+        else:  # core/timescheme.py:152-164
+            self.dynamics(x, self.dx0, 0)
+            xs = x + (dt / 3.0) * self.dx0
+            self.dynamics(xs, self.dx1, 1)
+            xs = x + (0.5 * dt) * self.dx1
+            self.dynamics(xs, self.dx2, 2)
+            x += dt * self.dx2
         if self.kind == "euler":
             self.invert(x, True)
         else:
@@ -547,7 +561,7 @@ class RefModel:
         return d[1]
 
 
-def composed_fixture(kind, nx, ny, Lx, Ly, geometry, seed, nsteps):
+def composed_fixture(kind, nx, ny, Lx, Ly, geometry, seed, nsteps, scheme="RK3_SSP"):
     rng = np.random.default_rng(seed)
     nh = 3
     m, n = ny + 2 * nh, nx + 2 * nh
@@ -558,7 +572,8 @@ def composed_fixture(kind, nx, ny, Lx, Ly, geometry, seed, nsteps):
     if geometry == "closed":
         msk[:, :nh] = 0
         msk[:, -nh:] = 0
-    M = RefModel(kind, nx, ny, Lx, Ly, msk, Kdiff=1e-3, diffusion=(kind == "boussinesq"))
+    M = RefModel(kind, nx, ny, Lx, Ly, msk, Kdiff=1e-3, diffusion=(kind == "boussinesq"),
+                 scheme=scheme)
     w = rng.standard_normal((m, n)) * msk
     if geometry == "perio":
         w -= np.mean(w[nh:-nh, nh:-nh])
@@ -599,6 +614,13 @@ def composed_fixture(kind, nx, ny, Lx, Ly, geometry, seed, nsteps):
 def main():
     if not os.path.isdir(REFDIR):
         sys.exit("oracle/_ref missing: run `make -C oracle ref` first")
+    if sys.argv[1:] == ["rk3"]:
+        np.savez_compressed(os.path.join(HERE, "composed_euler_closed_rk3_32x16.npz"),
+                            **composed_fixture("euler", 32, 16, 2.0, 1.0, "closed", 24, 3, "RK3"))
+        np.savez_compressed(os.path.join(HERE, "composed_bous_xchannel_rk3_32x16.npz"),
+                            **composed_fixture("boussinesq", 32, 16, 2.0, 1.0, "xchannel", 25, 3,
+                                               "RK3"))
+        return
     np.savez_compressed(os.path.join(HERE, "noslip_24x16.npz"), **noslip_fixtures(24, 16, 13))
     if sys.argv[1:] == ["noslip"]:
         return
@@ -610,6 +632,10 @@ def main():
                         **composed_fixture("euler", 32, 16, 2.0, 1.0, "closed", 22, 3))
     np.savez_compressed(os.path.join(HERE, "composed_bous_xchannel_32x16.npz"),
                         **composed_fixture("boussinesq", 32, 16, 2.0, 1.0, "xchannel", 23, 3))
+    np.savez_compressed(os.path.join(HERE, "composed_euler_closed_rk3_32x16.npz"),
+                        **composed_fixture("euler", 32, 16, 2.0, 1.0, "closed", 24, 3, "RK3"))
+    np.savez_compressed(os.path.join(HERE, "composed_bous_xchannel_rk3_32x16.npz"),
+                        **composed_fixture("boussinesq", 32, 16, 2.0, 1.0, "xchannel", 25, 3, "RK3"))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -135,18 +135,21 @@ def test_multigrid_kernels(kf, mname):
 
 
 CFILES = {
-    "composed_euler_perio_16.npz": ("euler", 16, 16, 1.0, 1.0, "perio"),
-    "composed_euler_closed_32x16.npz": ("euler", 32, 16, 2.0, 1.0, "closed"),
-    "composed_bous_xchannel_32x16.npz": ("boussinesq", 32, 16, 2.0, 1.0, "xchannel"),
+    "composed_euler_perio_16.npz": ("euler", 16, 16, 1.0, 1.0, "perio", "RK3_SSP"),
+    "composed_euler_closed_32x16.npz": ("euler", 32, 16, 2.0, 1.0, "closed", "RK3_SSP"),
+    "composed_bous_xchannel_32x16.npz": ("boussinesq", 32, 16, 2.0, 1.0, "xchannel", "RK3_SSP"),
+    # core/timescheme.py:152-164 (dt/3, dt/2, dt; kforcing 0, dtcoef 1)
+    "composed_euler_closed_rk3_32x16.npz": ("euler", 32, 16, 2.0, 1.0, "closed", "RK3"),
+    "composed_bous_xchannel_rk3_32x16.npz": ("boussinesq", 32, 16, 2.0, 1.0, "xchannel", "RK3"),
 }
 
 
 @pytest.mark.parametrize("fname", sorted(CFILES))
 def test_composed(fname):
     g = load(fname)
-    kind, nx, ny, Lx, Ly, geom = CFILES[fname]
+    kind, nx, ny, Lx, Ly, geom, scheme = CFILES[fname]
     M = orc.OracleModel(kind, nx, ny, Lx, Ly, geometry=geom, diffusion=(kind == "boussinesq"),
-                        Kdiff=1e-3, msk=g["msk"])
+                        Kdiff=1e-3, msk=g["msk"], timestepping=scheme)
     gmg = orc.OracleGmg(nx, ny, M.grid["dx"], M.grid["dy"], M.mskp)
     assert gmg.nlevs == len(g["levels"])
     for lev in range(gmg.nlevs):
