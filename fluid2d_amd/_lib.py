@@ -72,6 +72,7 @@ SIGNATURES = {
     "f2d_domain_sum_f64": [_p, _i, _i, _i, _p, _p],
     "f2d_gmg_create": [_i, _i, _i, _d, _d, _d, _d, _p, _pp],
     "f2d_gmg_destroy": [_p],
+    "f2d_gmg_set_helmholtz": [_p, _d],
     "f2d_gmg_nlevels": [_p, _ip],
     "f2d_gmg_level": [_p, _i, _i, _pp, _ip, _ip],
     "f2d_gmg_vcycle": [_p, _i],

@@ -37,10 +37,13 @@ _SIG = {
     "f2d_batch_state_ptr": ([_p, _i, C.POINTER(C.c_void_p)], _i),
     "f2d_batch_set_dt": ([_p, _i, _d], _i),
     "f2d_batch_set_adaptive": ([_p, _i, _d, _d], _i),
+    "f2d_batch_set_fast_solve": ([_p, _i, _d], _i),
     "f2d_batch_set_steps": ([_p, _i, _i], _i),
     "f2d_batch_progress": ([_p, _i, C.POINTER(_i), C.POINTER(C.c_longlong)], _i),
     "f2d_batch_use_graph": ([_p, _i], _i),
     "f2d_batch_set_psi_from_vorticity": ([_p], _i),
+    "f2d_batch_reset_instance": ([_p, _i], _i),
+    "f2d_batch_init_instance": ([_p, _i], _i),
     "f2d_batch_refresh_diagnostics": ([_p], _i),
     "f2d_batch_step": ([_p, _i], _i),
     "f2d_batch_step_async": ([_p, _i], _i),
@@ -200,8 +203,24 @@ class Batch:
     def set_adaptive(self, on=True, cfl=0.5, dtmax=5.0):
         _call("f2d_batch_set_adaptive", self._h, 1 if on else 0, float(cfl), float(dtmax))
 
+    def set_fast_solve(self, maxite, tol=1e-10):
+        """Per-stage inversions by gmg.solve(maxite, tol) -- the alternative
+        core/operators.py:452-455 keeps commented out -- instead of
+        twoVcycle; in mixed precision this is FP64 iterative refinement of
+        FP32 F-cycles.  maxite = 0 restores the reference's twoVcycle."""
+        _call("f2d_batch_set_fast_solve", self._h, int(maxite), float(tol))
+
     def set_psi_from_vorticity(self):
         _call("f2d_batch_set_psi_from_vorticity", self._h)
+
+    def reset_instance(self, i):
+        """Instance i becomes a fresh slot (its carried buffers zeroed, its
+        device record reset); the other instances are untouched."""
+        _call("f2d_batch_reset_instance", self._h, int(i))
+
+    def init_instance(self, i):
+        """set_psi_from_vorticity of instance i alone (enqueued)."""
+        _call("f2d_batch_init_instance", self._h, int(i))
 
     def refresh_diagnostics(self):
         """Recompute every instance's diagnostics (and next umax) from its

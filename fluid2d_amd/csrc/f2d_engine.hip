@@ -602,7 +602,9 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
 // the coarse interior + images.
 // flags: 1 store r on the owned fine cells (+ images) into rout (tk.xc),
 //        2 restrict b itself instead of the residual (F-cycle descent),
-//        4 norm partial of r, 8 norm partial of b (owned interior, msk != 0)
+//        4 norm partial of r, 8 norm partial of b (owned interior, msk != 0),
+//        16 rhs an R plane at tk.xout, 32 the FP64 level-0 operator A0d,
+//        64 rhs = celltocorner of the T plane at tk.xout, 128 x in the T pool
 // ---------------------------------------------------------------------------
 template <typename T, typename R, int TY2V>
 __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__ tasks,
@@ -627,7 +629,10 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
   // operands in RP: the level's A, x and b, or (flag 32) the FP64 level-0
   // operator with an explicit rhs buffer (flag 16) -- the outer residual
   const R* A = RP + ((flags & 32) ? in.A0d : tk.A);
+  // flag 128 (FP64 pool type only): x is a plane of the T pool
   const R* x = RP + tk.xin;
+  if constexpr (std::is_same<T, R>::value)
+    if (flags & 128) x = Q + tk.xin;
   const R* bp = (flags & 64) ? x : RP + ((flags & 16) ? tk.xout : tk.b);
   // flag 64: the rhs is celltocorner of the T plane at tk.xout
   // (core/fortran_operators.f90:44-64, evaluated as k_c2c does)
@@ -1379,22 +1384,27 @@ __global__ void __launch_bounds__(256) k_update(const Task* __restrict__ tasks,
     x[q] = x[q] + (double)y[q];
 }
 
-// psi = (psi - psim) * mskp (core/operators.py:474-481), psim used if perio
+// psi = (psi - psim) * mskp (core/operators.py:474-481), psim used if perio.
+// Task flag 2: the plane at tk.xout instead of the state's psi, masked only
+// (the fast path, core/operators.py:483); flag 1: that plane is in PQ (the
+// T pool of an FP64 batch).
 __global__ void __launch_bounds__(256) k_finalize_psi(const Task* __restrict__ tasks,
                                                       const InstD* __restrict__ I, double* P,
-                                                      const int8_t* __restrict__ B,
+                                                      double* PQ, const int8_t* __restrict__ B,
                                                       const DynD* __restrict__ dyn) {
   const int ti = task_index();
   const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
   const size_t F = (size_t)in.m * in.n;
-  double* psi = P + in.state + F;
+  const bool plane = tk.flags & 2;
+  double* psi = plane ? ((tk.flags & 1) ? PQ : P) + tk.xout : P + in.state + F;
+  const bool demean = !plane && in.perio;
   const int8_t* mp = B + in.mskp;
   const double psim = dyn[tk.inst].psim;
   for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
     double v = psi[q];
-    if (in.perio) v = v - psim;
+    if (demean) v = v - psim;
     psi[q] = v * (double)mp[q];
   }
 }
@@ -1580,15 +1590,17 @@ __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
 struct FoldSpan {
   int first, count;  // partial slots of this instance
 };
+// One block per instance of the window [inst0, inst0 + gridDim.x); spans
+// holds one entry per instance of the window.
 __global__ void __launch_bounds__(256) k_fold(const FoldSpan* __restrict__ spans,
                                               const InstD* __restrict__ I, DynD* dyn,
                                               const double* __restrict__ partial, int what,
                                               int maxite, double tol, int pred, int adapt_dt,
-                                              double cfl, double dtmax) {
+                                              double cfl, double dtmax, int inst0) {
   __shared__ double sh[8][256];
-  const int inst = blockIdx.x, tid = threadIdx.x;
+  const int inst = inst0 + blockIdx.x, tid = threadIdx.x;
   DynD& d = dyn[inst];
-  const FoldSpan sp = spans[inst];
+  const FoldSpan sp = spans[blockIdx.x];
   if (sp.count == 0 || !d.run) return;
   if (what == 3) {
     if (tid != 0 || !I[inst].perio) return;
@@ -1718,10 +1730,13 @@ struct Launch {
   int pred = 0, flags = 0;   // predication, variant flags
   int stage = 0, outk = 0;   // RK stage, tendency slot (k_rhs)
   int mode = 0;              // coarse mode / fold what
-  int span_off = 0;          // fold spans (ninst entries)
+  int span_off = 0;          // fold spans (one entry per instance of the window)
+  int inst0 = 0, ninst = 0;  // fold: instance window [inst0, inst0 + ninst)
   int slot_off = 0;          // npsum slots
   size_t shmem = 0;          // k_coarse dynamic LDS
   int ty = TY;               // fine tile rows of k_smooth (k_resid_restrict: ty / 2 coarse rows)
+  int maxite = -1;           // k_fold stopping test (< 0: the batch's full-solve values)
+  double tol = 0.;
 };
 
 struct HostInst {
@@ -1754,6 +1769,9 @@ struct Batch {
   std::vector<FoldSpan> spans;
   std::vector<int> slots;
   std::vector<Launch> prog_init, prog_step, prog_diag;
+  // per instance: its own initial inversion + diagnostics (a slot re-used
+  // for a new run while the other instances keep stepping)
+  std::vector<std::vector<Launch>> prog_one;
   hipStream_t s = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1762,6 +1780,13 @@ struct Batch {
   double cfl = 0.8, dtmax = 5.0;
   int maxite = 4;
   double tol = 1e-11;
+  // fast (per-stage) inversions: twoVcycle (fast_maxite == 0, the
+  // reference) or gmg.solve(fast_maxite, fast_tol) -- the alternative
+  // core/operators.py:452-455 keeps commented out; in mixed precision its
+  // F-cycles run in FP32 between FP64 residuals and corrections (iterative
+  // refinement)
+  int fast_maxite = 0;
+  double fast_tol = 1e-10;
   size_t max_shmem = 0;
   int min_tiles = 0;           // fine-phase tile choice (Builder::pick_ty)
   int coarse_wave_cells = 256; // coarse levels of at most this many cells run on one wavefront
@@ -1812,23 +1837,29 @@ struct Builder {
     return prog->back();
   }
   // spans of a reduction launch (tasks grouped by instance, contiguous)
+  // instance window of the program being built: folds cover [lo, hi)
+  int lo = 0, hi = 0;
   int spans_of(const std::vector<Task>& ts) {
     const int off = (int)b.spans.size();
-    std::vector<FoldSpan> sp(b.ninst, FoldSpan{0, 0});
+    std::vector<FoldSpan> sp(hi - lo, FoldSpan{0, 0});
     for (int k = 0; k < (int)ts.size(); k++) {
-      FoldSpan& f = sp[ts[k].inst];
+      FoldSpan& f = sp[ts[k].inst - lo];
       if (f.count == 0) f.first = k;
       f.count++;
     }
     b.spans.insert(b.spans.end(), sp.begin(), sp.end());
     return off;
   }
-  void fold(int what, int span_off, int pred) {
+  void fold(int what, int span_off, int pred, int maxite = -1, double tol = 0.) {
     Launch L;
     L.kind = L_FOLD;
     L.mode = what;
     L.span_off = span_off;
+    L.inst0 = lo;
+    L.ninst = hi - lo;
     L.pred = pred;
+    L.maxite = maxite;
+    L.tol = tol;
     prog->push_back(L);
   }
 
@@ -2044,6 +2075,7 @@ struct Builder {
   // fast inversion of tendency slot k (core/operators.py:421-498, 'fast')
   void invert_fast(const std::vector<int>& who, int k) {
     if (who.empty()) return;
+    if (b.fast_maxite > 0) return invert_fast_solve(who, k);
     if (b.prec != 0) return invert_fast_refined(who, k);
     for (int i : who) {
       HostInst& h = b.H[i];
@@ -2098,10 +2130,14 @@ struct Builder {
     ortho(who, [k](HostInst& h) { return h.d.k[k]; }, 2, [k](HostInst& h) { return h.d.kp[k]; });
   }
 
-  // full solve of the state's psi (core/operators.py:470-481)
-  void solve_full(const std::vector<int>& who) {
-    if (who.empty()) return;
-    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0, 1);
+  // Defect-correction solve, core/gmg/hierarchy.py:154-192 (solve): the
+  // FP64 plane psi(h) (pool P, or the T pool of an FP64 batch when psi_q)
+  // against the right-hand side rhs(h): rhs_flag 16 an FP64 plane (pool P),
+  // 64 celltocorner of a T plane formed on the fly.  The residual and the
+  // correction are FP64; the F-cycles run in the pool type T.  The stopping
+  // test (maxite, tol) is evaluated on the device.
+  void solve_defect(const std::vector<int>& who, Off psi, bool psi_q, Off rhs, int rhs_flag,
+                    int maxite, double tol) {
     for (int i : who) {
       HostInst& h = b.H[i];
       h.buf[0][0] = h.d.x0[0];
@@ -2112,17 +2148,16 @@ struct Builder {
       std::vector<Task> rs;
       for (int i : who) {
         HostInst& h = b.H[i];
-        const size_t F = (size_t)h.d.m * h.d.n;
-        rr_tasks(i, 0, h.d.state + F, h.d.work, h.d.L[0].b, rs);
+        rr_tasks(i, 0, psi(h), rhs(h), h.d.L[0].b, rs);
       }
       const int sp = spans_of(rs);
       Launch& L = add(L_RR, rs);
-      L.flags = 1 | 4 | 8 | 16 | 32;  // FP64 outer residual of the refinement
+      L.flags = 1 | 4 | 8 | 32 | rhs_flag | (psi_q ? 128 : 0);  // FP64 outer residual
       L.pred = pred;
-      fold(what, sp, pred);
+      fold(what, sp, pred, maxite, tol);
     };
     resid_norm(0, 0);
-    for (int it = 0; it < b.maxite; it++) {
+    for (int it = 0; it < maxite; it++) {
       // F-cycle: b restricted down the fine levels (b0 -> b1 came with the
       // residual), coarse tail, then V-cycles from each fine level upward
       int Dmax = 0;
@@ -2163,14 +2198,54 @@ struct Builder {
         const size_t F = (size_t)h.d.m * h.d.n;
         for (size_t q = 0; q < F; q += FLAT) {
           Task t = mk_task(i, 0, (int)q, (int)std::min<size_t>(FLAT, F - q));
-          t.xout = h.d.state + F;
+          t.xout = psi(h);
           t.xin = h.buf[0][h.cur[0]];
           up.push_back(t);
         }
       }
-      add(L_UPDATE, up).pred = 1;
+      Launch& U = add(L_UPDATE, up);
+      U.pred = 1;
+      U.flags = psi_q ? 1 : 0;
       resid_norm(1, 1);
     }
+  }
+
+  // Fast inversion of tendency slot k by gmg.solve (fast_maxite, fast_tol)
+  // instead of twoVcycle: core/operators.py:452-455 (commented out in the
+  // reference), then psi *= mskp and (u, v) (:483-496).  The psi tendency
+  // is FP64: kp[k] (pool P) in mixed precision, the T pool's own plane in
+  // an FP64 batch.
+  void invert_fast_solve(const std::vector<int>& who, int k) {
+    const bool fp64 = b.prec == 0;
+    Off psi = [k, fp64](HostInst& h) {
+      return fp64 ? h.d.k[k] + (long long)h.d.m * h.d.n : h.d.kp[k];
+    };
+    solve_defect(who, psi, fp64, [k](HostInst& h) { return h.d.k[k]; }, 64, b.fast_maxite,
+                 b.fast_tol);
+    std::vector<Task> fin;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      const long long F = (long long)h.d.m * h.d.n;
+      for (long long q = 0; q < F; q += FLAT) {
+        Task t = mk_task(i, 1, (int)q, (int)std::min<long long>(FLAT, F - q));
+        t.xout = psi(h);
+        t.flags = 2 | (fp64 ? 1 : 0);
+        fin.push_back(t);
+      }
+    }
+    add(L_FINPSI, fin);
+    if (fp64)
+      ortho(who, [k](HostInst& h) { return h.d.k[k]; }, 0);
+    else
+      ortho(who, [k](HostInst& h) { return h.d.k[k]; }, 2, [k](HostInst& h) { return h.d.kp[k]; });
+  }
+
+  // full solve of the state's psi (core/operators.py:470-481)
+  void solve_full(const std::vector<int>& who) {
+    if (who.empty()) return;
+    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0, 1);
+    solve_defect(who, [](HostInst& h) { return h.d.state + (long long)h.d.m * h.d.n; }, false,
+                 [](HostInst& h) { return h.d.work; }, 16, b.maxite, b.tol);
     // zero mean (perio) and mask, then (u, v)
     std::vector<Task> ns;
     for (int i : who) {
@@ -2184,12 +2259,12 @@ struct Builder {
     }
     {
       const int soff = (int)b.slots.size();
-      std::vector<FoldSpan> sp(b.ninst, FoldSpan{0, 0});
+      std::vector<FoldSpan> sp(hi - lo, FoldSpan{0, 0});
       int slot = 0;
       for (const Task& t : ns) {
         b.slots.push_back(slot);
-        if (sp[t.inst].count == 0) sp[t.inst].first = slot;
-        sp[t.inst].count += t.x0;
+        if (sp[t.inst - lo].count == 0) sp[t.inst - lo].first = slot;
+        sp[t.inst - lo].count += t.x0;
         slot += t.x0;
       }
       const int spoff = (int)b.spans.size();
@@ -2204,7 +2279,7 @@ struct Builder {
 
   void diag() {
     std::vector<Task> ts;
-    for (int i = 0; i < b.ninst; i++) {
+    for (int i = lo; i < hi; i++) {
       HostInst& h = b.H[i];
       tiles(ts, mk_task(i, 0, 0, 0), NH, h.d.m - NH, NH, h.d.n - NH);
     }
@@ -2295,6 +2370,8 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
   const dim3 B2(64, 4), B1(256);
   T* Q = (T*)b.Q;
   double* P = b.P;
+  // the T pool as FP64 planes (FP64 batches only; null in mixed precision)
+  double* PQ = std::is_same<T, double>::value ? (double*)b.Q : nullptr;
   for (const Launch& L : prog) {
     const Task* TK = b.dT + L.toff;
     const unsigned g = (unsigned)L.ntask;
@@ -2364,10 +2441,12 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
         F2D_LAUNCH(k_combine<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD);
         break;
       case L_UPDATE:
-        F2D_LAUNCH(k_update<T>, dim3(g), B1, 0, b.s, TK, b.dI, P, Q, b.dD, L.pred);
+        // flag 1: the corrected plane is in the T pool (FP64 batches only)
+        F2D_LAUNCH(k_update<T>, dim3(g), B1, 0, b.s, TK, b.dI, (L.flags & 1) ? PQ : P, Q, b.dD,
+                   L.pred);
         break;
       case L_FINPSI:
-        F2D_LAUNCH(k_finalize_psi, dim3(g), B1, 0, b.s, TK, b.dI, P, b.B, b.dD);
+        F2D_LAUNCH(k_finalize_psi, dim3(g), B1, 0, b.s, TK, b.dI, P, PQ, b.B, b.dD);
         break;
       case L_NPSUM:
         F2D_LAUNCH(k_npsum, dim3(g), B1, 0, b.s, TK, b.dI, P, b.partial,
@@ -2380,8 +2459,10 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
         F2D_LAUNCH(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
       case L_FOLD:
-        F2D_LAUNCH(k_fold, dim3(b.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
-                           b.partial, L.mode, b.maxite, b.tol, L.pred, b.adapt, b.cfl, b.dtmax);
+        F2D_LAUNCH(k_fold, dim3(L.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
+                           b.partial, L.mode, L.maxite < 0 ? b.maxite : L.maxite,
+                           L.maxite < 0 ? b.tol : L.tol, L.pred, b.adapt, b.cfl, b.dtmax,
+                           L.inst0);
         break;
     }
     if (timed) ECHK(hipEventRecord(prof->ev.back().second, b.s));
@@ -2488,6 +2569,8 @@ static int build_programs(Batch& b) {
   b.prog_init.clear();
   b.prog_step.clear();
   Builder bd(b);
+  bd.lo = 0;
+  bd.hi = b.ninst;
   bd.prog = &b.prog_init;
   // initial inversion of every instance's state + diagnostics.  Euler and
   // Boussinesq both start from set_psi_from_vorticity (a full solve).
@@ -2504,8 +2587,21 @@ static int build_programs(Batch& b) {
     b.prog_step.push_back(g);
   }
   bd.rk_step();
+  // one instance's initial inversion and diagnostics, per instance
+  b.prog_one.assign(b.ninst, {});
+  for (int i = 0; i < b.ninst; i++) {
+    bd.lo = i;
+    bd.hi = i + 1;
+    bd.prog = &b.prog_one[i];
+    bd.solve_full({i});
+    bd.diag();
+  }
+  bd.lo = 0;
+  bd.hi = b.ninst;
   size_t np = 0;
-  for (auto* pr : {&b.prog_init, &b.prog_step, &b.prog_diag})
+  std::vector<const std::vector<Launch>*> progs = {&b.prog_init, &b.prog_step, &b.prog_diag};
+  for (const auto& po : b.prog_one) progs.push_back(&po);
+  for (auto* pr : progs)
     for (const Launch& L : *pr)
       if (L.kind == L_RR || L.kind == L_DIAG) np = std::max(np, (size_t)L.ntask * 8);
   np = std::max(np, (size_t)b.slots.size() * 256 + 256);
@@ -2999,9 +3095,66 @@ int f2d_batch_set_adaptive(f2d_batch* b, int on, double cfl, double dtmax) {
   return F2D_OK;
 }
 
+int f2d_batch_set_fast_solve(f2d_batch* b, int maxite, double tol) {
+  if (maxite < 0 || maxite > 64) return efail(F2D_E_ARG, "maxite must be in 0..64 (got %d)", maxite);
+  if (maxite > 0 && !(tol >= 0.)) return efail(F2D_E_ARG, "tol must be >= 0");
+  // the step's launch program changes: rebuild it (and recapture the graph)
+  ECHK(hipStreamSynchronize(b->s));
+  drop_graph(*b);
+  b->fast_maxite = maxite;
+  b->fast_tol = tol;
+  return build_programs(*b);
+}
+
 int f2d_batch_use_graph(f2d_batch* b, int on) {
   b->use_graph = on != 0;
   return F2D_OK;
+}
+
+// A slot of the batch re-used for a new run: every per-instance buffer that
+// carries values from one step (or solve) to the next is zeroed -- the state,
+// the full-solve rhs, the three tendencies (their psi planes are the next
+// inversions' first guesses), the FP64 psi tendencies of mixed precision, the
+// level-0 correction pair and every level's rhs and iterates -- and the
+// device record is reset, as at creation.  The operators, masks and
+// coordinates stay.  The other instances are untouched.
+int f2d_batch_reset_instance(f2d_batch* b, int inst) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  const InstD& d = b->H[inst].d;
+  const size_t F = (size_t)d.m * d.n;
+  const size_t tsz = b->prec == 0 ? sizeof(double) : sizeof(float);
+  char* Qb = (char*)b->Q;
+  ECHK(hipStreamSynchronize(b->s));
+  ECHK(hipMemsetAsync(b->P + d.state, 0, (size_t)d.nvar * F * 8, b->s));
+  ECHK(hipMemsetAsync(b->P + d.work, 0, F * 8, b->s));
+  for (int k = 0; k < 3; k++) {
+    ECHK(hipMemsetAsync(Qb + d.k[k] * tsz, 0, (size_t)d.nvar * F * tsz, b->s));
+    if (d.kp[k] >= 0) ECHK(hipMemsetAsync(b->P + d.kp[k], 0, F * 8, b->s));
+  }
+  ECHK(hipMemsetAsync(Qb + d.x0[0] * tsz, 0, F * tsz, b->s));
+  ECHK(hipMemsetAsync(Qb + d.x0[1] * tsz, 0, F * tsz, b->s));
+  ECHK(hipMemsetAsync(Qb + b->H[inst].x0alt * tsz, 0, F * tsz, b->s));
+  for (int l = 0; l < d.nlev; l++) {
+    const LevD& L = d.L[l];
+    const size_t S = (size_t)L.m * L.n;
+    ECHK(hipMemsetAsync(Qb + L.b * tsz, 0, S * tsz, b->s));
+    if (l > 0)
+      for (int q = 0; q < 2; q++) ECHK(hipMemsetAsync(Qb + L.x[q] * tsz, 0, S * tsz, b->s));
+  }
+  ECHK(hipStreamSynchronize(b->s));
+  memset(&b->hD[inst], 0, sizeof(DynD));
+  b->hD[inst].run = 1;
+  b->hD[inst].steps_left = -1;
+  ECHK(hipMemcpyAsync(&b->dD[inst], &b->hD[inst], sizeof(DynD), hipMemcpyHostToDevice, b->s));
+  ECHK(hipStreamSynchronize(b->s));
+  return F2D_OK;
+}
+
+// set_psi_from_vorticity of one instance (its solve and diagnostics only),
+// enqueued on the batch's stream
+int f2d_batch_init_instance(f2d_batch* b, int inst) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  return run_prog(*b, b->prog_one[inst]);
 }
 
 int f2d_batch_set_psi_from_vorticity(f2d_batch* b) {

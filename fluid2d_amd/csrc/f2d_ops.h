@@ -84,6 +84,8 @@ void launch_coarsen(const double* Af, const int8_t* msk1, const int8_t* msk2, do
 void launch_afinest(const int8_t* msk, double* A, const St9& st, double coef, int m, int n,
                     hipStream_t s);
 void launch_corner_mask(const int8_t* msk, int8_t* mskp, int m, int n, hipStream_t s);
+// A5 -= shift wherever A5 != 0 (the Helmholtz operator of core/gmg/hierarchy.py:80-84)
+void launch_helmholtz_shift(double* A5, size_t cnt, double shift, hipStream_t s);
 void launch_coarse_mask(const int8_t* msk1, int8_t* msk2, int m2, int n2, int n1, hipStream_t s);
 
 // reductions: partial must hold kRedMaxBlocks*8 doubles; out on the device

@@ -1019,6 +1019,14 @@ void launch_afinest(const int8_t* msk, double* A, const St9& st, double coef, in
                     hipStream_t s) {
   hipLaunchKernelGGL(k_afinest, grid2(n - 2 * NH, m - 2 * NH), B2, 0, s, msk, A, st, coef, m, n);
 }
+// core/gmg/hierarchy.py:80-84: A5 -= shift wherever A5 != 0 (whole plane)
+__global__ void k_helmholtz_shift(double* __restrict__ A5, size_t cnt, double shift) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < cnt && A5[q] != 0.) A5[q] -= shift;
+}
+void launch_helmholtz_shift(double* A5, size_t cnt, double shift, hipStream_t s) {
+  hipLaunchKernelGGL(k_helmholtz_shift, grid1(cnt), B1, 0, s, A5, cnt, shift);
+}
 void launch_corner_mask(const int8_t* msk, int8_t* mskp, int m, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_corner_mask, grid2(n, m), B2, 0, s, msk, mskp, m, n);
 }

@@ -894,6 +894,15 @@ int f2d_gmg_create(int nx, int ny, int nh, double dx, double dy, double hydroeps
   return gmg_build(nx, ny, dx, dy, hydroepsilon, omega, mskp_host, out);
 }
 int f2d_gmg_destroy(f2d_gmg* g) { return gmg_free(g); }
+int f2d_gmg_set_helmholtz(f2d_gmg* g, double Rd) {
+  if (!(Rd > 0.) || !isfinite(Rd)) return fail(F2D_E_ARG, "Rd must be positive and finite");
+  const double shift = 1. / (Rd * Rd);
+  for (int l = 0; l < g->nlev; l++)
+    launch_helmholtz_shift(g->L[l].A + 4 * g->L[l].size(), g->L[l].size(), shift, g->s);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(g->s));
+  return F2D_OK;
+}
 int f2d_gmg_nlevels(const f2d_gmg* g, int* nlev) {
   *nlev = g->nlev;
   return F2D_OK;

@@ -15,7 +15,9 @@ from .device import DeviceArray, ptr
 
 class Gmg:
     """param: the dict core/operators.py:35-41 builds ('n', 'm', 'nh', 'dx',
-    'dy', 'omega', 'hydroepsilon', ...); mskp: level-0 corner mask (host)."""
+    'dy', 'omega', 'hydroepsilon', ...; with 'qgoperator' and 'Rd' the
+    operator is the Helmholtz A - 1/Rd^2, core/operators.py:74-77);
+    mskp: level-0 corner mask (host)."""
 
     def __init__(self, param, mskp, handle=None):
         self._own = handle is None
@@ -28,6 +30,9 @@ class Gmg:
                       C.byref(h))
             handle = h.value
         self._h = handle
+        if self._own and "qgoperator" in param:
+            # core/gmg/hierarchy.py:80-84: the Helmholtz operator A - 1/Rd^2
+            _lib.call("f2d_gmg_set_helmholtz", self._h, float(param["Rd"]))
         nl = C.c_int()
         _lib.call("f2d_gmg_nlevels", self._h, C.byref(nl))
         self.nlevs = nl.value

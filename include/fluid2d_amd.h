@@ -161,6 +161,11 @@ typedef struct f2d_gmg f2d_gmg;
 int f2d_gmg_create(int nx, int ny, int nh, double dx, double dy, double hydroepsilon,
                    double omega, const int8_t *mskp_host, f2d_gmg **out);
 int f2d_gmg_destroy(f2d_gmg *g);
+/* core/gmg/hierarchy.py:80-84 (param 'qgoperator'): turn the Laplacian into
+ * the Helmholtz operator A - 1/Rd^2: on every level, the centre coefficient
+ * A5 becomes A5 - 1/Rd^2 wherever it is nonzero (halo included).  Call once,
+ * after f2d_gmg_create and before any cycle. */
+int f2d_gmg_set_helmholtz(f2d_gmg *g, double Rd);
 int f2d_gmg_nlevels(const f2d_gmg *g, int *nlev);
 /* which: 0 msk (int8), 1 A (5 planes), 2 x, 3 b, 4 r */
 int f2d_gmg_level(const f2d_gmg *g, int lev, int which, void **dev_ptr, int *m, int *n);
@@ -299,9 +304,29 @@ int f2d_batch_set_steps(f2d_batch *b, int inst, int nsteps);
 int f2d_batch_progress(f2d_batch *b, int inst, int *steps_left, long long *steps_done);
 /* adaptive dt = cfl*min(dx,dy)/maxspeed capped at dtmax (core/fluid2d.py:376-397) */
 int f2d_batch_set_adaptive(f2d_batch *b, int on, double cfl, double dtmax);
+/* Fast (per-stage) inversions by gmg.solve(maxite, tol) instead of
+ * twoVcycle: the alternative core/operators.py:452-455 keeps commented out
+ * (ite, res = self.gmg.solve(psi, self.work, {'maxite': .., 'tol': ..})),
+ * then psi *= mskp and (u, v) as on the fast path.  In mixed precision the
+ * F-cycles run in FP32 between FP64 residuals and FP64 corrections
+ * (iterative refinement to tol).  maxite = 0 restores twoVcycle (the
+ * reference).  Rebuilds the step program; call between steps. */
+int f2d_batch_set_fast_solve(f2d_batch *b, int maxite, double tol);
 int f2d_batch_use_graph(f2d_batch *b, int on);
 /* set_psi_from_vorticity of every instance, then diagnostics (maxspeed) */
 int f2d_batch_set_psi_from_vorticity(f2d_batch *b);
+/* Continuous batching (no reference counterpart: the reference runs one
+ * model per process).  reset_instance: instance `inst` becomes a fresh slot
+ * -- every buffer that carries values between steps is zeroed and its
+ * device record reset as at creation; operators, masks and the other
+ * instances are untouched (synchronises the batch's stream).
+ * init_instance: set_psi_from_vorticity (core/euler.py / boussinesq.py,
+ * the full solve + diagnostics) of that instance alone, enqueued on the
+ * batch's stream.  With set_field, set_dt and set_steps in between, a
+ * request admitted into a running batch computes exactly what it computes
+ * in a batch of its own. */
+int f2d_batch_reset_instance(f2d_batch *b, int inst);
+int f2d_batch_init_instance(f2d_batch *b, int inst);
 /* diagnostics (and the next step's umax, core/fluid2d.py:397) of every
  * instance's current state, e.g. after a host write of the state
  * (Euler.diagnostics, core/euler.py:185-221) */

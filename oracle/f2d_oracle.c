@@ -797,6 +797,21 @@ orc_gmg *orc_gmg_create(int nx, int ny, int nh, double dx, double dy,
   return g;
 }
 
+/* core/gmg/hierarchy.py:80-84: the Laplacian turned into a Helmholtz
+ * operator (qgoperator): on every level, after the truncation to the five
+ * lower planes, the centre coefficient becomes A5 - 1/Rd^2 wherever A5 is
+ * nonzero, halo included. */
+void orc_gmg_set_helmholtz(orc_gmg *g, double Rd) {
+  const double shift = 1. / (Rd * Rd);
+  for (int l = 0; l < g->nlev; l++) {
+    orc_level *L = &g->lev[l];
+    const size_t sz = (size_t)L->m * L->n;
+    double *A5 = L->A + 4 * sz;
+    for (size_t q = 0; q < sz; q++)
+      if (A5[q] != 0.) A5[q] -= shift;
+  }
+}
+
 int orc_gmg_level_shape(orc_gmg *g, int l, int *m, int *n) {
   if (l < 0 || l >= g->nlev) return 1;
   *m = g->lev[l].m; *n = g->lev[l].n;
@@ -953,6 +968,11 @@ typedef struct {
   double bcarea, x2bc, y2bc;
   /* test-only: every inversion solved to convergence (temporal-order checks) */
   int exact_inv;
+  /* fast (per-stage) inversions by gmg.solve(maxite, tol) instead of
+   * twoVcycle when fast_maxite > 0: the alternative core/operators.py:452-455
+   * keeps commented out (no zero-mean step, as on the fast path) */
+  int fast_maxite;
+  double fast_tol;
 } orc_model;
 
 /* field indices */
@@ -1026,6 +1046,10 @@ static int invert_vorticity(orc_model *M, double *x, int full) {
       const double psim = orc_np_sum_interior(psi, M->nh, M->m, M->n) * 1. / M->area;
       for (size_t q = 0; q < sz; q++) psi[q] -= psim;
     }
+  } else if (!full && M->fast_maxite > 0) {
+    int nite; double res;
+    st = orc_gmg_solve(M->gmg, psi, M->work, M->fast_maxite, M->fast_tol, &nite, &res);
+    M->last_nite = nite; M->last_res = res; M->last_status = st;
   } else if (!full) {
     orc_gmg_twovcycle(M->gmg, psi, M->work);
   } else {
@@ -1322,6 +1346,10 @@ orc_gmg *orc_model_gmg(orc_model *M) { return M->gmg; }
 void orc_model_set_umax(orc_model *M, double umax) { M->cst[3] = umax; }
 /* test-only knob: solve every inversion to convergence */
 void orc_model_set_exact_inversions(orc_model *M, int on) { M->exact_inv = on; }
+void orc_model_set_fast_solve(orc_model *M, int maxite, double tol) {
+  M->fast_maxite = maxite > 0 ? maxite : 0;
+  M->fast_tol = tol;
+}
 void orc_model_last_solve(orc_model *M, int *nite, double *res, int *status) {
   *nite = M->last_nite; *res = M->last_res; *status = M->last_status;
 }

@@ -26,14 +26,19 @@ Functions and the reference routine each restates:
 ``coarsenmatrix``      core/gmg/fortran_multigrid.f90:706-811
 ``kemaxu`` etc.        core/fortran_diag.f90:3-30, 63-120, 155-195
 ``np_sum_interior``    core/grid.py:132-140 (NumPy's buffered pairwise sum)
-``OracleGmg``          core/gmg/hierarchy.py:21-218, core/gmg/level.py:24-496
+``OracleGmg``          core/gmg/hierarchy.py:21-218 (Helmholtz shift :80-84),
+                       core/gmg/level.py:24-496
 ``OracleModel``        core/euler.py, core/boussinesq.py, core/timescheme.py,
                        core/operators.py:12-155, 214-498, core/fluid2d.py:235-268
 ``make_grid``          core/grid.py:12-130 (one subdomain)
 =====================  =====================================================
 
-Parity status of each function (how it is pinned) is listed in the module
-docstring of ``tests/test_oracle_pins.py``.
+How each function is pinned is stated in the module docstrings of the
+tests that check it: ``tests/test_oracle_golden.py`` (bit for bit against
+the reference Fortran built from source), ``tests/test_oracle_closed_forms.py``,
+``tests/test_oracle_cycles.py``, ``tests/test_oracle_timescheme.py``,
+``tests/test_oracle_fluxes.py``, ``tests/test_oracle_noslip.py`` and
+``tests/test_oracle_helmholtz.py``.
 """
 import ctypes as C
 import os
@@ -110,6 +115,7 @@ def lib():
         "orc_gmg_nlevels": (_i, [_i, _i]),
         "orc_gmg_create": (_vp, [_i, _i, _i, _d, _d, _d, _d, _bp]),
         "orc_gmg_destroy": (None, [_vp]),
+        "orc_gmg_set_helmholtz": (None, [_vp, _d]),
         "orc_gmg_level_shape": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i)]),
         "orc_gmg_level_ptr": (_vp, [_vp, _i, _i]),
         "orc_gmg_vcycle": (None, [_vp, _i]),
@@ -134,6 +140,7 @@ def lib():
         "orc_model_set_noslip": (_i, [_vp, _bp, _bp, _dp, _dp, _d, _d, _d, _i]),
         "orc_model_set_islands": (_i, [_vp, _dp, _dp]),
         "orc_model_set_exact_inversions": (None, [_vp, _i]),
+        "orc_model_set_fast_solve": (None, [_vp, _i, _d]),
         "orc_rk_linear": (_i, [_i, _i, _dp, _dp, _d]),
     }
     for name, (res, args) in sig.items():
@@ -321,12 +328,15 @@ def make_grid(nx, ny, Lx=1.0, Ly=1.0, geometry="perio", nh=3):
 class OracleGmg:
     """core/gmg/hierarchy.py Gmg with one subdomain (npre=npost=1, ndeepest=16)."""
 
-    def __init__(self, nx, ny, dx, dy, mskp, nh=3, omega=8.0 / 9.0, hydroepsilon=1.0):
+    def __init__(self, nx, ny, dx, dy, mskp, nh=3, omega=8.0 / 9.0, hydroepsilon=1.0, Rd=None):
         self._h = lib().orc_gmg_create(nx, ny, nh, dx, dy, hydroepsilon, omega,
                                        _c(np.ascontiguousarray(mskp, dtype=np.int8), np.int8))
         if not self._h:
             raise ValueError("bad grid size")
         self.nlevs = lib().orc_gmg_nlevels(nx, ny)
+        if Rd is not None:
+            # core/gmg/hierarchy.py:80-84 (the qgoperator Helmholtz shift)
+            lib().orc_gmg_set_helmholtz(self._h, float(Rd))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -517,6 +527,12 @@ class OracleModel:
         """Test-only: solve every inversion (fast and full) to the round-off
         floor, so that a step integrates the semi-discrete ODE."""
         self._L.orc_model_set_exact_inversions(self._h, 1 if on else 0)
+
+    def set_fast_solve(self, maxite, tol):
+        """Fast (per-stage) inversions by gmg.solve(maxite, tol) -- the
+        alternative core/operators.py:452-455 keeps commented out -- instead
+        of twoVcycle; maxite <= 0 restores twoVcycle."""
+        self._L.orc_model_set_fast_solve(self._h, int(maxite), float(tol))
 
     def run(self, nsteps, dt):
         """Fluid2d.loop body with fixed dt; returns the F-cycle count per step."""
