@@ -41,7 +41,7 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 HEADLINE = "euler-closed-1024"
 BOUSSINESQ = "boussinesq-rt-2048x1024"
 KERNEL_NAMES = {"smooth": "k_smooth", "resid_restrict": "k_resid_restrict", "rhs": "k_rhs",
-                "coarse": "k_coarse"}
+                "coarse": "k_coarse", "persist": "k_persist"}
 
 
 def parse():
@@ -167,7 +167,7 @@ def roofline(b, nsteps=2):
     The one-workgroup coarse kernel is latency-bound, not HBM-bound: its
     share is reported beside the others."""
     shares = {}
-    for kind in ("smooth", "resid_restrict", "rhs", "coarse"):
+    for kind in ("smooth", "resid_restrict", "rhs", "coarse", "persist"):
         pr = b.profile(kind, -1, nsteps)
         shares[kind] = pr["avg_ms"] * pr["launches"] / nsteps
     dom = max(("smooth", "resid_restrict", "rhs"), key=shares.get)
@@ -206,11 +206,22 @@ def cpu_baseline(name, seconds, variant="serial"):
     el = time.perf_counter() - t0
     orc.use("serial")
     return {"value": g.nx * g.ny * steps / el, "unit": "cell-updates/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu": cpu_model(), "nproc": os.cpu_count(),
             "sample": "%s: %d RK3_SSP steps (after 1 untimed) of the C oracle, FP64, %s, "
-                      "gcc -O2 -ffp-contract=off, %.1f s" % (
+                      "gcc -O3 -ffp-contract=off, %.1f s" % (
                           name, steps, "1 thread" if variant == "serial" else
                           "%d OpenMP threads over grid rows" % cores, el)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def finest_tile_height(nx, ny, min_tiles):
@@ -342,6 +353,7 @@ def main():
         log("plain FP64 %s: warmup %d, %d steps" % (name, a.warmup, a.steps))
         b, g, dt = config_batch(name, "fp64", tune=not a.no_tune)
         out["config"]["tuning"] = b.tuning()
+        out["config"]["launches_per_step"] = b.info(0)["launches_per_step"]
         b.step(a.warmup)
         el = timed(b, a.steps)
         cells = g.nx * g.ny
@@ -517,6 +529,14 @@ def main():
         out["cpu_baseline"] = cpu_baseline(name, a.cpu_seconds, "omp")
         log("cpu baseline, 1 thread (~%.0f s)" % a.cpu_seconds)
         out["cpu_baseline_1thread"] = cpu_baseline(name, a.cpu_seconds, "serial")
+        # the other two single-grid configurations, shorter samples
+        other = {}
+        for cname in ("euler-perio-128", BOUSSINESQ):
+            other[cname] = {}
+            for variant in ("omp", "serial"):
+                log("cpu baseline %s, %s (~%.0f s)" % (cname, variant, a.cpu_seconds / 3))
+                other[cname][variant] = cpu_baseline(cname, a.cpu_seconds / 3, variant)
+        out["cpu_baseline_other_configs"] = other
     print(json.dumps(out), flush=True)
 
 

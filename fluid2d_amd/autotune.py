@@ -11,6 +11,14 @@ Two knobs shape the launches of a batch (``f2d_batch_tuning``):
 * ``coarse_wave_cells``: inside the coarse kernel, levels up to this size
   run on a single wavefront (no workgroup barriers) instead of the whole
   workgroup.
+* ``persist``: runs of consecutive smoother and residual/restriction phases
+  below the finest level run as one persistent launch with device-wide
+  barriers (at most this many workgroups per compute unit), or one launch
+  per phase (0).  Not searched by default: on MI355X a device-wide barrier
+  costs more than a dependent launch of a replayed graph (about 1.7 us)
+  as soon as more than a few workgroups take part (about 4 us at 64 and
+  12 us at 256 workgroups), so one launch per phase wins.  ``tune(...,
+  persist=(1, 2))`` tries it on the best of the other three.
 
 :func:`tune` builds a scratch batch with the same instance specs (so the
 caller's state is never touched), starts it from seeded turbulence,
@@ -30,6 +38,7 @@ from .param import Param
 BUDGETS = (0, 98304, 49152, 24576)
 MIN_TILES = (0, 1024, 4096, 1 << 24)
 WAVE_CELLS = (256, 0)
+PERSIST = ()
 
 
 def _pg(specs):
@@ -48,7 +57,7 @@ def signature(pg, precision):
     for p, g in pg:
         k = "%s:%dx%d:%s" % (p.modelname, g.nx, g.ny, "perio" if g.geometry == "perio" else "wall")
         keys[k] = keys.get(k, 0) + 1
-    return precision + "|" + ",".join("%s*%d" % (k, keys[k]) for k in sorted(keys))
+    return "v2|" + precision + "|" + ",".join("%s*%d" % (k, keys[k]) for k in sorted(keys))
 
 
 def _cache_path():
@@ -94,9 +103,12 @@ def candidates():
             for cb in BUDGETS for mt in MIN_TILES for wc in WAVE_CELLS]
 
 
-def tune(specs, precision="fp64", steps=3, cands=None, use_cache=True, verbose=False):
+def tune(specs, precision="fp64", steps=3, cands=None, use_cache=True, verbose=False,
+         persist=PERSIST):
     """Best tuning for a batch of ``specs``: returns (tuning dict, table of
-    (tuning, ms per step)) with the default configuration first."""
+    (tuning, ms per step)) with the default configuration first.  The
+    persistent multigrid launches (``persist`` workgroups per compute unit)
+    are tried on the best configuration found without them."""
     pg = _pg(specs)
     sig = signature(pg, precision)
     cache = _load_cache() if use_cache else {}
@@ -106,20 +118,27 @@ def tune(specs, precision="fp64", steps=3, cands=None, use_cache=True, verbose=F
     b = Batch(pg, precision=precision)
     _init(b)
     table = []
-    for c in cands:
+
+    def trial(c):
         try:
             b.retune(c)
+            b.step(1)  # capture + warm
+            t0 = time.perf_counter()
+            b.step(steps)
         except Exception as e:  # a budget too small for the coarsest level
             if verbose:
                 print("autotune: skip %s (%s)" % (c, e))
-            continue
-        b.step(1)  # capture + warm
-        t0 = time.perf_counter()
-        b.step(steps)
+            return
         ms = (time.perf_counter() - t0) / steps * 1e3
         table.append((c, ms))
         if verbose:
             print("autotune: %s %.3f ms/step" % (c, ms), flush=True)
+
+    for c in cands:
+        trial(dict(c, persist=0))
+    best = min(table, key=lambda r: r[1])[0]
+    for p in persist:
+        trial(dict(best, persist=p))
     del b
     best = min(table, key=lambda r: r[1])[0]
     if use_cache:

@@ -81,14 +81,15 @@ def _call(name, *args):
 class Tuning(C.Structure):
     """f2d_batch_tuning: coarse-kernel LDS budget (bytes, 0 = device
     maximum), the minimum workgroup count of a fine phase (0 = always
-    16-row tiles) and the largest coarse level (cells) run on one wavefront
-    (-1 = default 256)."""
+    16-row tiles), the largest coarse level (cells) run on one wavefront
+    (-1 = default 256) and the persistent multigrid launches (workgroups per
+    compute unit, 0 = one launch per phase)."""
     _fields_ = [("coarse_lds_budget", C.c_int), ("min_tiles", C.c_int),
-                ("coarse_wave_cells", C.c_int)]
+                ("coarse_wave_cells", C.c_int), ("persist", C.c_int)]
 
     def as_dict(self):
         return {"coarse_lds_budget": self.coarse_lds_budget, "min_tiles": self.min_tiles,
-                "coarse_wave_cells": self.coarse_wave_cells}
+                "coarse_wave_cells": self.coarse_wave_cells, "persist": self.persist}
 
 
 def _tuning(t):
@@ -96,7 +97,7 @@ def _tuning(t):
         return None
     if isinstance(t, dict):
         t = Tuning(int(t.get("coarse_lds_budget", 0)), int(t.get("min_tiles", 0)),
-                   int(t.get("coarse_wave_cells", -1)))
+                   int(t.get("coarse_wave_cells", -1)), int(t.get("persist", 0)))
     return C.byref(t)
 
 
@@ -242,7 +243,7 @@ class Batch:
         _call("f2d_batch_query", self._h, C.byref(d))
         return bool(d.value)
 
-    KINDS = {"smooth": 0, "resid_restrict": 1, "rhs": 2, "coarse": 3}
+    KINDS = {"smooth": 0, "resid_restrict": 1, "rhs": 2, "coarse": 3, "persist": 4}
 
     def profile(self, kind, level=-1, nsteps=2):
         """Time every launch of ``kind`` at ``level`` over ``nsteps`` eager

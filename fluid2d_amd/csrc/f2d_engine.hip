@@ -368,11 +368,14 @@ __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
 // MODE 0: x = xin; 1: x = 0; 2: x = xin + I(xc); 3: x = I(xc).
 // FINAL: each stored copy multiplied by mskp (psi * mskp).
 // ---------------------------------------------------------------------------
+// One tile (task ti of `tasks`) of a (64, 4) workgroup; the kernel k_smooth
+// below runs one tile per workgroup, the persistent kernel k_persist any
+// number of them one after another.
 template <typename T, int TYV>
-__global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
-                                                const InstD* __restrict__ I, T* Q,
-                                                const int8_t* __restrict__ B,
-                                                const DynD* __restrict__ dyn, int pred) {
+__device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, const int ti,
+                                            const InstD* __restrict__ I, T* Q,
+                                            const int8_t* __restrict__ B,
+                                            const DynD* __restrict__ dyn, int pred) {
   // Block (64, 4).  Each phase first issues the global loads of all its
   // iterations (compile-time trip counts, clamped in-bounds addresses, no
   // branch between the loads), then computes: one memory round trip per
@@ -386,7 +389,6 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   __shared__ T j1[JH][JW];
   __shared__ T bs[JH][JW];         // right-hand side on the first sweep's region
   __shared__ uint8_t sbs[XH][XW];  // stencil bits (finest level)
-  const int ti = task_index();
   const Task tk = tasks[ti];
   // the skip test is read and taken after the tile loads are issued (the
   // task holds the addressing, so the data loads follow the task read)
@@ -596,6 +598,14 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
   }
 }
 
+template <typename T, int TYV>
+__global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
+                                                const InstD* __restrict__ I, T* Q,
+                                                const int8_t* __restrict__ B,
+                                                const DynD* __restrict__ dyn, int pred) {
+  smooth_tile<T, TYV>(tasks, task_index(), I, Q, B, dyn, pred);
+}
+
 // ---------------------------------------------------------------------------
 // k_resid_restrict: r = b - A x on the fine points a coarse tile restricts
 // (halo points through their periodic images), then the full weighting onto
@@ -607,16 +617,14 @@ __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
 //        64 rhs = celltocorner of the T plane at tk.xout, 128 x in the T pool
 // ---------------------------------------------------------------------------
 template <typename T, typename R, int TY2V>
-__global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__ tasks,
-                                                        const InstD* __restrict__ I,
-                                                        const R* RP, T* Q,
-                                                        const int8_t* __restrict__ B,
-                                                        const DynD* __restrict__ dyn, int pred,
-                                                        int flags, double* partial) {
+__device__ __forceinline__ void rr_tile(const Task* __restrict__ tasks, const int ti,
+                                        const InstD* __restrict__ I, const R* RP, T* Q,
+                                        const int8_t* __restrict__ B,
+                                        const DynD* __restrict__ dyn, int pred, int flags,
+                                        double* partial) {
   constexpr int RY = 2 * TY2V + 1, RX = 2 * TX2 + 1;
   __shared__ R rs[RY][RX];
   __shared__ double red[2][8];
-  const int ti = task_index();
   const Task tk = tasks[ti];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
   // the skip test is read and taken after the tile loads are issued (the
@@ -792,6 +800,94 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
       v = restrict_r<R>([&](int dj, int di) { return rs[cy + dj][cx + di]; });
     }
     store_img_mask32<T>(b2, nullptr, J2, I2, T(v), m2, n2);
+  }
+}
+
+template <typename T, typename R, int TY2V>
+__global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__ tasks,
+                                                        const InstD* __restrict__ I,
+                                                        const R* RP, T* Q,
+                                                        const int8_t* __restrict__ B,
+                                                        const DynD* __restrict__ dyn, int pred,
+                                                        int flags, double* partial) {
+  rr_tile<T, R, TY2V>(tasks, task_index(), I, RP, Q, B, dyn, pred, flags, partial);
+}
+
+// ---------------------------------------------------------------------------
+// k_persist: a run of consecutive smoother and residual/restriction phases
+// of the multigrid levels below the finest (the tile lists of what would
+// otherwise be one launch each), executed by one launch of G co-resident
+// workgroups with a device-wide barrier between phases.  Each phase is the
+// same tile function over the same tasks as its own launch would run, so the
+// results are bit for bit those of the launch-per-phase program.
+//
+// Barrier: one slot per persistent launch of the step program.  Workgroups
+// add to `arrive` and wait until it reaches (phase + 1) * G; the last
+// workgroup to leave the kernel re-arms the slot (every workgroup has then
+// left every wait).  Every wait is bounded: after BAR_SPIN polls a
+// workgroup gives up, raises the batch's fault flag (the host reports it as
+// an error) and the launch ends, so no configuration can hang the device.
+// ---------------------------------------------------------------------------
+struct BarSlot {
+  unsigned arrive, leave;
+  unsigned pad[30];  // one slot per 128-byte line
+};
+struct PPhase {
+  int kind;   // 0 smoother, 1 residual/restriction
+  int toff, ntask, pred, flags;
+};
+constexpr int BAR_SPIN = 1 << 18;
+
+__device__ __forceinline__ bool grid_sync(BarSlot* bs, int* fault, unsigned target) {
+  __shared__ int ok_sh;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    __hip_atomic_fetch_add(&bs->arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    for (int n = 0;; n++) {
+      if (__hip_atomic_load(&bs->arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target)
+        break;
+      if (n > BAR_SPIN || __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    ok_sh = ok;
+  }
+  __syncthreads();
+  return ok_sh != 0;
+}
+
+template <typename T, int TYV>
+__global__ void __launch_bounds__(256) k_persist(const PPhase* __restrict__ ph, int nph,
+                                                 const Task* __restrict__ tasks,
+                                                 const InstD* __restrict__ I, T* Q,
+                                                 const int8_t* __restrict__ B,
+                                                 const DynD* __restrict__ dyn, BarSlot* bar,
+                                                 int* fault) {
+  const unsigned G = gridDim.x;
+  bool ok = true;
+  for (int p = 0; p < nph && ok; p++) {
+    const PPhase f = ph[p];
+    for (int t = blockIdx.x; t < f.ntask; t += G) {
+      if (f.kind == 0)
+        smooth_tile<T, TYV>(tasks + f.toff, t, I, Q, B, dyn, f.pred);
+      else
+        rr_tile<T, T, TYV / 2>(tasks + f.toff, t, I, (const T*)Q, Q, B, dyn, f.pred, f.flags,
+                               nullptr);
+      __syncthreads();  // the next tile reuses the LDS tiles
+    }
+    if (p + 1 < nph) ok = grid_sync(bar, fault, (unsigned)(p + 1) * G);
+  }
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const unsigned e =
+        __hip_atomic_fetch_add(&bar->leave, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (e + 1 == G) {
+      __hip_atomic_store(&bar->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar->leave, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1721,7 +1817,8 @@ enum Kind {
   L_NPSUM,
   L_DIAG,
   L_FOLD,
-  L_GATE
+  L_GATE,
+  L_PERSIST
 };
 
 struct Launch {
@@ -1732,6 +1829,8 @@ struct Launch {
   int mode = 0;              // coarse mode / fold what
   int span_off = 0;          // fold spans (one entry per instance of the window)
   int inst0 = 0, ninst = 0;  // fold: instance window [inst0, inst0 + ninst)
+  int ph_off = 0, nph = 0;   // persistent launch: phase table slice
+  int bar = 0;               // persistent launch: barrier slot
   int slot_off = 0;          // npsum slots
   size_t shmem = 0;          // k_coarse dynamic LDS
   int ty = TY;               // fine tile rows of k_smooth (k_resid_restrict: ty / 2 coarse rows)
@@ -1788,6 +1887,13 @@ struct Batch {
   int fast_maxite = 0;
   double fast_tol = 1e-10;
   size_t max_shmem = 0;
+  // persistent multigrid launches (tuning.persist workgroups per CU, 0 off)
+  int persist = 0;
+  std::vector<PPhase> phases;
+  PPhase* dPh = nullptr;
+  BarSlot* bar = nullptr;
+  int nbar = 0;
+  int* fault = nullptr;  // device flag: a persistent barrier gave up
   int min_tiles = 0;           // fine-phase tile choice (Builder::pick_ty)
   int coarse_wave_cells = 256; // coarse levels of at most this many cells run on one wavefront
   size_t coarse_budget = 0;    // LDS bytes for k_coarse (0: device maximum)
@@ -2458,6 +2564,20 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
       case L_GATE:
         F2D_LAUNCH(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
+      case L_PERSIST: {
+        const PPhase* PH = b.dPh + L.ph_off;
+        BarSlot* BS = b.bar + L.bar;
+        if (L.ty == 4)
+          F2D_LAUNCH((k_persist<T, 4>), dim3(g), B2, 0, b.s, PH, L.nph, b.dT, b.dI, Q, b.B, b.dD,
+                     BS, b.fault);
+        else if (L.ty == 8)
+          F2D_LAUNCH((k_persist<T, 8>), dim3(g), B2, 0, b.s, PH, L.nph, b.dT, b.dI, Q, b.B, b.dD,
+                     BS, b.fault);
+        else
+          F2D_LAUNCH((k_persist<T, TY>), dim3(g), B2, 0, b.s, PH, L.nph, b.dT, b.dI, Q, b.B,
+                     b.dD, BS, b.fault);
+        break;
+      }
       case L_FOLD:
         F2D_LAUNCH(k_fold, dim3(L.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
                            b.partial, L.mode, L.maxite < 0 ? b.maxite : L.maxite,
@@ -2481,6 +2601,24 @@ static int run_prog(Batch& b, const std::vector<Launch>& prog, ProfSel* prof = n
 static void launch_bytes(const Batch& b, const Launch& L, double* alg, double* impl) {
   const double tz = b.prec == 0 ? 8. : 4.;
   double a = 0., im = 0.;
+  if (L.kind == L_PERSIST) {  // the sum over its phases
+    for (int p = 0; p < L.nph; p++) {
+      const PPhase& f = b.phases[L.ph_off + p];
+      Launch q;
+      q.kind = f.kind == 0 ? L_SMOOTH : L_RR;
+      q.toff = f.toff;
+      q.ntask = f.ntask;
+      q.flags = f.flags;
+      q.ty = L.ty;
+      double qa = 0., qi = 0.;
+      launch_bytes(b, q, &qa, &qi);
+      a += qa;
+      im += qi;
+    }
+    *alg = a;
+    *impl = im;
+    return;
+  }
   for (int k = 0; k < L.ntask; k++) {
     const Task& t = b.tasks[L.toff + k];
     const InstD& d = b.H[t.inst].d;
@@ -2542,6 +2680,19 @@ static void drop_graph(Batch& b) {
   b.graph = nullptr;
 }
 
+// After the batch's stream has drained: did a persistent launch's barrier
+// give up?  (The flag is cleared; the results of that step are invalid.)
+static int check_fault(Batch& b) {
+  if (b.nbar == 0 || !b.fault) return F2D_OK;
+  int f = 0;
+  ECHK(hipMemcpy(&f, b.fault, sizeof(int), hipMemcpyDeviceToHost));
+  if (!f) return F2D_OK;
+  ECHK(hipMemset(b.fault, 0, sizeof(int)));
+  return efail(F2D_E_HIP, "a persistent multigrid barrier gave up waiting: not all of the "
+               "launch's workgroups could be resident (another persistent batch running on "
+               "the device?); the results since the last check are invalid");
+}
+
 static void batch_free(Batch* b) {
   if (!b) return;
   // everything enqueued on the batch's stream reads the buffers freed below
@@ -2558,8 +2709,78 @@ static void batch_free(Batch* b) {
   if (b->dS) (void)hipFree(b->dS);
   if (b->dSlot) (void)hipFree(b->dSlot);
   if (b->partial) (void)hipFree(b->partial);
+  if (b->dPh) (void)hipFree(b->dPh);
+  if (b->bar) (void)hipFree(b->bar);
+  if (b->fault) (void)hipFree(b->fault);
   if (b->s) (void)hipStreamDestroy(b->s);
   delete b;
+}
+
+// Workgroups of one persistent launch of tile height ty: at most what can be
+// resident at once (occupancy of the kernel times the compute units), at
+// most `persist` per compute unit, at most the largest phase's tile count.
+template <typename T>
+static int persist_grid(int ty, int per_cu, int max_tasks, int* grid) {
+  int dev = 0, ncu = 0, occ = 0;
+  ECHK(hipGetDevice(&dev));
+  ECHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const void* fn = ty == 4 ? (const void*)k_persist<T, 4>
+                           : (ty == 8 ? (const void*)k_persist<T, 8> : (const void*)k_persist<T, TY>);
+  ECHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0));
+  if (occ < 1) return efail(F2D_E_HIP, "persistent kernel cannot be resident");
+  *grid = std::max(1, std::min({max_tasks, std::min(occ, per_cu) * ncu}));
+  return F2D_OK;
+}
+
+// Runs of consecutive smoother and residual/restriction launches (the
+// residual only in its plain forms: restrict r or b, no norms, no stored
+// residual) whose tasks all lie below the finest level and share a tile
+// height become one persistent launch (k_persist), phases in order.
+static int merge_persistent(Batch& b, std::vector<Launch>& prog) {
+  if (b.persist <= 0) return F2D_OK;
+  auto mergeable = [&](const Launch& L) {
+    if (L.ntask <= 0) return false;
+    if (L.kind == L_RR) {
+      if ((L.flags & ~2) != 0) return false;
+    } else if (L.kind != L_SMOOTH) {
+      return false;
+    }
+    for (int k = 0; k < L.ntask; k++)
+      if (b.tasks[L.toff + k].lev == 0) return false;
+    return true;
+  };
+  std::vector<Launch> out;
+  size_t i = 0;
+  while (i < prog.size()) {
+    size_t j = i;
+    if (mergeable(prog[i]))
+      while (j < prog.size() && mergeable(prog[j]) && prog[j].ty == prog[i].ty) j++;
+    if (j - i < 2) {
+      out.push_back(prog[i]);
+      i++;
+      continue;
+    }
+    Launch P;
+    P.kind = L_PERSIST;
+    P.ty = prog[i].ty;
+    P.ph_off = (int)b.phases.size();
+    P.nph = (int)(j - i);
+    P.bar = b.nbar++;
+    int maxt = 0;
+    for (size_t k = i; k < j; k++) {
+      const Launch& L = prog[k];
+      b.phases.push_back(PPhase{L.kind == L_SMOOTH ? 0 : 1, L.toff, L.ntask, L.pred, L.flags});
+      maxt = std::max(maxt, L.ntask);
+    }
+    int g = 0;
+    ETRY(b.prec == 0 ? persist_grid<double>(P.ty, b.persist, maxt, &g)
+                     : persist_grid<float>(P.ty, b.persist, maxt, &g));
+    P.ntask = g;  // the grid
+    out.push_back(P);
+    i = j;
+  }
+  prog.swap(out);
+  return F2D_OK;
 }
 
 static int build_programs(Batch& b) {
@@ -2598,6 +2819,13 @@ static int build_programs(Batch& b) {
   }
   bd.lo = 0;
   bd.hi = b.ninst;
+  // persistent launches over runs of mid-level multigrid phases
+  b.phases.clear();
+  b.nbar = 0;
+  ETRY(merge_persistent(b, b.prog_init));
+  ETRY(merge_persistent(b, b.prog_step));
+  ETRY(merge_persistent(b, b.prog_diag));
+  for (auto& po : b.prog_one) ETRY(merge_persistent(b, po));
   size_t np = 0;
   std::vector<const std::vector<Launch>*> progs = {&b.prog_init, &b.prog_step, &b.prog_diag};
   for (const auto& po : b.prog_one) progs.push_back(&po);
@@ -2628,6 +2856,21 @@ static int build_programs(Batch& b) {
     t.sb = (t.lev == 0 && d.a0fly) ? d.sb0 : -1;
   }
   ECHK(hipMemcpy(b.dT, b.tasks.data(), b.tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
+  if (b.dPh) (void)hipFree(b.dPh);
+  if (b.bar) (void)hipFree(b.bar);
+  b.dPh = nullptr;
+  b.bar = nullptr;
+  if (!b.phases.empty()) {
+    ECHK(hipMalloc(&b.dPh, b.phases.size() * sizeof(PPhase)));
+    ECHK(hipMemcpy(b.dPh, b.phases.data(), b.phases.size() * sizeof(PPhase),
+                   hipMemcpyHostToDevice));
+    ECHK(hipMalloc(&b.bar, (size_t)b.nbar * sizeof(BarSlot)));
+    ECHK(hipMemset(b.bar, 0, (size_t)b.nbar * sizeof(BarSlot)));
+  }
+  if (!b.fault) {
+    ECHK(hipMalloc(&b.fault, sizeof(int)));
+    ECHK(hipMemset(b.fault, 0, sizeof(int)));
+  }
   ECHK(hipMemcpy(b.dS, b.spans.data(), b.spans.size() * sizeof(FoldSpan), hipMemcpyHostToDevice));
   if (!b.slots.empty())
     ECHK(hipMemcpy(b.dSlot, b.slots.data(), b.slots.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -2705,6 +2948,7 @@ static int apply_tuning(Batch& b, const f2d_batch_tuning* t) {
     maxlds = std::max(maxlds, b.H[i].coarse_lds);
   }
   b.min_tiles = t ? std::max(0, t->min_tiles) : 0;
+  b.persist = t ? std::max(0, t->persist) : 0;
   b.coarse_wave_cells = (t && t->coarse_wave_cells >= 0) ? t->coarse_wave_cells : 256;
   b.coarse_budget = budget;
   if (maxlds > 65536) {
@@ -3001,6 +3245,7 @@ int f2d_batch_tuning_info(f2d_batch* b, f2d_batch_tuning* out) {
   out->coarse_lds_budget = (int)b->coarse_budget;
   out->min_tiles = b->min_tiles;
   out->coarse_wave_cells = b->coarse_wave_cells;
+  out->persist = b->persist;
   return F2D_OK;
 }
 
@@ -3033,6 +3278,7 @@ int f2d_batch_get_state(f2d_batch* b, int inst, double* host) {
   if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
   const InstD& d = b->H[inst].d;
   ECHK(hipStreamSynchronize(b->s));
+  ETRY(check_fault(*b));
   ECHK(hipMemcpy(host, b->P + d.state, (size_t)d.nvar * d.m * d.n * 8, hipMemcpyDeviceToHost));
   return F2D_OK;
 }
@@ -3160,13 +3406,13 @@ int f2d_batch_init_instance(f2d_batch* b, int inst) {
 int f2d_batch_set_psi_from_vorticity(f2d_batch* b) {
   ETRY(run_prog(*b, b->prog_init));
   ECHK(hipStreamSynchronize(b->s));
-  return F2D_OK;
+  return check_fault(*b);
 }
 
 int f2d_batch_refresh_diagnostics(f2d_batch* b) {
   ETRY(run_prog(*b, b->prog_diag));
   ECHK(hipStreamSynchronize(b->s));
-  return F2D_OK;
+  return check_fault(*b);
 }
 
 static int ensure_graph(f2d_batch* b) {
@@ -3198,19 +3444,19 @@ int f2d_batch_step_async(f2d_batch* b, int nsteps) {
 int f2d_batch_step(f2d_batch* b, int nsteps) {
   ETRY(f2d_batch_step_async(b, nsteps));
   ECHK(hipStreamSynchronize(b->s));
-  return F2D_OK;
+  return check_fault(*b);
 }
 
 int f2d_batch_synchronize(f2d_batch* b) {
   ECHK(hipStreamSynchronize(b->s));
-  return F2D_OK;
+  return check_fault(*b);
 }
 
 int f2d_batch_query(f2d_batch* b, int* done) {
   const hipError_t e = hipStreamQuery(b->s);
   if (e == hipSuccess) {
     *done = 1;
-    return F2D_OK;
+    return check_fault(*b);
   }
   if (e == hipErrorNotReady) {
     *done = 0;
@@ -3221,8 +3467,8 @@ int f2d_batch_query(f2d_batch* b, int* done) {
 
 int f2d_batch_profile(f2d_batch* b, int kind, int level, int nsteps, double* avg_ms,
                       long long* nlaunch, double* alg_bytes, double* impl_bytes) {
-  const int kinds[4] = {L_SMOOTH, L_RR, L_RHS, L_COARSE};
-  if (kind < 0 || kind > 3) return efail(F2D_E_ARG, "kind 0..3");
+  const int kinds[5] = {L_SMOOTH, L_RR, L_RHS, L_COARSE, L_PERSIST};
+  if (kind < 0 || kind > 4) return efail(F2D_E_ARG, "kind 0..4");
   ProfSel ps;
   ps.kind = kinds[kind];
   ps.level = level;

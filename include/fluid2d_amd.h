@@ -273,11 +273,20 @@ const char *f2d_batch_last_error(void);
  *   cells (halo included) run on one wavefront without workgroup barriers;
  *   larger ones use the whole workgroup (0 = none on one wavefront;
  *   negative = the default, 256).
+ * persist: > 0 runs each run of consecutive smoother and
+ *   residual/restriction phases below the finest level as one persistent
+ *   launch of at most `persist` workgroups per compute unit, with a bounded
+ *   device-wide barrier between phases (same results bit for bit); 0 = one
+ *   launch per phase.  A batch with persistent launches must not share the
+ *   device with another such batch at the same time (their workgroups could
+ *   not all be resident); a barrier that cannot complete gives up and the
+ *   step reports F2D_E_HIP instead of hanging.
  * No reference counterpart (the reference has no launches). */
 typedef struct {
   int coarse_lds_budget;
   int min_tiles;
   int coarse_wave_cells;
+  int persist;
 } f2d_batch_tuning;
 /* precision: 0 FP64 throughout (bit-identical to the reference algorithm);
  *   1 mixed: FP64 state, RK combination, full-solve rhs, outer residual and

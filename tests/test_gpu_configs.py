@@ -222,6 +222,37 @@ def test_c5_full_stream_alone_equals_in_mix():
         assert (a == res[r.id]["state"]).all(), "request %d differs alone vs in the stream" % r.id
 
 
+def abs_sums(o, kind, g):
+    """Sums of the magnitudes of the terms of each diagnostic sum
+    (core/fortran_diag.f90: masks msk == 1 or msk != 0 as the routine
+    takes them), over the interior."""
+    s = np.s_[3:-3, 3:-3]
+    st = o.state
+    m1 = (o.msk[s] == 1)
+    m0 = (o.msk[s] != 0)
+    xr, yr = o.grid["xr"][s], o.grid["yr"][s]
+    w = st[o.varname_list.index("vorticity")][s]
+    u, v = st[o.varname_list.index("u")], st[o.varname_list.index("v")]
+    ke = (u[s] ** 2 + u[3:-3, 2:-4] ** 2 + v[s] ** 2 + v[2:-4, 3:-3] ** 2)
+    out = np.zeros(8)
+    out[1] = np.sum(ke * m1)
+    out[2] = np.sum(np.abs(w) * m1)
+    out[3] = np.sum(w ** 2 * m1)
+    if kind == "euler":
+        psi = st[o.varname_list.index("psi")][s]
+        src = st[o.varname_list.index("source")][s]
+        out[4] = np.sum(np.abs(w * xr) * m0)
+        out[5] = np.sum(np.abs(w * yr) * m0)
+        out[6] = np.sum(np.abs(psi) * m0)
+        out[7] = np.sum(np.abs(w * src) * m0)
+    else:
+        bu = st[o.varname_list.index("buoyancy")][s]
+        out[4] = np.sum(np.abs(bu * yr) * m0)
+        out[5] = np.sum(np.abs(bu) * m1)
+        out[6] = np.sum(bu ** 2 * m1)
+    return np.maximum(out, 1e-300)
+
+
 @pytest.mark.parametrize("case", [
     ("euler", 128, 128, 1.0, 1.0, "closed"),
     ("euler", 128, 128, 1.0, 1.0, "perio"),
@@ -254,15 +285,15 @@ def test_diagnostic_sums_vs_oracle(case):
             o.run(4, dt)
         got = b.raw_diagnostics(0)
         ref = o.diagnostics()["raw"]
-        # maxspeed is a max: exact; the sums are tree-ordered on the device,
-        # serial in the reference
+        # maxspeed is a max: exact.  The sums are tree-ordered on the device
+        # and serial in the reference: they agree to rounding, i.e. to a
+        # small multiple of eps times the sum of the terms' magnitudes (a sum
+        # that cancels, such as the total vorticity, is compared at that
+        # scale, not at its own)
         assert got[0] == ref[0], when
+        scale = abs_sums(o, kind, g)
         for k in range(1, 8):
-            scale = max(abs(ref[k]), 1e-300)
-            assert abs(got[k] - ref[k]) <= 1e-12 * scale + 1e-14, (when, k, got[k], ref[k])
-        gd, od = b.diagnostics(0), o.diags
-        for key in gd:
-            assert gd[key] == pytest.approx(od[key], rel=1e-12, abs=1e-14), (when, key)
+            assert abs(got[k] - ref[k]) <= 1e-13 * scale[k], (when, k, got[k], ref[k], scale[k])
 
 
 @pytest.mark.parametrize("cfl,cap", [(0.5, None), (0.8, 0.9)])

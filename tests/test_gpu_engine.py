@@ -271,3 +271,38 @@ def test_mixed_precision_other_schemes(order, method, scheme):
     for k in range(out["fp64"].shape[0]):
         r = rel(out["mixed"][k], out["fp64"][k])
         assert r <= 1e-6, "field %d rel-L2 %.3e" % (k, r)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_persistent_multigrid_equals_launch_per_phase(precision):
+    """Runs of mid-level smoother and residual/restriction phases as one
+    persistent launch with device-wide barriers: bit for bit the
+    launch-per-phase program, single instance and mixed batch, with the
+    reference schedule and with the refined per-stage solves."""
+    specs = [
+        spec("euler", 256, 256, 1.0, 1.0, "closed"),
+        spec("euler", 128, 128, 1.0, 1.0, "perio"),
+        spec("boussinesq", 256, 128, 2.0, 1.0, "xchannel", diffusion=True, Kdiff=1e-5),
+    ]
+    for fast in (0, 4):
+        for group in ([specs[0]], specs):
+            print("case persistent", precision, fast, len(group), flush=True)
+            out = []
+            for persist in (0, 2):
+                b = Batch(group, precision=precision,
+                          tuning={"min_tiles": 1024, "persist": persist})
+                if fast:
+                    b.set_fast_solve(fast, 1e-10)
+                for i in range(len(group)):
+                    init_fields(b, i, 50 + i)
+                b.set_psi_from_vorticity()
+                for i in range(len(group)):
+                    b.set_dt(dt_for(b, i, b.diagnostics(i)["maxspeed"]), i)
+                b.step(8)
+                out.append((b.info(0)["launches_per_step"], [b.state(i) for i in range(len(group))]))
+                assert b.tuning()["persist"] == persist
+                del b
+            assert out[1][0] < out[0][0], "persistent program has %d launches, per-phase %d" % (
+                out[1][0], out[0][0])
+            for a, m in zip(out[0][1], out[1][1]):
+                assert (a == m).all()
