@@ -70,6 +70,7 @@ SIGNATURES = {
     "f2d_sum_f64": [_p, _p, _i, _i, _i, _p, _p],
     "f2d_sumandnorm_f64": [_p, _p, _i, _i, _i, _p, _p],
     "f2d_domain_sum_f64": [_p, _i, _i, _i, _p, _p],
+    "f2d_domain_sum_f64_noalloc": [_p, _i, _i, _i, _p, _p],
     "f2d_gmg_create": [_i, _i, _i, _d, _d, _d, _d, _p, _pp],
     "f2d_gmg_destroy": [_p],
     "f2d_gmg_set_helmholtz": [_p, _d],

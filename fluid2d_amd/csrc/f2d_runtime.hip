@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -882,6 +883,55 @@ int f2d_domain_sum_f64(const double* x, int nh, int m, int n, double* out, f2d_s
   const size_t tot = (size_t)(m - 2 * NH) * (n - 2 * NH);
   if ((tot + 8191) / 8192 > (size_t)kRedMaxBlocks * 8)
     return fail(F2D_E_ARG, "field too large for the NumPy-order sum");
+  launch_domain_sum(x, m, n, 0., partial, out, S(stream));
+  LAUNCHCHK();
+  return F2D_OK;
+}
+
+// ---- allocation-free NumPy-order sum ----
+// One partial per 8192-element chunk of the interior lives in a file-scope
+// device array: at most kNoAllocChunks chunks (64 KiB per GPU, held until
+// the process ends), i.e. interiors of up to 8192 * 8192 cells.  Every chunk
+// partial is written before the fold reads it, so nothing needs clearing
+// between calls.  All calls on one GPU share the array: calls that overlap
+// on that GPU (other streams, other host threads) race.
+constexpr int kNoAllocChunks = kRedMaxBlocks * 8;
+static __device__ __align__(256) double scratch[kNoAllocChunks];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the name itself is a host-side stand-in, not a device address).
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[64] = {};
+
+static int scratch_address(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(F2D_E_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(F2D_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return F2D_OK;
+}
+
+int f2d_domain_sum_f64_noalloc(const double* x, int nh, int m, int n, double* out,
+                               f2d_stream_t stream) {
+  TRY(check_field(nh, m, n));
+  const size_t tot = (size_t)(m - 2 * NH) * (n - 2 * NH);
+  const size_t nch = (tot + 8191) / 8192;
+  if (nch > (size_t)kNoAllocChunks)
+    return fail(F2D_E_ARG,
+                "f2d_domain_sum_f64_noalloc: %zu interior cells need %zu chunk partials; the "
+                "allocation-free sum holds at most %d (%lld cells)",
+                tot, nch, kNoAllocChunks, (long long)kNoAllocChunks * 8192);
+  double* partial = nullptr;
+  TRY(scratch_address(&partial));
   launch_domain_sum(x, m, n, 0., partial, out, S(stream));
   LAUNCHCHK();
   return F2D_OK;

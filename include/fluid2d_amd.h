@@ -153,6 +153,18 @@ int f2d_sumandnorm_f64(const int8_t *msk, const double *x, int nh, int m, int n,
  * NumPy's own summation order (bit-identical to NumPy 2.x); out[0] (device). */
 int f2d_domain_sum_f64(const double *x, int nh, int m, int n, double *out, f2d_stream_t stream);
 
+/* Allocation-free form of f2d_domain_sum_f64 (same kernels, same result bit
+ * for bit).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its chunk partials live in a
+ * library-owned device array of 64 KiB per GPU (held until the process
+ * ends), so interiors of at most 8192*8192 cells are accepted; larger ones
+ * return F2D_E_ARG.  It calls no hipMalloc/hipFree.  Calls on one GPU share
+ * that array: calls that overlap on the same GPU (other streams or host
+ * threads) race and must be serialised by the caller. */
+int f2d_domain_sum_f64_noalloc(const double *x, int nh, int m, int n, double *out,
+                               f2d_stream_t stream);
+
 /* ---- multigrid hierarchy (core/gmg/hierarchy.py Gmg, one subdomain) ---- */
 typedef struct f2d_gmg f2d_gmg;
 

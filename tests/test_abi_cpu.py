@@ -111,3 +111,13 @@ def test_batch_rejects_arrays_beyond_32bit_offsets(nx, ny, model, prec, ok):
     msg = lib.f2d_batch_last_error().decode()
     assert "2^32" in msg and "instance 0" in msg
     assert h.value is None
+
+
+def test_noalloc_sum_rejects_oversize_without_device_work():
+    """The allocation-free NumPy-order sum accepts interiors of at most
+    8192 * 8192 cells (its chunk partials live in a fixed 64 KiB device
+    array); a larger one is refused before any device call."""
+    lib = fluid2d_amd.load()
+    rc = lib.f2d_domain_sum_f64_noalloc(None, 3, 8193 + 6, 8192 + 6, None, None)
+    assert rc == _lib.F2D_E_ARG
+    assert "at most 8192" in _lib.last_error()
