@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--workload", default=HEADLINE,
                     help="workload of the plain leg; other than %s only with --legs plain or copy,plain "
                          "(kernel traces of the other configurations)" % HEADLINE)
+    ap.add_argument("--copy-mib", type=int, default=256,
+                    help="array size of the copy leg's calibration passes (MiB)")
     ap.add_argument("--eager", action="store_true",
                     help="launch the step kernel by kernel instead of replaying a hipGraph "
                          "(for kernel-trace profilers)")
@@ -300,7 +302,7 @@ def main():
         # calibration for the HBM byte counters: a plain device copy of a
         # known size (the runtime's copy kernel), timed with the same clock
         from fluid2d_amd.device import DeviceArray
-        nbytes = 256 << 20
+        nbytes = a.copy_mib << 20
         src = DeviceArray((nbytes // 8,), np.float64)
         dst = DeviceArray((nbytes // 8,), np.float64)
         for _ in range(5):

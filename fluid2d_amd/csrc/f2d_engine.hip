@@ -814,6 +814,143 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// k_smooth_rr: the down-leg of a V-cycle on a level below the finest entered
+// with x = 0 (hierarchy.py:103-108 for lev > lev1: smooth, residual,
+// restrict) in one launch instead of two.  One task is one coarse tile
+// (TY2V x TX2 cells of level lev+1); its workgroup smooths the fine region
+// the tile's full weighting reads (the 2 TY2V + 1 by 2 TX2 + 1 residual
+// points, plus one ring for the residual's stencil and one for the second
+// sweep's), forms the residuals, restricts them onto the tile (+ images) and
+// stores the smoothed x on the fine cells the tile owns (+ images).  The
+// regions of neighbouring tiles overlap and are recomputed with the same
+// operands in the same order.
+//
+// Positions are taken at their periodic images.  On every level below the
+// finest, the right-hand side, the operator and the mask are stored with
+// periodic images in their halo (the restriction, the coarsening and the
+// coarse-mask set-up all fill it), so a value evaluated at a halo position
+// from halo operands equals the value at its image: the first sweep's halo
+// rows, which smoothtwicewithA evaluates from the halo, and the second
+// sweep's and the residual's halo values, which the fills copy from the
+// images.  The per-cell arithmetic is jacobi_2d's and residual_2d's, as in
+// k_smooth (mode 1, explicit operator) and k_resid_restrict.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrap_img(int g, int m) {
+  const int M = m - 2 * NH;
+  while (g < NH) g += M;
+  while (g >= m - NH) g -= M;
+  return g;
+}
+
+template <typename T, int TY2V>
+__global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ tasks,
+                                                   const InstD* __restrict__ I, T* Q,
+                                                   const int8_t* __restrict__ B,
+                                                   const DynD* __restrict__ dyn, int pred) {
+  constexpr int NT = 256;
+  constexpr int RY = 2 * TY2V + 1, RX = 2 * TX2 + 1;  // residual points
+  constexpr int H1 = RY + 2, W1 = RX + 2;             // second sweep
+  constexpr int H2 = RY + 4, W2 = RX + 4;             // first sweep, rhs, mask
+  constexpr int NR2 = H2 * W2, IT2 = (NR2 + NT - 1) / NT;
+  __shared__ T x1s[H2][W2];
+  __shared__ T bs2[H2][W2];
+  __shared__ T x2s[H1][W1];
+  __shared__ T rs[RY][RX];
+  __shared__ int8_t ms[H2][W2];
+  __shared__ uint32_t cs[H2][W2];  // image cell index of each region position
+  const int ti = task_index();
+  const Task tk = tasks[ti];
+  const InstD& in = I[tk.inst];
+  const int m1 = tk.m, n1 = tk.n, m2 = tk.m2, n2 = tk.n2;
+  const size_t PL = (size_t)m1 * n1;
+  const int8_t* msk = B + tk.msk;
+  const int8_t* msk2 = B + tk.msk2;
+  const T* A = Q + tk.A;
+  const T* b = Q + tk.b;
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  const int fy0 = 2 * tk.y0 - 3, fx0 = 2 * tk.x0 - 3;  // first residual point
+  const int gy = fy0 - 2, gx = fx0 - 2;                // first-sweep region origin
+  // rhs and mask of the first sweep's region, every load issued before use
+  T bv[IT2];
+  int8_t mv[IT2];
+  uint32_t cv[IT2];
+#pragma unroll
+  for (int it = 0; it < IT2; it++) {
+    const int q = min(tid + it * NT, NR2 - 1);
+    const int rr = q / W2, cc = q - rr * W2;
+    cv[it] = at32(wrap_img(gy + rr, m1), wrap_img(gx + cc, n1), n1);
+    bv[it] = ld32(b, cv[it]);
+    mv[it] = ld32(msk, cv[it]);
+  }
+  if (skip(dyn, tk.inst, pred)) return;
+#pragma unroll
+  for (int it = 0; it < IT2; it++) {
+    const int q = tid + it * NT;
+    if (q < NR2) {
+      const int rr = q / W2, cc = q - rr * W2;
+      bs2[rr][cc] = bv[it];
+      ms[rr][cc] = mv[it];
+      cs[rr][cc] = cv[it];
+    }
+  }
+  const T c1 = T(in.omega), c2 = T(1. - in.omega);
+  auto Aat = [&](uint32_t c) {
+    const APlanes<T> Ap{A, PL, c};
+    return [=](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * n1 + di); };
+  };
+  // first sweep from x = 0 on the whole region (rhs from registers)
+#pragma unroll
+  for (int it = 0; it < IT2; it++) {
+    const int q = tid + it * NT;
+    if (q < NR2) {
+      const int rr = q / W2, cc = q - rr * W2;
+      x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aat(cv[it]),
+                                 [](int, int) { return T(0); });
+    }
+  }
+  __syncthreads();
+  // second sweep on the region less one ring
+  constexpr int NR1 = H1 * W1;
+  for (int q = tid; q < NR1; q += NT) {
+    const int r1 = q / W1, c1i = q - r1 * W1;
+    const int rr = r1 + 1, cc = c1i + 1;
+    x2s[r1][c1i] = jacobi_2d<T>(ms[rr][cc], bs2[rr][cc], c1, c2, Aat(cs[rr][cc]),
+                                [&](int dj, int di) { return x1s[rr + dj][cc + di]; });
+  }
+  __syncthreads();
+  // residuals at the restriction's points; the smoothed x of the owned
+  // fine cells (local rows/cols [0, 2*T) off the halo, under the coarse
+  // interior) stored with their images
+  T* xout = Q + tk.xout;
+  constexpr int NRR = RY * RX;
+  for (int q = tid; q < NRR; q += NT) {
+    const int r0 = q / RX, c0 = q - r0 * RX;
+    const int rr = r0 + 2, cc = c0 + 2;
+    rs[r0][c0] = residual_2d<T>(ms[rr][cc], bs2[rr][cc], Aat(cs[rr][cc]),
+                                [&](int dj, int di) { return x2s[r0 + 1 + dj][c0 + 1 + di]; });
+    const int jf = fy0 + r0, if_ = fx0 + c0;
+    if (r0 < 2 * TY2V && c0 < 2 * TX2 && jf < m1 - NH && if_ < n1 - NH &&
+        (tk.y0 + r0 / 2) < m2 - NH && (tk.x0 + c0 / 2) < n2 - NH)
+      store_img_mask32<T>(xout, nullptr, jf, if_, x2s[r0 + 1][c0 + 1], m1, n1);
+  }
+  __syncthreads();
+  // full weighting onto the coarse tile (+ images)
+  T* b2 = Q + in.L[tk.lev + 1].b;
+  constexpr int NC = TY2V * TX2;
+  for (int q = tid; q < NC; q += NT) {
+    const int ry = q / TX2, rx = q - ry * TX2;
+    const int J2 = tk.y0 + ry, I2 = tk.x0 + rx;
+    if (J2 >= m2 - NH || I2 >= n2 - NH) continue;
+    T v = T(0);
+    if (ld32(msk2, at32(J2, I2, n2)) != 0) {
+      const int cy = 2 * ry + 1, cx = 2 * rx + 1;
+      v = restrict_r<T>([&](int dj, int di) { return rs[cy + dj][cx + di]; });
+    }
+    store_img_mask32<T>(b2, nullptr, J2, I2, v, m2, n2);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_persist: a run of consecutive smoother and residual/restriction phases
 // of the multigrid levels below the finest (the tile lists of what would
 // otherwise be one launch each), executed by one launch of G co-resident
@@ -1818,7 +1955,8 @@ enum Kind {
   L_DIAG,
   L_FOLD,
   L_GATE,
-  L_PERSIST
+  L_PERSIST,
+  L_SMOOTH_RR
 };
 
 struct Launch {
@@ -1886,6 +2024,9 @@ struct Batch {
   // refinement)
   int fast_maxite = 0;
   double fast_tol = 1e-10;
+  // down-legs below the finest level entered from zero in one launch
+  // (k_smooth_rr) instead of a smoothing and a residual/restriction launch
+  bool fuse_down = true;
   size_t max_shmem = 0;
   // persistent multigrid launches (tuning.persist workgroups per CU, 0 off)
   int persist = 0;
@@ -2070,13 +2211,22 @@ struct Builder {
     for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1 - start[i]);
     for (int dd = Dmax; dd >= 0; dd--) {
       const int ty = pick_ty(who, start, dd);
-      std::vector<Task> ts, rs;
+      std::vector<Task> ts, rs, fs;
       for (int i : who) {
         const int D = b.H[i].d.nfine - 1 - start[i];
         if (dd > D) continue;
         const int l = b.H[i].d.nfine - 1 - dd;
-        smooth_task(i, l, dd == D ? m0 : 1, 0, ts, ty);
+        const int mode = dd == D ? m0 : 1;
         HostInst& h = b.H[i];
+        if (mode == 1 && l >= 1 && b.fuse_down) {
+          // entered with x = 0 below the finest level: smoothing, residual
+          // and restriction in one launch (k_smooth_rr)
+          const long long xo = h.buf[l][h.cur[l] ^ 1];  // the smoothed x (task xout)
+          h.cur[l] ^= 1;
+          rr_tasks(i, l, 0, xo, -1, fs, ty / 2);
+          continue;
+        }
+        smooth_task(i, l, mode, 0, ts, ty);
         rr_tasks(i, l, h.buf[l][h.cur[l]], -1, -1, rs, ty / 2);
       }
       if (!ts.empty()) {
@@ -2088,6 +2238,11 @@ struct Builder {
         Launch& Lr = add(L_RR, rs);
         Lr.pred = pred;
         Lr.ty = ty;
+      }
+      if (!fs.empty()) {
+        Launch& Lf = add(L_SMOOTH_RR, fs);
+        Lf.pred = pred;
+        Lf.ty = ty;
       }
     }
     coarse(who, 0, pred);
@@ -2564,6 +2719,15 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
       case L_GATE:
         F2D_LAUNCH(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
+      case L_SMOOTH_RR:
+        if (L.ty == 4)
+          F2D_LAUNCH((k_smooth_rr<T, 2>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        else if (L.ty == 8)
+          F2D_LAUNCH((k_smooth_rr<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        else
+          F2D_LAUNCH((k_smooth_rr<T, TY2>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                     L.pred);
+        break;
       case L_PERSIST: {
         const PPhase* PH = b.dPh + L.ph_off;
         BarSlot* BS = b.bar + L.bar;
@@ -2631,6 +2795,14 @@ static void launch_bytes(const Batch& b, const Launch& L, double* alg, double* i
       a += cells * (wrd + 1.) * tz;               // + x written
       const bool fly = t.lev == 0 && d.a0fly;
       im += cells * ((wrd + 1.) * tz + 1. + (fly ? 1. : 5. * tz) + (t.mode >= 2 ? 0.25 * tz : 0.));
+    } else if (L.kind == L_SMOOTH_RR) {
+      // rhs read and smoothed x written on the tile's fine cells, the
+      // restricted residual written on its coarse cells
+      const LevD& lc = d.L[t.lev + 1];
+      const int cj = std::min(L.ty / 2, lc.m - NH - t.y0), ci = std::min(TX2, lc.n - NH - t.x0);
+      const double cc = (double)std::max(cj, 0) * std::max(ci, 0);
+      a += 4. * cc * 2. * tz + cc * tz;
+      im += 4. * cc * (2. * tz + 1. + 5. * tz) * 1.3 + cc * (tz + 1.);
     } else if (L.kind == L_RR) {
       const LevD& lc = d.L[t.lev + 1];
       const int cj = std::min(L.ty / 2, lc.m - NH - t.y0), ci = std::min(TX2, lc.n - NH - t.x0);
@@ -3467,8 +3639,8 @@ int f2d_batch_query(f2d_batch* b, int* done) {
 
 int f2d_batch_profile(f2d_batch* b, int kind, int level, int nsteps, double* avg_ms,
                       long long* nlaunch, double* alg_bytes, double* impl_bytes) {
-  const int kinds[5] = {L_SMOOTH, L_RR, L_RHS, L_COARSE, L_PERSIST};
-  if (kind < 0 || kind > 4) return efail(F2D_E_ARG, "kind 0..4");
+  const int kinds[6] = {L_SMOOTH, L_RR, L_RHS, L_COARSE, L_PERSIST, L_SMOOTH_RR};
+  if (kind < 0 || kind > 5) return efail(F2D_E_ARG, "kind 0..5");
   ProfSel ps;
   ps.kind = kinds[kind];
   ps.level = level;
