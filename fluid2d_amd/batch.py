@@ -243,7 +243,8 @@ class Batch:
         _call("f2d_batch_query", self._h, C.byref(d))
         return bool(d.value)
 
-    KINDS = {"smooth": 0, "resid_restrict": 1, "rhs": 2, "coarse": 3, "persist": 4}
+    KINDS = {"smooth": 0, "resid_restrict": 1, "rhs": 2, "coarse": 3, "persist": 4,
+             "smooth_rr": 5}
 
     def profile(self, kind, level=-1, nsteps=2):
         """Time every launch of ``kind`` at ``level`` over ``nsteps`` eager

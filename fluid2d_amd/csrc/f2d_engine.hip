@@ -416,19 +416,47 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
   // first is used -- the right-hand side of both sweeps, x and the stencil
   // bits on the first sweep's stencil region, and for a prolongation the
   // fine mask there and the coarse tile under it.
+  //
+  // Two forms of the same computation.  The general form maps thread t to
+  // the flat region positions t, t + 256, ... and clips them to the array.
+  // The unclipped form serves tiles whose x region lies inside the array
+  // (every tile of a grid that the tile size divides): thread (x, y) holds
+  // column x of the rows y, y + 4, ..., the four (x region) or two
+  // (first-sweep region) columns past the 64th go to the first threads of
+  // the block, and every offset is the thread's base plus a multiple of
+  // four rows -- no division, bounds test or select per element.
+  constexpr int KX = XH / 4, KB = (JH + 3) / 4;
+  static_assert(XH % 4 == 0 && KB <= IT2 && KX + 1 <= IT1, "unclipped form registers");
+  const bool lean = y0 + TYV + 1 <= m - 2 && x0 + TX + 1 <= n - 2;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const uint32_t s4 = 4u * (uint32_t)n;
+  const uint32_t ox = at32(y0 - 2 + ty, x0 - 2 + tx, n);  // x region (ty, tx)
+  const uint32_t ob = ox + (uint32_t)n + 1u;              // first-sweep region (ty, tx)
+  const bool xe_on = tid < 4 * XH, be_on = tid < 2 * JH;
+  const int xer = xe_on ? tid >> 2 : 0, xec = TX + (tid & 3);
+  const int ber = be_on ? tid >> 1 : 0, bec = TX + (tid & 1);
+  const uint32_t oxe = at32(y0 - 2 + xer, x0 - 2 + xec, n);
+  const uint32_t obe = at32(y0 - 1 + ber, x0 - 1 + bec, n);
   uint32_t cc2[IT2];
   bool ok2[IT2];
   T bv[IT2];
+  T be = T(0);
+  if (lean) {
 #pragma unroll
-  for (int it = 0; it < IT2; it++) {
-    const int q = tid + it * NT;
-    const int rr = q / JW, cc = q - rr * JW;
-    const int j = y0 - 1 + rr, i = x0 - 1 + cc;
-    ok2[it] = q < N2 && j <= m - NH && i <= n - NH;
-    cc2[it] = ok2[it] ? at32(j, i, n) : 0u;
+    for (int k = 0; k < KB; k++) bv[k] = ld32(b, ty + 4 * k < JH ? ob + k * s4 : ob);
+    be = ld32(b, obe);
+  } else {
+#pragma unroll
+    for (int it = 0; it < IT2; it++) {
+      const int q = tid + it * NT;
+      const int rr = q / JW, cc = q - rr * JW;
+      const int j = y0 - 1 + rr, i = x0 - 1 + cc;
+      ok2[it] = q < N2 && j <= m - NH && i <= n - NH;
+      cc2[it] = ok2[it] ? at32(j, i, n) : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < IT2; it++) bv[it] = ld32(b, cc2[it]);
   }
-#pragma unroll
-  for (int it = 0; it < IT2; it++) bv[it] = ld32(b, cc2[it]);
   const int Ic = x0 + threadIdx.x;
   // x region: rows [y0-2, y0+TY+1] x cols [x0-2, x0+TX+1], clipped to
   // [1, m-2]; the coarse points its prolongation reads are rows
@@ -444,18 +472,28 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
   int8_t m1v[IT1];
   T cxv[ITC];
   int8_t cmv[ITC];
+  // the fine mask of the prolongation: bit 4 of the stencil bits where
+  // they exist (the level's 0/1 mask), else loaded
+  if (lean) {
 #pragma unroll
-  for (int it = 0; it < IT1; it++) {
-    const int q = tid + it * NT;
-    const int rr = q / XW, cc = q - rr * XW;
-    const int j = y0 - 2 + rr, i = x0 - 2 + cc;
-    ok1[it] = q < N1 && j <= m - 2 && i <= n - 2;
-    const uint32_t c = ok1[it] ? at32(j, i, n) : at32(1, 1, n);
-    xv[it] = rdx ? ld32(xin, c) : T(0);
-    bt[it] = fly ? ld32(sb, c) : (uint8_t)0;
-    // the fine mask of the prolongation: bit 4 of the stencil bits where
-    // they exist (the level's 0/1 mask), else loaded
-    m1v[it] = (rdi && !fly) ? ld32(msk, c) : (int8_t)0;
+    for (int k = 0; k <= KX; k++) {  // rows ty + 4k, then the extra columns
+      const uint32_t c = k < KX ? ox + k * s4 : oxe;
+      xv[k] = rdx ? ld32(xin, c) : T(0);
+      bt[k] = fly ? ld32(sb, c) : (uint8_t)0;
+      m1v[k] = (rdi && !fly) ? ld32(msk, c) : (int8_t)0;
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+      const int q = tid + it * NT;
+      const int rr = q / XW, cc = q - rr * XW;
+      const int j = y0 - 2 + rr, i = x0 - 2 + cc;
+      ok1[it] = q < N1 && j <= m - 2 && i <= n - 2;
+      const uint32_t c = ok1[it] ? at32(j, i, n) : at32(1, 1, n);
+      xv[it] = rdx ? ld32(xin, c) : T(0);
+      bt[it] = fly ? ld32(sb, c) : (uint8_t)0;
+      m1v[it] = (rdi && !fly) ? ld32(msk, c) : (int8_t)0;
+    }
   }
   if (rdi) {
 #pragma unroll
@@ -468,24 +506,40 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
     }
   }
   if (skip(dyn, tk.inst, pred)) return;
-#pragma unroll
-  for (int it = 0; it < IT2; it++) {
-    const int q = tid + it * NT;
-    if (q < N2) {
-      const int rr = q / JW, cc = q - rr * JW;
-      bs[rr][cc] = ok2[it] ? bv[it] : T(0);
-    }
-  }
   bool full = true;
+  if (lean) {
 #pragma unroll
-  for (int it = 0; it < IT1; it++) {
-    const int q = tid + it * NT;
-    if (q < N1) {
-      const int rr = q / XW, cc = q - rr * XW;
-      xs[rr][cc] = ok1[it] ? xv[it] : T(0);
-      sbs[rr][cc] = ok1[it] ? bt[it] : (uint8_t)0;
-      ms1[rr][cc] = fly ? (int8_t)((bt[it] >> 4) & 1) : m1v[it];
-      full = full && ok1[it] && bt[it] == 31;
+    for (int k = 0; k < KB; k++)
+      if (ty + 4 * k < JH) bs[ty + 4 * k][tx] = bv[k];
+    if (be_on) bs[ber][bec] = be;
+#pragma unroll
+    for (int k = 0; k <= KX; k++) {
+      if (k == KX && !xe_on) continue;
+      const int rr = k < KX ? ty + 4 * k : xer, cc = k < KX ? tx : xec;
+      xs[rr][cc] = xv[k];
+      sbs[rr][cc] = bt[k];
+      ms1[rr][cc] = fly ? (int8_t)((bt[k] >> 4) & 1) : m1v[k];
+      full = full && bt[k] == 31;
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < IT2; it++) {
+      const int q = tid + it * NT;
+      if (q < N2) {
+        const int rr = q / JW, cc = q - rr * JW;
+        bs[rr][cc] = ok2[it] ? bv[it] : T(0);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+      const int q = tid + it * NT;
+      if (q < N1) {
+        const int rr = q / XW, cc = q - rr * XW;
+        xs[rr][cc] = ok1[it] ? xv[it] : T(0);
+        sbs[rr][cc] = ok1[it] ? bt[it] : (uint8_t)0;
+        ms1[rr][cc] = fly ? (int8_t)((bt[it] >> 4) & 1) : m1v[it];
+        full = full && ok1[it] && bt[it] == 31;
+      }
     }
   }
   if (rdi) {
@@ -574,22 +628,56 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
       store_img_mask32<T>(xout, mp, J, Ic, v, m, n);
     }
   };
+  // the same two sweeps in the unclipped form; stores without images when
+  // no cell of the tile has one
+  const bool noimg =
+      y0 >= 2 * NH && y0 + TYV <= m - 2 * NH && x0 >= 2 * NH && x0 + TX <= n - 2 * NH;
+  auto sweeps_lean = [&](auto Jac) {
+#pragma unroll
+    for (int k = 0; k < KB; k++) {
+      const int rr = ty + 4 * k;
+      if (rr < JH)
+        j1[rr][tx] = Jac(ob + k * s4, rr + 1, tx + 1, bv[k],
+                         [&](int dj, int di) { return xs[rr + 1 + dj][tx + 1 + di]; });
+    }
+    if (be_on)
+      j1[ber][bec] = Jac(obe, ber + 1, bec + 1, be,
+                         [&](int dj, int di) { return xs[ber + 1 + dj][bec + 1 + di]; });
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IT3; k++) {
+      const int rr = ty + 4 * k;
+      const uint32_t c = ob + (uint32_t)n + 1u + k * s4;  // (y0 + rr, x0 + tx)
+      const T v = Jac(c, rr + 2, tx + 2, bs[rr + 1][tx + 1],
+                      [&](int dj, int di) { return j1[rr + 1 + dj][tx + 1 + di]; });
+      if (noimg)
+        st32(xout, c, mp ? v * T(ld32(mp, c)) : v);
+      else
+        store_img_mask32<T>(xout, mp, y0 + rr, Ic, v, m, n);
+    }
+  };
+  auto run = [&](auto Jac) {
+    if (lean)
+      sweeps_lean(Jac);
+    else
+      sweeps(Jac);
+  };
   if (full_tile) {
     // A_k = w_k everywhere, the mask is 1, and c1 / |A_5| is one value
     const T c3 = c1 / fabs(w[4]);
-    sweeps([&](size_t, int, int, T bc, auto X) {
+    run([&](size_t, int, int, T bc, auto X) {
       return jacobi_c3_2d<T>(bc, c3, c2, [&](int k, int, int) { return w[k]; }, X);
     });
   } else if (fly) {
     // A_5 = w_5 wherever the mask (bit 4) is set, and the value is
     // discarded where it is not: one c1 / |A_5| serves the tile
     const T c3 = c1 / fabs(w[4]);
-    sweeps([&](size_t, int r, int q, T bc, auto X) {
+    run([&](size_t, int r, int q, T bc, auto X) {
       const T v = jacobi_c3_2d<T>(bc, c3, c2, ABitsTile2<T, XW>{&sbs[r][q], w}, X);
       return ((sbs[r][q] >> 4) & 1) ? v : T(0);
     });
   } else {
-    sweeps([&](size_t c, int, int, T bc, auto X) {
+    run([&](size_t c, int, int, T bc, auto X) {
       const APlanes<T> Ap{A, PL, c};
       return jacobi_2d<T>(msk[c], bc, c1, c2,
                           [&](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * n + di); },
@@ -816,7 +904,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
 // ---------------------------------------------------------------------------
 // k_smooth_rr: the down-leg of a V-cycle on a level below the finest entered
 // with x = 0 (hierarchy.py:103-108 for lev > lev1: smooth, residual,
-// restrict) in one launch instead of two.  One task is one coarse tile
+// restrict) or, at the start level of an F-cycle's V-cycle, with
+// x = I(x of the coarser level) (hierarchy.py:145-151), in one launch
+// instead of two.  One task is one coarse tile
 // (TY2V x TX2 cells of level lev+1); its workgroup smooths the fine region
 // the tile's full weighting reads (the 2 TY2V + 1 by 2 TX2 + 1 residual
 // points, plus one ring for the residual's stencil and one for the second
@@ -842,16 +932,23 @@ __device__ __forceinline__ int wrap_img(int g, int m) {
   return g;
 }
 
-template <typename T, int TY2V>
+// X0: 1 the level is entered with x = 0 (a V-cycle's descent), 3 with
+// x = I(x of the coarser level) (an F-cycle's V-cycle from this level,
+// hierarchy.py:145-151); the prolongation is evaluated at the images too (the
+// coarse x and both masks are stored with their images).
+template <typename T, int TY2V, int X0>
 __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ tasks,
                                                    const InstD* __restrict__ I, T* Q,
                                                    const int8_t* __restrict__ B,
                                                    const DynD* __restrict__ dyn, int pred) {
+  static_assert(X0 == 1 || (X0 == 3 && TY2V <= 4), "x0 form");
   constexpr int NT = 256;
   constexpr int RY = 2 * TY2V + 1, RX = 2 * TX2 + 1;  // residual points
   constexpr int H1 = RY + 2, W1 = RX + 2;             // second sweep
   constexpr int H2 = RY + 4, W2 = RX + 4;             // first sweep, rhs, mask
+  constexpr int H3 = X0 == 3 ? RY + 6 : 1, W3 = X0 == 3 ? RX + 6 : 1;  // x0 (prolongation)
   constexpr int NR2 = H2 * W2, IT2 = (NR2 + NT - 1) / NT;
+  __shared__ T x0s[H3][W3];
   __shared__ T x1s[H2][W2];
   __shared__ T bs2[H2][W2];
   __shared__ T x2s[H1][W1];
@@ -882,6 +979,37 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
     bv[it] = ld32(b, cv[it]);
     mv[it] = ld32(msk, cv[it]);
   }
+  // X0 3: x = I(xc) on the first sweep's region plus one ring.  Fine row
+  // gy - 1 + rr (gy odd) lies over coarse row cy + rr / 2, so the coarse x
+  // and mask under the region and the fine mask on it are loaded at their
+  // images in the same round trip, and ip_eval (interp_at's expressions)
+  // runs out of LDS
+  constexpr int CH3 = (H3 - 1) / 2 + 2, CW3 = (W3 - 1) / 2 + 2, NC3 = CH3 * CW3;
+  constexpr int NM3 = H3 * W3;
+  constexpr int ITC3 = X0 == 3 ? (NC3 + NT - 1) / NT : 1, ITM3 = X0 == 3 ? (NM3 + NT - 1) / NT : 1;
+  __shared__ T cx3[CH3][CW3];
+  __shared__ int8_t cm3[CH3][CW3];
+  __shared__ int8_t m13[H3][W3];
+  const int cy = (gy + 1) >> 1, cxo = (gx + 1) >> 1;
+  T c3v[ITC3];
+  int8_t s3v[ITC3], m3v[ITM3];
+  if constexpr (X0 == 3) {
+    const T* xc = Q + tk.xc;
+#pragma unroll
+    for (int it = 0; it < ITC3; it++) {
+      const int q = min(tid + it * NT, NC3 - 1);
+      const int a = q / CW3, bb = q - a * CW3;
+      const uint32_t c = at32(wrap_img(cy + a, m2), wrap_img(cxo + bb, n2), n2);
+      c3v[it] = ld32(xc, c);
+      s3v[it] = ld32(msk2, c);
+    }
+#pragma unroll
+    for (int it = 0; it < ITM3; it++) {
+      const int q = min(tid + it * NT, NM3 - 1);
+      const int rr = q / W3, cc = q - rr * W3;
+      m3v[it] = ld32(msk, at32(wrap_img(gy - 1 + rr, m1), wrap_img(gx - 1 + cc, n1), n1));
+    }
+  }
   if (skip(dyn, tk.inst, pred)) return;
 #pragma unroll
   for (int it = 0; it < IT2; it++) {
@@ -893,19 +1021,61 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
       cs[rr][cc] = cv[it];
     }
   }
+  if constexpr (X0 == 3) {
+#pragma unroll
+    for (int it = 0; it < ITC3; it++) {
+      const int q = tid + it * NT;
+      if (q < NC3) {
+        const int a = q / CW3, bb = q - a * CW3;
+        cx3[a][bb] = c3v[it];
+        cm3[a][bb] = s3v[it];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < ITM3; it++) {
+      const int q = tid + it * NT;
+      if (q < NM3) {
+        const int rr = q / W3, cc = q - rr * W3;
+        m13[rr][cc] = m3v[it];
+      }
+    }
+    __syncthreads();
+    for (int q = tid; q < NM3; q += NT) {
+      const int rr = q / W3, cc = q - rr * W3;
+      const int a = rr >> 1, bb = cc >> 1;
+      IpOps<T> o;
+      o.a = cx3[a][bb];
+      o.b = cx3[a][bb + 1];
+      o.c = cx3[a + 1][bb];
+      o.d = cx3[a + 1][bb + 1];
+      o.sa = cm3[a][bb];
+      o.sb = cm3[a][bb + 1];
+      o.sc = cm3[a + 1][bb];
+      o.sd = cm3[a + 1][bb + 1];
+      o.m1 = m13[rr][cc];
+      o.par = ((gx - 1 + cc) & 1) | (((gy - 1 + rr) & 1) << 1);
+      x0s[rr][cc] = ip_eval<T>(o);
+    }
+    __syncthreads();
+  }
   const T c1 = T(in.omega), c2 = T(1. - in.omega);
   auto Aat = [&](uint32_t c) {
     const APlanes<T> Ap{A, PL, c};
     return [=](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * n1 + di); };
   };
-  // first sweep from x = 0 on the whole region (rhs from registers)
+  // first sweep on the whole region (rhs from registers), from x = 0 or
+  // from the prolongation
 #pragma unroll
   for (int it = 0; it < IT2; it++) {
     const int q = tid + it * NT;
     if (q < NR2) {
       const int rr = q / W2, cc = q - rr * W2;
-      x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aat(cv[it]),
-                                 [](int, int) { return T(0); });
+      if constexpr (X0 == 3)
+        x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aat(cv[it]),
+                                   [&](int dj, int di) { return x0s[rr + 1 + dj][cc + 1 + di]; });
+      else
+        x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aat(cv[it]),
+                                   [](int, int) { return T(0); });
     }
   }
   __syncthreads();
@@ -2027,6 +2197,9 @@ struct Batch {
   // down-legs below the finest level entered from zero in one launch
   // (k_smooth_rr) instead of a smoothing and a residual/restriction launch
   bool fuse_down = true;
+  // the same for the start level of an F-cycle's V-cycle below the finest
+  // (entered from the prolongation)
+  bool fuse_ip = true;
   size_t max_shmem = 0;
   // persistent multigrid launches (tuning.persist workgroups per CU, 0 off)
   int persist = 0;
@@ -2211,19 +2384,23 @@ struct Builder {
     for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1 - start[i]);
     for (int dd = Dmax; dd >= 0; dd--) {
       const int ty = pick_ty(who, start, dd);
-      std::vector<Task> ts, rs, fs;
+      std::vector<Task> ts, rs, fs, f3;
       for (int i : who) {
         const int D = b.H[i].d.nfine - 1 - start[i];
         if (dd > D) continue;
         const int l = b.H[i].d.nfine - 1 - dd;
         const int mode = dd == D ? m0 : 1;
         HostInst& h = b.H[i];
-        if (mode == 1 && l >= 1 && b.fuse_down) {
-          // entered with x = 0 below the finest level: smoothing, residual
-          // and restriction in one launch (k_smooth_rr)
+        if (l >= 1 && b.fuse_down && (mode == 1 || (mode == 3 && ty <= 8 && b.fuse_ip))) {
+          // below the finest level, entered with x = 0 or x = I(xc):
+          // smoothing, residual and restriction in one launch (k_smooth_rr)
           const long long xo = h.buf[l][h.cur[l] ^ 1];  // the smoothed x (task xout)
+          const long long xc = h.buf[l + 1][h.cur[l + 1]];
           h.cur[l] ^= 1;
-          rr_tasks(i, l, 0, xo, -1, fs, ty / 2);
+          std::vector<Task>& out = mode == 3 ? f3 : fs;
+          const size_t n0 = out.size();
+          rr_tasks(i, l, 0, xo, mode == 3 ? xc : -1, out, ty / 2);
+          for (size_t k = n0; k < out.size(); k++) out[k].mode = mode;
           continue;
         }
         smooth_task(i, l, mode, 0, ts, ty);
@@ -2239,10 +2416,12 @@ struct Builder {
         Lr.pred = pred;
         Lr.ty = ty;
       }
-      if (!fs.empty()) {
-        Launch& Lf = add(L_SMOOTH_RR, fs);
+      for (std::vector<Task>* f : {&fs, &f3}) {
+        if (f->empty()) continue;
+        Launch& Lf = add(L_SMOOTH_RR, *f);
         Lf.pred = pred;
         Lf.ty = ty;
+        Lf.mode = f == &f3 ? 3 : 1;
       }
     }
     coarse(who, 0, pred);
@@ -2720,13 +2899,23 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
         F2D_LAUNCH(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
       case L_SMOOTH_RR:
-        if (L.ty == 4)
-          F2D_LAUNCH((k_smooth_rr<T, 2>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
-        else if (L.ty == 8)
-          F2D_LAUNCH((k_smooth_rr<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
-        else
-          F2D_LAUNCH((k_smooth_rr<T, TY2>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+        if (L.mode == 3) {  // x0 = I(xc): tiles of at most 4 coarse rows
+          if (L.ty == 4)
+            F2D_LAUNCH((k_smooth_rr<T, 2, 3>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                       L.pred);
+          else
+            F2D_LAUNCH((k_smooth_rr<T, 4, 3>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                       L.pred);
+        } else if (L.ty == 4) {
+          F2D_LAUNCH((k_smooth_rr<T, 2, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                      L.pred);
+        } else if (L.ty == 8) {
+          F2D_LAUNCH((k_smooth_rr<T, 4, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                     L.pred);
+        } else {
+          F2D_LAUNCH((k_smooth_rr<T, TY2, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                     L.pred);
+        }
         break;
       case L_PERSIST: {
         const PPhase* PH = b.dPh + L.ph_off;
