@@ -38,6 +38,11 @@ _SIG = {
     "f2d_batch_set_dt": ([_p, _i, _d], _i),
     "f2d_batch_set_adaptive": ([_p, _i, _d, _d], _i),
     "f2d_batch_set_fast_solve": ([_p, _i, _d], _i),
+    "f2d_batch_set_noslip": ([_p, _i, _p, _p, _p, _p, _d, _d, _d, _i], _i),
+    "f2d_batch_set_islands": ([_p, _i, _p, _p], _i),
+    "f2d_batch_corner_mask": ([_p, _i, _p], _i),
+    "f2d_batch_set_momentum": ([_p, _i, _p, _p], _i),
+    "f2d_batch_enforce_momentum": ([_p, _i, _d, _d], _i),
     "f2d_batch_set_steps": ([_p, _i, _i], _i),
     "f2d_batch_progress": ([_p, _i, C.POINTER(_i), C.POINTER(C.c_longlong)], _i),
     "f2d_batch_use_graph": ([_p, _i], _i),
@@ -131,10 +136,6 @@ class Batch:
                 prm = Param(**s)
                 g = Grid(prm)
             prm.checkall()
-            for k in ("noslip", "isisland"):
-                if getattr(prm, k, False):
-                    raise ValueError("%s runs on the single-model path (Fluid2d), not in a "
-                                     "batch" % k)
             self.params.append(prm)
             self.grids.append(g)
         n = len(self.params)
@@ -149,6 +150,52 @@ class Batch:
         self.n = n
         _call("f2d_batch_use_graph", self._h, 1 if use_graph else 0)
         self.varnames = [VARNAMES[p.modelname] for p in self.params]
+        for i, (prm, g) in enumerate(zip(self.params, self.grids)):
+            if getattr(prm, "noslip", False):
+                # Operators copies enforce_momentum from the parameters as
+                # given (core/operators.py:17), not as Fluid2d narrows it
+                self.set_noslip(i, enforce=bool(getattr(prm, "enforce_momentum", False)))
+            if getattr(prm, "isisland", False):
+                g.island.finalize()
+                self.set_islands(i, g.island.rhsp, g.island.psi)
+
+    def set_noslip(self, inst, enforce=False):
+        """No-slip walls on instance ``inst`` (core/operators.py:157-185,
+        :245-290) from its grid's ``msknoslip`` and boundary mask; with
+        ``enforce`` the source's momentum is removed as well."""
+        g = self.grids[inst]
+        mskbc, bcarea, x2bc, y2bc = g.boundary_mask()
+        a = [np.ascontiguousarray(v) for v in (g.msknoslip.astype(np.int8), mskbc.astype(np.int8),
+                                              np.asarray(g.xr0, np.float64),
+                                              np.asarray(g.yr0, np.float64))]
+        _call("f2d_batch_set_noslip", self._h, inst, *[v.ctypes.data for v in a],
+              float(bcarea), float(x2bc), float(y2bc), 1 if enforce else 0)
+
+    def set_islands(self, inst, rhsp, psi):
+        """Islands on instance ``inst``: the rhs correction ``rhsp`` and the
+        boundary psi of core/island.py:21-43."""
+        r = np.ascontiguousarray(rhsp, dtype=np.float64)
+        p = np.ascontiguousarray(psi, dtype=np.float64)
+        _call("f2d_batch_set_islands", self._h, inst, r.ctypes.data, p.ctypes.data)
+
+    def corner_mask(self, inst):
+        """The instance's corner mask mskp (int8, the state's plane shape)."""
+        g = self.grids[inst]
+        out = np.zeros((g.nyl, g.nxl), dtype=np.int8)
+        _call("f2d_batch_corner_mask", self._h, inst, out.ctypes.data)
+        return out
+
+    def set_momentum(self, inst):
+        """Enable the loop's momentum enforcement on instance ``inst``."""
+        g = self.grids[inst]
+        x = np.ascontiguousarray(g.xr0, dtype=np.float64)
+        y = np.ascontiguousarray(g.yr0, dtype=np.float64)
+        _call("f2d_batch_set_momentum", self._h, inst, x.ctypes.data, y.ctypes.data)
+
+    def enforce_momentum(self, inst, px, py):
+        """Fluid2d.enforce_zero_momentum (core/fluid2d.py:352-358) on instance
+        ``inst``: vorticity -= xr0 px + yr0 py, fast inversion, diagnostics."""
+        _call("f2d_batch_enforce_momentum", self._h, inst, float(px), float(py))
 
     def __del__(self):
         if getattr(self, "_h", None):

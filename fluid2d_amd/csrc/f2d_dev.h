@@ -297,6 +297,51 @@ __device__ __forceinline__ T ip_eval(const IpOps<T>& o) {
   return o.m1 <= 0 ? T(0) : r;
 }
 
+// No-slip wall source of interior cell (j, i) (core/fortran_operators.f90:
+// 221-268), cff = 1 / (2 dx dy).  The routine scatters each edge term into
+// the fluid cell beside the wall; gathered here per cell in the order the
+// routine's loop adds them: the cell's own x-edge and y-edge terms, then
+// the term of its east neighbour's x-edge (loop index i+1) and of its north
+// neighbour's y-edge (j+1).
+__device__ __forceinline__ double noslip_source_at(const int8_t* msk, const double* x, int j,
+                                                   int i, int n, double cff) {
+  auto M = [&](int jj, int ii) { return (int)msk[(size_t)jj * n + ii]; };
+  auto X = [&](int jj, int ii) { return x[(size_t)jj * n + ii]; };
+  // v at the west edge of (jj, ii), u at its south edge
+  auto vx = [&](int jj, int ii) {
+    return (X(jj, ii) + X(jj - 1, ii) - X(jj, ii - 2) - X(jj - 1, ii - 2)) * cff;
+  };
+  auto uy = [&](int jj, int ii) {
+    return -(X(jj, ii) + X(jj, ii - 1) - X(jj - 2, ii) - X(jj - 2, ii - 1)) * cff;
+  };
+  const int mc = M(j, i), me = M(j, i + 1), mn = M(j + 1, i);
+  double r = 0.;
+  if (mc != 0 && M(j, i - 1) + mc == 1) r = r - vx(j, i);
+  if (mc != 0 && M(j - 1, i) + mc == 1) r = r + uy(j, i);
+  if (me == 0 && mc + me == 1) r = r + vx(j, i + 1);
+  if (mn == 0 && mc + mn == 1) r = r - uy(j + 1, i);
+  return r;
+}
+
+// Calls f(j, i) at interior (J, I) and at each of its periodic halo images
+// (the positions store_img_mask32 writes).
+template <typename F>
+__device__ __forceinline__ void each_img(int J, int I, int m, int n, F f) {
+  f(J, I);
+  if (J >= 2 * NH && J < m - 2 * NH && I >= 2 * NH && I < n - 2 * NH) return;
+  const int M = m - 2 * NH, N = n - 2 * NH;
+  const bool jp = J + M < m && J + M >= m - NH, jm = J - M >= 0 && J - M < NH;
+  const bool ip = I + N < n && I + N >= n - NH, im = I - N >= 0 && I - N < NH;
+  if (jp) f(J + M, I);
+  if (jm) f(J - M, I);
+  if (ip) f(J, I + N);
+  if (im) f(J, I - N);
+  if (jp && ip) f(J + M, I + N);
+  if (jp && im) f(J + M, I - N);
+  if (jm && ip) f(J - M, I + N);
+  if (jm && im) f(J - M, I - N);
+}
+
 // Element i of a per-instance array through a 32-bit byte offset from a
 // uniform base: the load or store then takes the base from scalar registers
 // and one 32-bit vector offset (arrays of one instance stay below 4 GiB).

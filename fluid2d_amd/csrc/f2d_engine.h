@@ -47,6 +47,16 @@ struct InstD {
   long long sb0;
   int a0fly, pad0;
   double w0[5];
+  // no-slip walls (core/operators.py:157-185, :245-290), islands
+  // (core/island.py:21-43) and the loop's momentum enforcement
+  // (core/fluid2d.py:352-358): set after creation, device arrays of m x n
+  // (nullptr when unused).  nsrc is the no-slip source plane (the Euler
+  // state's field 4, a separate plane for Boussinesq).
+  int noslip, enforce, island, momentum;
+  double bcarea, x2bc, y2bc;
+  const int8_t *msknoslip, *mskbc;
+  const double *xr0, *yr0, *rhsp, *ipsi;
+  double* nsrc;
   LevD L[MAXLEV];
 };
 
@@ -56,6 +66,9 @@ struct DynD {
   double normeb, res0, res;
   int nite, ndiv, active, diverged;
   double psim;
+  double srcm;      // no-slip: mean source over the boundary cells
+  double nsdot[2];  // no-slip momentum: sum source*xr0, sum source*yr0 (msk != 0)
+  double mom[2];    // loop momentum enforcement: px / x2, py / y2 (host-written)
   double diag[NDIAG];
   int run;         // this step runs for the instance (set by the step gate)
   int steps_left;  // remaining steps (< 0: unlimited)

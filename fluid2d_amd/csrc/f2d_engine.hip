@@ -355,11 +355,15 @@ __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
   const int m = in.m, n = in.n;
   const T* src = SP + tk.xin;
   D* dst = DP + tk.xout;
+  // task flag 1: minus the islands' rhs correction (core/operators.py:444-446)
+  const double* rp = (tk.flags & 1) ? I[tk.inst].rhsp : nullptr;
   for (int r = threadIdx.y; r < TY; r += blockDim.y) {
     const int j = tk.y0 + r, i = tk.x0 + threadIdx.x;
     if (j >= m - 1 || i >= n - 1) continue;
     const size_t c = at(j, i, n);
-    dst[c] = D(0.25) * (D(src[c]) + D(src[c + 1]) + D(src[c + n]) + D(src[c + n + 1]));
+    D v = D(0.25) * (D(src[c]) + D(src[c + 1]) + D(src[c + n]) + D(src[c + n + 1]));
+    if (rp) v = v - D(rp[c]);
+    dst[c] = v;
   }
 }
 
@@ -1803,12 +1807,147 @@ __global__ void __launch_bounds__(256) k_finalize_psi(const Task* __restrict__ t
   const bool plane = tk.flags & 2;
   double* psi = plane ? ((tk.flags & 1) ? PQ : P) + tk.xout : P + in.state + F;
   const bool demean = !plane && in.perio;
+  // the islands' boundary psi after the mask (core/operators.py:483-489):
+  // the state's inversions (flag 4 on an explicit plane)
+  const bool isl = in.island && (!plane || (tk.flags & 4));
   const int8_t* mp = B + in.mskp;
   const double psim = dyn[tk.inst].psim;
   for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
     double v = psi[q];
     if (demean) v = v - psim;
-    psi[q] = v * (double)mp[q];
+    v = v * (double)mp[q];
+    if (isl) v = v + in.ipsi[q];
+    psi[q] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// no-slip walls (core/operators.py:245-290) and momentum enforcement
+// ---------------------------------------------------------------------------
+// source = the no-slip wall source of the interior cells of the tile
+// (noslip_source_at, from the state's psi after the time step)
+__global__ void __launch_bounds__(256) k_ns_source(const Task* __restrict__ tasks,
+                                                   const InstD* __restrict__ I, const double* P,
+                                                   const DynD* __restrict__ dyn) {
+  const int ti = task_index();
+  const Task tk = tasks[ti];
+  if (idle(dyn, tk.inst)) return;
+  const InstD& in = I[tk.inst];
+  const int m = in.m, n = in.n;
+  const double* psi = P + in.state + (size_t)m * n;
+  const double cff = 1. / (2 * in.dx * in.dy);  // core/fortran_operators.f90:240
+  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+    const int J = tk.y0 + r, Ic = tk.x0 + threadIdx.x;
+    if (J >= m - NH || Ic >= n - NH) continue;
+    in.nsrc[(size_t)J * n + Ic] = noslip_source_at(in.msknoslip, psi, J, Ic, n, cff);
+  }
+}
+
+// Momentum part of the no-slip source (enforce_momentum): source -=
+// meansource * mskbc on the interior, and per-tile partials of
+// {sum source*xr0, sum source*yr0} over msk != 0 (computedotprod,
+// core/fortran_diag.f90:3-30, summed per tile and folded)
+__global__ void __launch_bounds__(256) k_ns_dot(const Task* __restrict__ tasks,
+                                                const InstD* __restrict__ I,
+                                                const int8_t* __restrict__ B, double* partial,
+                                                const DynD* __restrict__ dyn) {
+  __shared__ double red[2][4];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
+  if (idle(dyn, tk.inst)) return;
+  const InstD& in = I[tk.inst];
+  const int m = in.m, n = in.n;
+  const int8_t* msk = B + in.msk;
+  const double srcm = dyn[tk.inst].srcm;
+  double a[2] = {0., 0.};
+  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+    const int J = tk.y0 + r, Ic = tk.x0 + threadIdx.x;
+    if (J >= m - NH || Ic >= n - NH) continue;
+    const size_t c = (size_t)J * n + Ic;
+    const double v = in.nsrc[c] - srcm * (double)in.mskbc[c];
+    in.nsrc[c] = v;
+    if (msk[c] != 0) {
+      a[0] += v * in.xr0[c];
+      a[1] += v * in.yr0[c];
+    }
+  }
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  for (int k = 0; k < 2; k++) {
+    double x = a[k];
+    for (int o = 32; o > 0; o >>= 1) x = x + __shfl_xor(x, o, 64);
+    if ((tid & 63) == 0) red[k][tid >> 6] = x;
+  }
+  __syncthreads();
+  if (tid < 2) {
+    double x = red[tid][0];
+    for (int q = 1; q < (int)(blockDim.x * blockDim.y) / 64; q++) x = x + red[tid][q];
+    partial[(size_t)ti * 2 + tid] = x;
+  }
+}
+
+// The rest of rhs_noslip on the interior cells of the tile: source -=
+// meansource * mskbc (or, after k_ns_dot, source -= (px xr0 + py yr0) mskbc
+// with px = sum/x2bc, py = sum/y2bc), the periodic fill, then vorticity -=
+// source on the cell and its images (core/operators.py:274-290)
+__global__ void __launch_bounds__(256) k_ns_fix(const Task* __restrict__ tasks,
+                                                const InstD* __restrict__ I, double* P,
+                                                const DynD* __restrict__ dyn) {
+  const int ti = task_index();
+  const Task tk = tasks[ti];
+  if (idle(dyn, tk.inst)) return;
+  const InstD& in = I[tk.inst];
+  const int m = in.m, n = in.n;
+  double* w = P + in.state;
+  const DynD& d = dyn[tk.inst];
+  for (int r = threadIdx.y; r < TY; r += blockDim.y) {
+    const int J = tk.y0 + r, Ic = tk.x0 + threadIdx.x;
+    if (J >= m - NH || Ic >= n - NH) continue;
+    const size_t c = (size_t)J * n + Ic;
+    double v = in.nsrc[c];
+    if (in.enforce) {
+      const double px = d.nsdot[0] / in.x2bc, py = d.nsdot[1] / in.y2bc;
+      v = v - (px * in.xr0[c] + py * in.yr0[c]) * (double)in.mskbc[c];
+    } else {
+      v = v - d.srcm * (double)in.mskbc[c];
+    }
+    each_img(J, Ic, m, n, [&](int j, int i) {
+      const size_t e = (size_t)j * n + i;
+      in.nsrc[e] = v;
+      w[e] = w[e] - v;
+    });
+  }
+}
+
+// Flat element ranges (task y0 = first element, x0 = count) of one plane:
+// flag 1: the no-slip source divided by the step's dt (core/euler.py:112);
+// flag 2: the loop's momentum enforcement, vorticity -= xr0 px + yr0 py
+// (core/fluid2d.py:352-358, px and py host-written);
+// flag 4 / 8: copy of an FP64 plane into the T pool of an FP64 batch (P at
+// xin -> PQ at xout) / back (PQ at xin -> P at xout, plus the islands'
+// boundary psi when the task's mode is 1: psi += island psi,
+// core/operators.py:483-489)
+__global__ void __launch_bounds__(256) k_flat_ops(const Task* __restrict__ tasks,
+                                                  const InstD* __restrict__ I, double* P,
+                                                  double* PQ, const DynD* __restrict__ dyn) {
+  const int ti = task_index();
+  const Task tk = tasks[ti];
+  if (idle(dyn, tk.inst)) return;
+  const InstD& in = I[tk.inst];
+  const DynD& d = dyn[tk.inst];
+  const int f = tk.flags;
+  for (int q = tk.y0 + threadIdx.x; q < tk.y0 + tk.x0; q += blockDim.x) {
+    if (f & 1) {
+      in.nsrc[q] = in.nsrc[q] / d.dt;
+    } else if (f & 2) {
+      double* w = P + in.state;
+      w[q] = w[q] - (in.xr0[q] * d.mom[0] + in.yr0[q] * d.mom[1]);
+    } else if (f & 4) {
+      PQ[tk.xout + q] = P[tk.xin + q];
+    } else {
+      double v = PQ[tk.xin + q];
+      if (tk.mode == 1) v = v + in.ipsi[q];
+      P[tk.xout + q] = v;
+    }
   }
 }
 
@@ -1886,11 +2025,14 @@ __global__ void __launch_bounds__(256) k_npsum(const Task* __restrict__ tasks,
   const Task tk = tasks[ti];
   if (idle(dyn, tk.inst)) return;
   const InstD& in = I[tk.inst];
-  if (!in.perio) return;
+  // task flag 1: the no-slip source (meansource, core/operators.py:274);
+  // else the state's psi of a periodic instance (core/operators.py:474-477)
+  const bool src = tk.flags & 1;
+  if (!src && !in.perio) return;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int m = in.m, n = in.n, N = n - 2 * NH;
   const size_t tot = (size_t)(m - 2 * NH) * N;
-  const double* psi = P + in.state + (size_t)m * n;
+  const double* psi = src ? in.nsrc : P + in.state + (size_t)m * n;
   for (int ch = wv; ch < tk.x0; ch += blockDim.x >> 6) {
     const size_t s0 = (size_t)(tk.y0 + ch) * 8192;
     const size_t cnt = tot - s0 < 8192 ? tot - s0 : 8192;
@@ -1989,7 +2131,8 @@ __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
 // threads, so the result is the same on every run.
 // what: 0 initial norms (slot pairs {r, b}), 1 iteration norms,
 //       2 diagnostics (8 per slot), 3 NumPy chunks -> psim (sequential,
-//       as NumPy adds its chunk sums in order)
+//       as NumPy adds its chunk sums in order), 4 NumPy chunks of the
+//       no-slip source -> srcm, 5 no-slip momentum sums (pairs)
 struct FoldSpan {
   int first, count;  // partial slots of this instance
 };
@@ -2005,14 +2148,18 @@ __global__ void __launch_bounds__(256) k_fold(const FoldSpan* __restrict__ spans
   DynD& d = dyn[inst];
   const FoldSpan sp = spans[blockIdx.x];
   if (sp.count == 0 || !d.run) return;
-  if (what == 3) {
-    if (tid != 0 || !I[inst].perio) return;
+  if (what == 3 || what == 4) {
+    if (tid != 0 || (what == 3 && !I[inst].perio)) return;
     double acc = 0.;
     for (int k = 0; k < sp.count; k++) acc = acc + partial[sp.first + k];
-    d.psim = acc * 1. / I[inst].area;
+    // domain_integration(z) = np.sum(interior) * 1. (core/grid.py:132-140)
+    if (what == 3)
+      d.psim = acc * 1. / I[inst].area;
+    else
+      d.srcm = acc * 1. / I[inst].bcarea;
     return;
   }
-  if (what != 2 && pred && !d.active) return;
+  if (what < 2 && pred && !d.active) return;
   const int nv = what == 2 ? 8 : 2;
   double a[8];
   for (int k = 0; k < nv; k++) a[k] = 0.;
@@ -2032,6 +2179,11 @@ __global__ void __launch_bounds__(256) k_fold(const FoldSpan* __restrict__ spans
   }
   if (tid != 0) return;
   for (int k = 0; k < nv; k++) a[k] = sh[k][0];
+  if (what == 5) {
+    d.nsdot[0] = a[0];
+    d.nsdot[1] = a[1];
+    return;
+  }
   if (what == 2) {
     a[0] *= 0.5;
     a[1] *= 0.25;
@@ -2126,7 +2278,11 @@ enum Kind {
   L_FOLD,
   L_GATE,
   L_PERSIST,
-  L_SMOOTH_RR
+  L_SMOOTH_RR,
+  L_NSSRC,
+  L_NSDOT,
+  L_NSFIX,
+  L_FLATOPS
 };
 
 struct Launch {
@@ -2179,6 +2335,10 @@ struct Batch {
   // per instance: its own initial inversion + diagnostics (a slot re-used
   // for a new run while the other instances keep stepping)
   std::vector<std::vector<Launch>> prog_one;
+  // per instance: the loop's momentum enforcement (empty unless enabled)
+  std::vector<std::vector<Launch>> prog_mom;
+  // device arrays of the no-slip walls, islands and momentum enforcement
+  std::vector<void*> extra;
   hipStream_t s = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -2464,13 +2624,16 @@ struct Builder {
   using Off = std::function<long long(HostInst&)>;
 
   // pools: 0 T -> T, 1 FP64 -> FP64 (state / work), 2 T -> FP64
-  void c2c(const std::vector<int>& who, Off src, Off dst, int pred, int pools) {
+  // island: minus the islands' rhs correction on instances that have them
+  void c2c(const std::vector<int>& who, Off src, Off dst, int pred, int pools,
+           bool island = false) {
     std::vector<Task> ts;
     for (int i : who) {
       HostInst& h = b.H[i];
       Task t = mk_task(i, 0, 0, 0);
       t.xin = src(h);
       t.xout = dst(h);
+      t.flags = (island && h.d.island) ? 1 : 0;
       tiles(ts, t, 0, h.d.m - 1, 0, h.d.n - 1);
     }
     Launch& L = add(L_C2C, ts);
@@ -2681,12 +2844,10 @@ struct Builder {
   }
 
   // full solve of the state's psi (core/operators.py:470-481)
-  void solve_full(const std::vector<int>& who) {
-    if (who.empty()) return;
-    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0, 1);
-    solve_defect(who, [](HostInst& h) { return h.d.state + (long long)h.d.m * h.d.n; }, false,
-                 [](HostInst& h) { return h.d.work; }, 16, b.maxite, b.tol);
-    // zero mean (perio) and mask, then (u, v)
+  // NumPy-order sum of an interior plane per instance (k_npsum chunks, then
+  // the fold `what`): flags 0 the state's psi (periodic instances), 1 the
+  // no-slip source
+  void npsum(const std::vector<int>& who, int flags, int what) {
     std::vector<Task> ns;
     for (int i : who) {
       HostInst& h = b.H[i];
@@ -2694,27 +2855,155 @@ struct Builder {
       const int nch = (int)((tot + 8191) / 8192);
       for (int c = 0; c < nch; c += 4) {  // one wavefront per chunk
         Task t = mk_task(i, 0, c, std::min(4, nch - c));
+        t.flags = flags;
         ns.push_back(t);
       }
     }
-    {
-      const int soff = (int)b.slots.size();
-      std::vector<FoldSpan> sp(hi - lo, FoldSpan{0, 0});
-      int slot = 0;
-      for (const Task& t : ns) {
-        b.slots.push_back(slot);
-        if (sp[t.inst - lo].count == 0) sp[t.inst - lo].first = slot;
-        sp[t.inst - lo].count += t.x0;
-        slot += t.x0;
-      }
-      const int spoff = (int)b.spans.size();
-      b.spans.insert(b.spans.end(), sp.begin(), sp.end());
-      Launch& L = add(L_NPSUM, ns);
-      L.slot_off = soff;
-      fold(3, spoff, 0);
+    const int soff = (int)b.slots.size();
+    std::vector<FoldSpan> sp(hi - lo, FoldSpan{0, 0});
+    int slot = 0;
+    for (const Task& t : ns) {
+      b.slots.push_back(slot);
+      if (sp[t.inst - lo].count == 0) sp[t.inst - lo].first = slot;
+      sp[t.inst - lo].count += t.x0;
+      slot += t.x0;
     }
+    const int spoff = (int)b.spans.size();
+    b.spans.insert(b.spans.end(), sp.begin(), sp.end());
+    Launch& L = add(L_NPSUM, ns);
+    L.slot_off = soff;
+    fold(what, spoff, 0);
+  }
+
+  // full solve of the state's psi (core/operators.py:470-481), with the
+  // islands' rhs correction and boundary psi where the instance has islands
+  void solve_full(const std::vector<int>& who) {
+    if (who.empty()) return;
+    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0, 1,
+        true);
+    solve_defect(who, [](HostInst& h) { return h.d.state + (long long)h.d.m * h.d.n; }, false,
+                 [](HostInst& h) { return h.d.work; }, 16, b.maxite, b.tol);
+    // zero mean (perio) and mask, then (u, v)
+    npsum(who, 0, 3);
     add(L_FINPSI, flat(who, [](HostInst& h) { return (long long)h.d.m * h.d.n; }, 1));
     ortho(who, [](HostInst& h) { return h.d.state; }, 1);
+  }
+
+  // k_flat_ops tasks over the m x n plane of instance i
+  void flat_ops(std::vector<Task>& out, int i, int flags, long long xin, long long xout,
+                int mode) {
+    const HostInst& h = b.H[i];
+    const long long F = (long long)h.d.m * h.d.n;
+    for (long long q = 0; q < F; q += FLAT) {
+      Task t = mk_task(i, 0, (int)q, (int)std::min<long long>(FLAT, F - q));
+      t.flags = flags;
+      t.xin = xin;
+      t.xout = xout;
+      t.mode = mode;
+      out.push_back(t);
+    }
+  }
+
+  // Fast inversion of the state (core/operators.py:421-498, flag 'fast':
+  // twoVcycle on the state's psi, psi * mskp, then (u, v)); island: with the
+  // islands' rhs correction and boundary psi where the instance has them
+  // (no-slip: core/euler.py:171-179; the loop's momentum enforcement runs
+  // it without islands).  An FP64 batch runs the two V-cycles on T-pool
+  // copies of psi and the rhs (bit-identical to the reference); a mixed
+  // batch in the correction form of invert_fast_refined: FP64 defect, FP32
+  // V-cycles, FP64 correction.
+  void invert_state_fast(const std::vector<int>& who, bool island) {
+    if (who.empty()) return;
+    c2c(who, [](HostInst& h) { return h.d.state; }, [](HostInst& h) { return h.d.work; }, 0, 1,
+        island);
+    std::vector<int> st(b.ninst, 0);
+    if (b.prec == 0) {
+      std::vector<Task> cp;
+      for (int i : who) {
+        HostInst& h = b.H[i];
+        const long long F = (long long)h.d.m * h.d.n;
+        h.buf[0][0] = h.x0alt;
+        h.buf[0][1] = h.d.x0[0];
+        h.cur[0] = 0;
+        flat_ops(cp, i, 4, h.d.state + F, h.x0alt, 0);
+        flat_ops(cp, i, 4, h.d.work, h.d.L[0].b, 0);
+      }
+      add(L_FLATOPS, cp);
+      vcycle(who, st, 0, 0, false);
+      vcycle(who, st, 0, 0, true);  // the last smoothing stores psi * mskp
+      std::vector<Task> back;
+      for (int i : who) {
+        HostInst& h = b.H[i];
+        const long long F = (long long)h.d.m * h.d.n;
+        flat_ops(back, i, 8, h.buf[0][h.cur[0]], h.d.state + F, (island && h.d.island) ? 1 : 0);
+      }
+      add(L_FLATOPS, back);
+    } else {
+      std::vector<Task> rs;
+      for (int i : who) {
+        HostInst& h = b.H[i];
+        h.buf[0][0] = h.d.x0[0];
+        h.buf[0][1] = h.d.x0[1];
+        h.cur[0] = 0;
+        rr_tasks(i, 0, h.d.state + (long long)h.d.m * h.d.n, h.d.work, h.d.L[0].b, rs);
+      }
+      add(L_RR, rs).flags = 1 | 16 | 32;  // FP64 defect, stored (T) and restricted
+      vcycle(who, st, 1, 0, false);
+      vcycle(who, st, 0, 0, false);
+      std::vector<Task> up, fin;
+      for (int i : who) {
+        HostInst& h = b.H[i];
+        const long long F = (long long)h.d.m * h.d.n;
+        for (long long q = 0; q < F; q += FLAT) {
+          Task t = mk_task(i, 1, (int)q, (int)std::min<long long>(FLAT, F - q));
+          t.xout = h.d.state + F;
+          t.xin = h.buf[0][h.cur[0]];
+          up.push_back(t);
+          t.flags = 2 | ((island && h.d.island) ? 4 : 0);
+          fin.push_back(t);
+        }
+      }
+      add(L_UPDATE, up);
+      add(L_FINPSI, fin);
+    }
+    ortho(who, [](HostInst& h) { return h.d.state; }, 1);
+  }
+
+  // No-slip walls after the time step (core/euler.py:108-112,
+  // core/boussinesq.py:92-94; Operators.rhs_noslip, core/operators.py:
+  // 245-290): the wall source, minus its mean over the boundary cells (and,
+  // with enforce_momentum, its momentum), filled and subtracted from the
+  // vorticity, then the fast inversion with islands; Euler keeps source/dt.
+  void noslip(const std::vector<int>& who) {
+    if (who.empty()) return;
+    std::vector<Task> ts, en;
+    for (int i : who) {
+      HostInst& h = b.H[i];
+      tiles(ts, mk_task(i, 0, 0, 0), NH, h.d.m - NH, NH, h.d.n - NH);
+      if (h.d.enforce) tiles(en, mk_task(i, 0, 0, 0), NH, h.d.m - NH, NH, h.d.n - NH);
+    }
+    add(L_NSSRC, ts);
+    npsum(who, 1, 4);
+    if (!en.empty()) {
+      const int sp = spans_of(en);
+      add(L_NSDOT, en);
+      fold(5, sp, 0);
+    }
+    add(L_NSFIX, ts);
+    invert_state_fast(who, true);
+    std::vector<Task> dv;
+    for (int i : who)
+      if (b.H[i].d.model == 0) flat_ops(dv, i, 1, 0, 0, 0);
+    if (!dv.empty()) add(L_FLATOPS, dv);
+  }
+
+  // The loop's momentum enforcement of instance i (core/fluid2d.py:352-358):
+  // vorticity -= xr0 px + yr0 py (px, py host-written), fast inversion.
+  void momentum(int i) {
+    std::vector<Task> mt;
+    flat_ops(mt, i, 2, 0, 0, 0);
+    add(L_FLATOPS, mt);
+    invert_state_fast({i}, false);
   }
 
   void diag() {
@@ -2768,6 +3057,10 @@ struct Builder {
       if (h.d.model == 0) eul.push_back(i);
     }
     add(L_COMBINE, cs);
+    std::vector<int> ns;
+    for (int i : who)
+      if (b.H[i].d.noslip) ns.push_back(i);
+    noslip(ns);
     solve_full(eul);
     diag();
   }
@@ -2931,6 +3224,18 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
                      b.dD, BS, b.fault);
         break;
       }
+      case L_NSSRC:
+        F2D_LAUNCH(k_ns_source, dim3(g), B2, 0, b.s, TK, b.dI, P, b.dD);
+        break;
+      case L_NSDOT:
+        F2D_LAUNCH(k_ns_dot, dim3(g), B2, 0, b.s, TK, b.dI, b.B, b.partial, b.dD);
+        break;
+      case L_NSFIX:
+        F2D_LAUNCH(k_ns_fix, dim3(g), B2, 0, b.s, TK, b.dI, P, b.dD);
+        break;
+      case L_FLATOPS:
+        F2D_LAUNCH(k_flat_ops, dim3(g), B1, 0, b.s, TK, b.dI, P, PQ, b.dD);
+        break;
       case L_FOLD:
         F2D_LAUNCH(k_fold, dim3(L.ninst), B1, 0, b.s, b.dS + L.span_off, b.dI, b.dD,
                            b.partial, L.mode, L.maxite < 0 ? b.maxite : L.maxite,
@@ -3073,6 +3378,7 @@ static void batch_free(Batch* b) {
   if (b->dPh) (void)hipFree(b->dPh);
   if (b->bar) (void)hipFree(b->bar);
   if (b->fault) (void)hipFree(b->fault);
+  for (void* p : b->extra) (void)hipFree(p);
   if (b->s) (void)hipStreamDestroy(b->s);
   delete b;
 }
@@ -3171,12 +3477,18 @@ static int build_programs(Batch& b) {
   bd.rk_step();
   // one instance's initial inversion and diagnostics, per instance
   b.prog_one.assign(b.ninst, {});
+  b.prog_mom.assign(b.ninst, {});
   for (int i = 0; i < b.ninst; i++) {
     bd.lo = i;
     bd.hi = i + 1;
     bd.prog = &b.prog_one[i];
     bd.solve_full({i});
     bd.diag();
+    if (b.H[i].d.momentum) {
+      bd.prog = &b.prog_mom[i];
+      bd.momentum(i);
+      bd.diag();
+    }
   }
   bd.lo = 0;
   bd.hi = b.ninst;
@@ -3187,12 +3499,15 @@ static int build_programs(Batch& b) {
   ETRY(merge_persistent(b, b.prog_step));
   ETRY(merge_persistent(b, b.prog_diag));
   for (auto& po : b.prog_one) ETRY(merge_persistent(b, po));
+  for (auto& po : b.prog_mom) ETRY(merge_persistent(b, po));
   size_t np = 0;
   std::vector<const std::vector<Launch>*> progs = {&b.prog_init, &b.prog_step, &b.prog_diag};
   for (const auto& po : b.prog_one) progs.push_back(&po);
+  for (const auto& po : b.prog_mom) progs.push_back(&po);
   for (auto* pr : progs)
     for (const Launch& L : *pr)
-      if (L.kind == L_RR || L.kind == L_DIAG) np = std::max(np, (size_t)L.ntask * 8);
+      if (L.kind == L_RR || L.kind == L_DIAG || L.kind == L_NSDOT)
+        np = std::max(np, (size_t)L.ntask * 8);
   np = std::max(np, (size_t)b.slots.size() * 256 + 256);
   if (b.partial) (void)hipFree(b.partial);
   if (b.dT) (void)hipFree(b.dT);
@@ -3711,6 +4026,116 @@ int f2d_batch_set_fast_solve(f2d_batch* b, int maxite, double tol) {
   b->fast_maxite = maxite;
   b->fast_tol = tol;
   return build_programs(*b);
+}
+
+}  // extern "C"
+
+// The launch programs again after an instance's options changed.
+static int reprogram(Batch& b) {
+  ECHK(hipStreamSynchronize(b.s));
+  drop_graph(b);
+  std::vector<InstD> di(b.ninst);
+  for (int i = 0; i < b.ninst; i++) di[i] = b.H[i].d;
+  ECHK(hipMemcpy(b.dI, di.data(), b.ninst * sizeof(InstD), hipMemcpyHostToDevice));
+  return build_programs(b);
+}
+// a device copy of cnt host values, owned by the batch
+template <typename T>
+static int upload(Batch& b, const T* host, size_t cnt, T** dev) {
+  void* p = nullptr;
+  ECHK(hipMalloc(&p, cnt * sizeof(T)));
+  b.extra.push_back(p);
+  ECHK(hipMemcpy(p, host, cnt * sizeof(T), hipMemcpyHostToDevice));
+  *dev = (T*)p;
+  return F2D_OK;
+}
+
+extern "C" {
+
+int f2d_batch_set_noslip(f2d_batch* b, int inst, const int8_t* msknoslip, const int8_t* mskbc,
+                         const double* xr0, const double* yr0, double bcarea, double x2bc,
+                         double y2bc, int enforce) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  if (!msknoslip || !mskbc || !xr0 || !yr0)
+    return efail(F2D_E_ARG, "f2d_batch_set_noslip: null argument");
+  if (!(bcarea > 0.)) return efail(F2D_E_ARG, "no boundary cells (bcarea = %g)", bcarea);
+  if (enforce && !(x2bc > 0. && y2bc > 0.))
+    return efail(F2D_E_ARG, "enforce_momentum needs x2bc, y2bc > 0 (got %g, %g)", x2bc, y2bc);
+  InstD& d = b->H[inst].d;
+  if (d.noslip) return efail(F2D_E_ARG, "instance %d: no-slip walls already set", inst);
+  const size_t F = (size_t)d.m * d.n;
+  ECHK(hipStreamSynchronize(b->s));
+  ETRY(upload(*b, msknoslip, F, const_cast<int8_t**>(&d.msknoslip)));
+  ETRY(upload(*b, mskbc, F, const_cast<int8_t**>(&d.mskbc)));
+  if (!d.xr0) {
+    ETRY(upload(*b, xr0, F, const_cast<double**>(&d.xr0)));
+    ETRY(upload(*b, yr0, F, const_cast<double**>(&d.yr0)));
+  }
+  if (d.model == 0) {
+    d.nsrc = b->P + d.state + 4 * F;  // the Euler state's source field
+  } else {
+    void* p = nullptr;
+    ECHK(hipMalloc(&p, F * sizeof(double)));
+    b->extra.push_back(p);
+    ECHK(hipMemset(p, 0, F * sizeof(double)));
+    d.nsrc = (double*)p;
+  }
+  d.noslip = 1;
+  d.enforce = enforce ? 1 : 0;
+  d.bcarea = bcarea;
+  d.x2bc = x2bc;
+  d.y2bc = y2bc;
+  return reprogram(*b);
+}
+
+int f2d_batch_corner_mask(f2d_batch* b, int inst, int8_t* host) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  const InstD& d = b->H[inst].d;
+  ECHK(hipStreamSynchronize(b->s));
+  ECHK(hipMemcpy(host, b->B + d.mskp, (size_t)d.m * d.n, hipMemcpyDeviceToHost));
+  return F2D_OK;
+}
+
+int f2d_batch_set_islands(f2d_batch* b, int inst, const double* rhsp, const double* psi) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  if (!rhsp || !psi) return efail(F2D_E_ARG, "f2d_batch_set_islands: null argument");
+  InstD& d = b->H[inst].d;
+  if (d.island) return efail(F2D_E_ARG, "instance %d: islands already set", inst);
+  const size_t F = (size_t)d.m * d.n;
+  ECHK(hipStreamSynchronize(b->s));
+  ETRY(upload(*b, rhsp, F, const_cast<double**>(&d.rhsp)));
+  ETRY(upload(*b, psi, F, const_cast<double**>(&d.ipsi)));
+  d.island = 1;
+  return reprogram(*b);
+}
+
+int f2d_batch_set_momentum(f2d_batch* b, int inst, const double* xr0, const double* yr0) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  if (!xr0 || !yr0) return efail(F2D_E_ARG, "f2d_batch_set_momentum: null argument");
+  InstD& d = b->H[inst].d;
+  if (d.momentum) return F2D_OK;
+  const size_t F = (size_t)d.m * d.n;
+  ECHK(hipStreamSynchronize(b->s));
+  if (!d.xr0) {
+    ETRY(upload(*b, xr0, F, const_cast<double**>(&d.xr0)));
+    ETRY(upload(*b, yr0, F, const_cast<double**>(&d.yr0)));
+  }
+  d.momentum = 1;
+  return reprogram(*b);
+}
+
+int f2d_batch_enforce_momentum(f2d_batch* b, int inst, double px, double py) {
+  if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");
+  if (!b->H[inst].d.momentum)
+    return efail(F2D_E_ARG, "instance %d: momentum enforcement not enabled", inst);
+  ECHK(hipStreamSynchronize(b->s));
+  b->hD[inst].mom[0] = px;
+  b->hD[inst].mom[1] = py;
+  ECHK(hipMemcpyAsync(&b->dD[inst].mom, &b->hD[inst].mom, sizeof(double) * 2,
+                      hipMemcpyHostToDevice, b->s));
+  ETRY(run_prog(*b, b->prog_mom[inst]));
+  ECHK(hipStreamSynchronize(b->s));
+  return check_fault(*b);
 }
 
 int f2d_batch_use_graph(f2d_batch* b, int on) {

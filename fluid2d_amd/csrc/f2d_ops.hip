@@ -7,33 +7,14 @@
 // variants live in f2d_fused.hip.  Each kernel cites the reference routine
 // whose arithmetic (operation order included) it reproduces.
 #include "f2d_ops.h"
+#include "f2d_dev.h"
 
 namespace f2d {
 
 // ---------------------------------------------------------------------------
 // advection
 // ---------------------------------------------------------------------------
-// Weights of the reference are float32 literal expressions stored into
-// real*8 (core/fortran_advection.f90:36-44): round in float, widen.
-#define W32(num, den) ((double)((float)(num) / (float)(den)))
-
-template <typename T>
-struct UpParams {
-  T zdx, zdy, u1, aa, bb, KK, a;
-  int method, order;
-};
-
-template <typename T>
-__device__ __forceinline__ T split_speed(T vel, const UpParams<T>& p) {
-  T UU = T(0);
-  if (p.method == 0) UU = fabs(vel);
-  if (p.method == 1) {
-    UU = fabs(vel);
-    if (UU < p.u1) UU = p.aa * (vel * vel) + p.bb;
-  }
-  if (p.method == 2) UU = T(2) * p.KK + log(cosh(vel * p.a)) / p.a;
-  return UU;
-}
+// (UpParams, split_speed and the float32-rounded weights W32: f2d_dev.h)
 
 // Flux through the edge between cell c and c+s (s = 1: east edge, s = n:
 // north edge).  core/fortran_advection.f90:73-146.
@@ -480,12 +461,9 @@ __global__ void k_sub_dev_scalar(T* __restrict__ x, const T* __restrict__ s, siz
   if (q < cnt) x[q] = x[q] - s[0];
 }
 
-// No-slip wall source (core/fortran_operators.f90:221-268).  The routine
-// scatters each edge term into the fluid cell beside the wall; gathered
-// here per cell in the order the routine's loop adds them: the cell's own
-// x-edge and y-edge terms, then the term of its east neighbour's x-edge
-// (loop index i+1) and of its north neighbour's y-edge (j+1).  Interior
-// cells only; the halo is zeroed (the caller's periodic fill rewrites it).
+// No-slip wall source (noslip_source_at, core/fortran_operators.f90:221-268)
+// on the interior cells; the halo is zeroed (the caller's periodic fill
+// rewrites it).
 __global__ void k_noslip_source(const int8_t* __restrict__ msk, const double* __restrict__ x,
                                 double* __restrict__ y, double cff, int m, int n) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -495,22 +473,7 @@ __global__ void k_noslip_source(const int8_t* __restrict__ msk, const double* __
     y[q] = 0.;
     return;
   }
-  auto M = [&](int jj, int ii) { return (int)msk[at(jj, ii, n)]; };
-  auto X = [&](int jj, int ii) { return x[at(jj, ii, n)]; };
-  // v at the west edge of (jj, ii), u at its south edge
-  auto vx = [&](int jj, int ii) {
-    return (X(jj, ii) + X(jj - 1, ii) - X(jj, ii - 2) - X(jj - 1, ii - 2)) * cff;
-  };
-  auto uy = [&](int jj, int ii) {
-    return -(X(jj, ii) + X(jj, ii - 1) - X(jj - 2, ii) - X(jj - 2, ii - 1)) * cff;
-  };
-  const int mc = M(j, i), me = M(j, i + 1), mn = M(j + 1, i);
-  double r = 0.;
-  if (mc != 0 && M(j, i - 1) + mc == 1) r = r - vx(j, i);
-  if (mc != 0 && M(j - 1, i) + mc == 1) r = r + uy(j, i);
-  if (me == 0 && mc + me == 1) r = r + vx(j, i + 1);
-  if (mn == 0 && mc + mn == 1) r = r - uy(j + 1, i);
-  y[q] = r;
+  y[q] = noslip_source_at(msk, x, j, i, n, cff);
 }
 // s -= d[0]*mskbc (core/operators.py:275-278)
 __global__ void k_sub_mean_mask(double* __restrict__ s, const int8_t* __restrict__ mk,

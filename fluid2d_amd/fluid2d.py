@@ -125,10 +125,12 @@ class Fluid2d:
             self.enforce_momentum = False
         else:
             raise ValueError("model %s is outside the device path" % param.modelname)
-        operator_level = (self.enforce_momentum or self.diag_fluxes
-                          or getattr(param, "noslip", False) or getattr(param, "isisland", False))
-        if getattr(param, "engine", True) and not operator_level:
+        # flux diagnostics re-run the time step term by term on the
+        # operator-level models
+        if getattr(param, "engine", True) and not self.diag_fluxes:
             self.model = EngineModel(param, grid)
+            if self.enforce_momentum:
+                self.model.batch.set_momentum(0)
         elif param.modelname == "euler":
             self.model = Euler(param, grid)
         else:
@@ -242,6 +244,9 @@ class Fluid2d:
             g = self.grid
             px = self.model.diags["px"] / g.x2
             py = self.model.diags["py"] / g.y2
+            if isinstance(self.model, EngineModel):
+                self.model.enforce_zero_momentum(px, py)
+                return
             vor = self.model.var.get("vorticity")
             vor[:] -= g.xr0 * px + g.yr0 * py
             self.model.ope.invert_vorticity(self.model.var.dev, flag="fast")

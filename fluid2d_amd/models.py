@@ -281,8 +281,8 @@ class EngineModel:
     1024^2.  Same ``var``, ``step``, ``run``, ``diagnostics``,
     ``set_psi_from_vorticity`` and ``diags``; the engine computes the
     diagnostics at the end of each step and inversion, and again after a
-    host write of the state.  Options the engine does not run
-    (no-slip, islands, momentum enforcement, flux diagnostics) use the
+    host write of the state.  No-slip walls, islands and the loop's
+    momentum enforcement run in the engine; the flux diagnostics use the
     operator-level models."""
 
     def __init__(self, param, grid):
@@ -356,6 +356,22 @@ class EngineModel:
         self.var.device_changed()
         self.diagnostics(self.var, None)
         return log
+
+    @property
+    def mskp(self):
+        return self.batch.corner_mask(0)
+
+    def set_islands(self, rhsp, psi):
+        """The batch took the grid's islands at construction (the same
+        ``rhsp`` and ``psi``: core/island.py:21-43 is deterministic)."""
+
+    def enforce_zero_momentum(self, px, py):
+        """core/fluid2d.py:352-358 in the engine: vorticity -= xr0 px + yr0 py,
+        the fast inversion of the state, fresh diagnostics."""
+        self._sync()
+        self.batch.enforce_momentum(0, px, py)
+        self._diag_fresh = True
+        self.var.device_changed()
 
     def diagnostics(self, var, t):
         """core/euler.py:185-221, core/boussinesq.py:123-153 (the engine's

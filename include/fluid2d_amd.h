@@ -333,6 +333,35 @@ int f2d_batch_set_adaptive(f2d_batch *b, int on, double cfl, double dtmax);
  * (iterative refinement to tol).  maxite = 0 restores twoVcycle (the
  * reference).  Rebuilds the step program; call between steps. */
 int f2d_batch_set_fast_solve(f2d_batch *b, int maxite, double tol);
+/* No-slip walls on instance inst (Operators.set_boundary_msk and
+ * rhs_noslip, core/operators.py:157-185, :245-290, run after each time
+ * step as core/euler.py:108-112 and core/boussinesq.py:92-94 do): the
+ * wall source from psi, minus its mean over the boundary cells mskbc (and,
+ * with enforce, its momentum against xr0, yr0 normalised by x2bc, y2bc),
+ * filled and subtracted from the vorticity, then the fast inversion of the
+ * state (with the instance's islands); Euler stores source/dt in its
+ * source field.  Host arrays of (ny+6) x (nx+6); once per instance;
+ * rebuilds the programs.  Same arguments as f2d_model_set_noslip. */
+int f2d_batch_set_noslip(f2d_batch *b, int inst, const int8_t *msknoslip, const int8_t *mskbc,
+                         const double *xr0, const double *yr0, double bcarea, double x2bc,
+                         double y2bc, int enforce);
+/* Islands on instance inst (core/island.py:21-43, core/fluid2d.py:110-113):
+ * the state's inversions (the full solve and the no-slip fast inversion)
+ * subtract rhsp from their right-hand side and add psi after the mask
+ * (core/operators.py:444-446, :483-489).  Once per instance. */
+int f2d_batch_set_islands(f2d_batch *b, int inst, const double *rhsp, const double *psi);
+/* the corner mask mskp of instance inst, (ny+6) x (nx+6) bytes (the
+ * operator-level f2d_model_corner_mask; Operators.mskp of
+ * core/operators.py:62-67) */
+int f2d_batch_corner_mask(f2d_batch *b, int inst, int8_t *host);
+/* The loop's momentum enforcement (Fluid2d.enforce_zero_momentum,
+ * core/fluid2d.py:352-358): set_momentum enables it with the centred
+ * coordinates xr0, yr0; enforce_momentum then subtracts xr0 px + yr0 py
+ * from the vorticity (px = diags px / x2, py = diags py / y2, from the
+ * host), runs the fast inversion of the state and refreshes the
+ * diagnostics (synchronous). */
+int f2d_batch_set_momentum(f2d_batch *b, int inst, const double *xr0, const double *yr0);
+int f2d_batch_enforce_momentum(f2d_batch *b, int inst, double px, double py);
 int f2d_batch_use_graph(f2d_batch *b, int on);
 /* set_psi_from_vorticity of every instance, then diagnostics (maxspeed) */
 int f2d_batch_set_psi_from_vorticity(f2d_batch *b);

@@ -183,9 +183,3 @@ def test_island_set_up_by_brute_force():
     want = np.where(cnt > 0, cnt * 0.3 / (g.dx * g.dy), 0.0)
     assert np.allclose(g.island.rhsp, want, rtol=1e-15, atol=0)
 
-
-def test_batch_rejects_noslip_and_islands():
-    from fluid2d_amd.batch import Batch
-    for k in ("noslip", "isisland"):
-        with pytest.raises(ValueError, match=k):
-            Batch([dict(nx=32, ny=32, geometry="closed", **{k: True})])
