@@ -1620,40 +1620,18 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
   LDSP unsigned char* lds = (LDSP unsigned char*)lds_raw;
   const int ti = task_index();
   const Task tk = tasks[ti];
-  if (skip(dyn, tk.inst, pred)) return;
   const InstD& in = I[tk.inst];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int k0 = in.nfine, deep = in.nlev - 1, nl = deep - k0 + 1;
   __shared__ CL<C, T> Ls[MAXLEV];
-  // carve LDS, 16-byte aligned regions: per level A (5 planes, pool type),
-  // x, b, t (FP64); then masks
+  // LDS: per level A (5 planes, pool type), x, b, t (FP64), 16-byte
+  // aligned; then the masks.  Every thread computes the layout (thread 0
+  // publishes it for the phases) and issues its share of the operator and
+  // mask DMA (global_load_lds, 16 bytes per lane, 1 KB per wavefront; the
+  // pools carry slack past their last array for the rounded-up tail) and
+  // of the rhs loads before the step gate is read, so the gate's round trip
+  // overlaps the transfers.
   auto r16 = [](unsigned v) { return (v + 15u) & ~15u; };
-  if (tid == 0) {
-    unsigned off = 0;
-    for (int l = 0; l < nl; l++) {
-      const LevD& G = in.L[k0 + l];
-      const unsigned sz = (unsigned)(G.m * G.n);
-      Ls[l].m = G.m;
-      Ls[l].n = G.n;
-      Ls[l].A = (LDSP T*)(lds + off);
-      off += r16(sizeof(T) * 5 * sz);
-      Ls[l].x = (LDSP C*)(lds + off);
-      off += sizeof(C) * sz;
-      Ls[l].b = (LDSP C*)(lds + off);
-      off += sizeof(C) * sz;
-      Ls[l].t = (LDSP C*)(lds + off);
-      off += sizeof(C) * sz;
-    }
-    for (int l = 0; l < nl; l++) {
-      const LevD& G = in.L[k0 + l];
-      Ls[l].msk = (LDSP int8_t*)(lds + off);
-      off += r16(G.m * G.n);
-    }
-  }
-  __syncthreads();
-  // operators and masks straight to LDS (global_load_lds, 16 bytes per
-  // lane, 1 KB per wavefront; the pools carry slack past their last array
-  // for the rounded-up tail), all issued before one wait
   {
     const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
     auto dma = [&](LDSP void* dst, const void* src, int bytes) {
@@ -1664,26 +1642,53 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
                                            (LDSP void*)((LDSP char*)dst + c * 1024), 16, 0, 0);
       }
     };
+    unsigned off = 0;
+    LDSP C* b0 = nullptr;
     for (int l = 0; l < nl; l++) {
       const LevD& G = in.L[k0 + l];
-      const int sz = G.m * G.n;
-      dma(Ls[l].A, Q + G.A, (int)(5 * sz * sizeof(T)));
-      dma(Ls[l].msk, B + G.msk, sz);
+      const unsigned sz = (unsigned)(G.m * G.n);
+      LDSP T* A = (LDSP T*)(lds + off);
+      off += r16(sizeof(T) * 5 * sz);
+      LDSP C* x = (LDSP C*)(lds + off);
+      off += sizeof(C) * sz;
+      LDSP C* bl = (LDSP C*)(lds + off);
+      off += sizeof(C) * sz;
+      LDSP C* t = (LDSP C*)(lds + off);
+      off += sizeof(C) * sz;
+      if (l == 0) b0 = bl;
+      if (tid == 0) {
+        Ls[l].m = G.m;
+        Ls[l].n = G.n;
+        Ls[l].A = A;
+        Ls[l].x = x;
+        Ls[l].b = bl;
+        Ls[l].t = t;
+      }
+      dma(A, Q + G.A, (int)(5 * sz * sizeof(T)));
+      for (unsigned q = tid; q < sz; q += nt) t[q] = C(0);
+    }
+    for (int l = 0; l < nl; l++) {
+      const LevD& G = in.L[k0 + l];
+      LDSP int8_t* mk = (LDSP int8_t*)(lds + off);
+      if (tid == 0) Ls[l].msk = mk;
+      off += r16(G.m * G.n);
+      dma(mk, B + G.msk, G.m * G.n);
+    }
+    if (skip(dyn, tk.inst, pred)) {
+      __builtin_amdgcn_s_waitcnt(0);  // no transfer into LDS outlives the block
+      return;
     }
     // the top level's rhs, widened to FP64
-    const int sz0 = Ls[0].m * Ls[0].n;
-    const T* bg = Q + in.L[k0].b;
+    const LevD& G0 = in.L[k0];
+    const int sz0 = G0.m * G0.n;
+    const T* bg = Q + G0.b;
     for (int base = tid; base < sz0; base += 8 * nt) {
       T v[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) v[k] = bg[min(base + k * nt, sz0 - 1)];
 #pragma unroll
       for (int k = 0; k < 8; k++)
-        if (base + k * nt < sz0) Ls[0].b[base + k * nt] = (C)v[k];
-    }
-    for (int l = 0; l < nl; l++) {
-      const int sz = Ls[l].m * Ls[l].n;
-      for (int q = tid; q < sz; q += nt) Ls[l].t[q] = C(0);
+        if (base + k * nt < sz0) b0[base + k * nt] = (C)v[k];
     }
     __builtin_amdgcn_s_waitcnt(0);
   }
