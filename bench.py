@@ -41,7 +41,8 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 HEADLINE = "euler-closed-1024"
 BOUSSINESQ = "boussinesq-rt-2048x1024"
 KERNEL_NAMES = {"smooth": "k_smooth", "resid_restrict": "k_resid_restrict", "rhs": "k_rhs",
-                "coarse": "k_coarse", "persist": "k_persist", "smooth_rr": "k_smooth_rr"}
+                "coarse": "k_coarse", "persist": "k_persist", "smooth_rr": "k_smooth_rr",
+                "up2": "k_up2"}
 
 
 def parse():
@@ -169,7 +170,7 @@ def roofline(b, nsteps=2):
     The one-workgroup coarse kernel is latency-bound, not HBM-bound: its
     share is reported beside the others."""
     shares = {}
-    for kind in ("smooth", "resid_restrict", "rhs", "coarse", "persist", "smooth_rr"):
+    for kind in ("smooth", "resid_restrict", "rhs", "coarse", "persist", "smooth_rr", "up2"):
         pr = b.profile(kind, -1, nsteps)
         shares[kind] = pr["avg_ms"] * pr["launches"] / nsteps
     dom = max(("smooth", "resid_restrict", "rhs"), key=shares.get)

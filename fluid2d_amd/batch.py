@@ -291,7 +291,7 @@ class Batch:
         return bool(d.value)
 
     KINDS = {"smooth": 0, "resid_restrict": 1, "rhs": 2, "coarse": 3, "persist": 4,
-             "smooth_rr": 5}
+             "smooth_rr": 5, "up2": 6}
 
     def profile(self, kind, level=-1, nsteps=2):
         """Time every launch of ``kind`` at ``level`` over ``nsteps`` eager

@@ -86,6 +86,10 @@ struct Task {
   int m, n, m2, n2;
   long long b, A, msk, msk2;  // level rhs, operator (T pool), mask; coarser mask (byte pool)
   long long sb;               // finest-level stencil bits (byte pool), -1 where the level has none
+  // two-level up-leg tasks (k_up2): the coarser level's input x, rhs and
+  // operator, the next coarser level's x, mask and size
+  long long xin2, b2, A2, xcc, msk3;
+  int m3, n3;
 };
 
 }  // namespace eng

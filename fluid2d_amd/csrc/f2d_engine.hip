@@ -1125,6 +1125,210 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
 }
 
 // ---------------------------------------------------------------------------
+// k_up2: two consecutive up-leg smoothings of a V-cycle (hierarchy.py:
+// 123-126 on levels lc = lev + 1 and lev, both below the finest) in one
+// launch.  A task is a tile of level lev; its workgroup computes the
+// coarse level's x + I(x of lc + 1) and two sweeps on the coarse region its
+// own prolongation reads (plus the rings the sweeps need), stores the
+// coarse cells its tile lies over (+ images), then the fine level's
+// x + I(coarse x) and two sweeps on the tile (+ images).  Neighbouring
+// tiles recompute the overlap of their coarse regions with the same
+// operands in the same order.  As in k_smooth_rr, every position is taken
+// at its periodic image (below the finest level every array is stored with
+// its images), so the values equal the launch-per-level ones bit for bit.
+// Tile origins are odd (NH plus multiples of the even tile sizes): fine
+// region row rr lies over coarse region row (rr + 1) / 2, and so on down.
+// ---------------------------------------------------------------------------
+template <typename T, int TYV>
+__global__ void __launch_bounds__(256) k_up2(const Task* __restrict__ tasks,
+                                             const InstD* __restrict__ I, T* Q,
+                                             const int8_t* __restrict__ B,
+                                             const DynD* __restrict__ dyn, int pred) {
+  static_assert(TYV == 4 || TYV == 8, "tile rows");
+  constexpr int NT = 256;
+  constexpr int XH = TYV + 4, XW = TX + 4, JH = TYV + 2, JW = TX + 2;  // fine x0, first sweep
+  constexpr int CH = TYV / 2 + 4, CW = TX / 2 + 4;  // coarse x the prolongation reads
+  constexpr int C1H = CH + 2, C1W = CW + 2;          // coarse first sweep
+  constexpr int C2H = CH + 4, C2W = CW + 4;          // coarse x0
+  constexpr int DH = C2H / 2 + 2, DW = C2W / 2 + 2;  // next coarser x under the coarse x0
+  constexpr int ND = DH * DW, NC2 = C2H * C2W, NC1 = C1H * C1W, NX = XH * XW, NJ = JH * JW;
+  constexpr int ITD = (ND + NT - 1) / NT, ITC2 = (NC2 + NT - 1) / NT, ITC1 = (NC1 + NT - 1) / NT;
+  constexpr int ITX = (NX + NT - 1) / NT, ITJ = (NJ + NT - 1) / NT;
+  __shared__ T xd[DH][DW];
+  __shared__ int8_t md[DH][DW];
+  __shared__ T x0c[C2H][C2W];
+  __shared__ int8_t mc[C2H][C2W];
+  __shared__ T bc[C1H][C1W];
+  __shared__ uint32_t ic[C1H][C1W];  // image cell index (coarse level)
+  __shared__ T j1c[C1H][C1W];
+  __shared__ T cx[CH][CW];
+  __shared__ T xs[XH][XW];
+  __shared__ int8_t mf[XH][XW];
+  __shared__ T bs[JH][JW];
+  __shared__ uint32_t jf[JH][JW];  // image cell index (fine level)
+  __shared__ T j1[JH][JW];
+  const int ti = task_index();
+  const Task tk = tasks[ti];
+  const InstD& in = I[tk.inst];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  const int m = tk.m, n = tk.n, m2 = tk.m2, n2 = tk.n2, m3 = tk.m3, n3 = tk.n3;
+  const int y0 = tk.y0, x0 = tk.x0;
+  const int cy0 = y0 >> 1, cx0 = x0 >> 1;  // coarse region origin
+  const int dy0 = cy0 >> 1, dx0 = cx0 >> 1;  // next coarser region origin (of coarse row cy0 - 2)
+  const int8_t* mskf = B + tk.msk;
+  const int8_t* mskc = B + tk.msk2;
+  const int8_t* mskd = B + tk.msk3;
+  // every load of the tile issued before the first is used
+  T vd[ITD], vc2[ITC2], vb1[ITC1], vx[ITX], vj[ITJ];
+  int8_t wd[ITD], wc2[ITC2], wx[ITX];
+  uint32_t uc1[ITC1], uj[ITJ];
+#pragma unroll
+  for (int it = 0; it < ITD; it++) {
+    const int q = min(tid + it * NT, ND - 1), a = q / DW, c = q - a * DW;
+    const uint32_t e = at32(wrap_img(dy0 + a, m3), wrap_img(dx0 + c, n3), n3);
+    vd[it] = ld32(Q + tk.xcc, e);
+    wd[it] = ld32(mskd, e);
+  }
+#pragma unroll
+  for (int it = 0; it < ITC2; it++) {
+    const int q = min(tid + it * NT, NC2 - 1), a = q / C2W, c = q - a * C2W;
+    const uint32_t e = at32(wrap_img(cy0 - 2 + a, m2), wrap_img(cx0 - 2 + c, n2), n2);
+    vc2[it] = ld32(Q + tk.xin2, e);
+    wc2[it] = ld32(mskc, e);
+  }
+#pragma unroll
+  for (int it = 0; it < ITC1; it++) {
+    const int q = min(tid + it * NT, NC1 - 1), a = q / C1W, c = q - a * C1W;
+    uc1[it] = at32(wrap_img(cy0 - 1 + a, m2), wrap_img(cx0 - 1 + c, n2), n2);
+    vb1[it] = ld32(Q + tk.b2, uc1[it]);
+  }
+#pragma unroll
+  for (int it = 0; it < ITX; it++) {
+    const int q = min(tid + it * NT, NX - 1), a = q / XW, c = q - a * XW;
+    const uint32_t e = at32(wrap_img(y0 - 2 + a, m), wrap_img(x0 - 2 + c, n), n);
+    vx[it] = ld32(Q + tk.xin, e);
+    wx[it] = ld32(mskf, e);
+  }
+#pragma unroll
+  for (int it = 0; it < ITJ; it++) {
+    const int q = min(tid + it * NT, NJ - 1), a = q / JW, c = q - a * JW;
+    uj[it] = at32(wrap_img(y0 - 1 + a, m), wrap_img(x0 - 1 + c, n), n);
+    vj[it] = ld32(Q + tk.b, uj[it]);
+  }
+  if (skip(dyn, tk.inst, pred)) return;
+#pragma unroll
+  for (int it = 0; it < ITD; it++) {
+    const int q = tid + it * NT, a = q / DW, c = q - a * DW;
+    if (q < ND) {
+      xd[a][c] = vd[it];
+      md[a][c] = wd[it];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITC2; it++) {
+    const int q = tid + it * NT, a = q / C2W, c = q - a * C2W;
+    if (q < NC2) mc[a][c] = wc2[it];
+  }
+#pragma unroll
+  for (int it = 0; it < ITC1; it++) {
+    const int q = tid + it * NT, a = q / C1W, c = q - a * C1W;
+    if (q < NC1) {
+      bc[a][c] = vb1[it];
+      ic[a][c] = uc1[it];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITX; it++) {
+    const int q = tid + it * NT, a = q / XW, c = q - a * XW;
+    if (q < NX) mf[a][c] = wx[it];
+  }
+#pragma unroll
+  for (int it = 0; it < ITJ; it++) {
+    const int q = tid + it * NT, a = q / JW, c = q - a * JW;
+    if (q < NJ) {
+      bs[a][c] = vj[it];
+      jf[a][c] = uj[it];
+    }
+  }
+  __syncthreads();
+  // prolongation at region position (rr, cc) from the coarser tile X with
+  // its mask M held at offset mo (ip_eval: interp_at's expressions)
+  auto ip = [&](auto& X, auto& M, int mo, int8_t m1, int rr, int cc) {
+    const int a = (rr + 1) >> 1, b = (cc + 1) >> 1;
+    IpOps<T> o;
+    o.a = X[a][b];
+    o.b = X[a][b + 1];
+    o.c = X[a + 1][b];
+    o.d = X[a + 1][b + 1];
+    o.sa = M[a + mo][b + mo];
+    o.sb = M[a + mo][b + 1 + mo];
+    o.sc = M[a + 1 + mo][b + mo];
+    o.sd = M[a + 1 + mo][b + 1 + mo];
+    o.m1 = m1;
+    o.par = ((cc + 1) & 1) | (((rr + 1) & 1) << 1);
+    return ip_eval<T>(o);
+  };
+  // coarse x0 = x + I(x of the next coarser level)
+#pragma unroll
+  for (int it = 0; it < ITC2; it++) {
+    const int q = tid + it * NT, a = q / C2W, c = q - a * C2W;
+    if (q < NC2) x0c[a][c] = vc2[it] + ip(xd, md, 0, mc[a][c], a, c);
+  }
+  __syncthreads();
+  const T c1 = T(in.omega), c2 = T(1. - in.omega);
+  const T* Ac = Q + tk.A2;
+  const T* Af = Q + tk.A;
+  const size_t PLc = (size_t)m2 * n2, PLf = (size_t)m * n;
+  auto Aat = [&](const T* A, size_t PL, int nn, uint32_t e) {
+    const APlanes<T> Ap{A, PL, e};
+    return [=](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * nn + di); };
+  };
+  // coarse first sweep (region less one ring of x0), second sweep on the
+  // prolongation's region; the cells the tile lies over are stored
+  for (int q = tid; q < NC1; q += NT) {
+    const int a = q / C1W, c = q - a * C1W;
+    j1c[a][c] = jacobi_2d<T>(mc[a + 1][c + 1], bc[a][c], c1, c2, Aat(Ac, PLc, n2, ic[a][c]),
+                             [&](int dj, int di) { return x0c[a + 1 + dj][c + 1 + di]; });
+  }
+  __syncthreads();
+  T* xoc = Q + tk.xc;
+  for (int q = tid; q < CH * CW; q += NT) {
+    const int a = q / CW, c = q - a * CW;
+    const T v = jacobi_2d<T>(mc[a + 2][c + 2], bc[a + 1][c + 1], c1, c2,
+                             Aat(Ac, PLc, n2, ic[a + 1][c + 1]),
+                             [&](int dj, int di) { return j1c[a + 1 + dj][c + 1 + di]; });
+    cx[a][c] = v;
+    const int J = cy0 + a, Ic = cx0 + c;
+    if (a >= 2 && a < 2 + TYV / 2 && c >= 2 && c < 2 + TX2 && J < m2 - NH && Ic < n2 - NH)
+      store_img_mask32<T>(xoc, nullptr, J, Ic, v, m2, n2);
+  }
+  __syncthreads();
+  // fine x0 = x + I(coarse x)
+#pragma unroll
+  for (int it = 0; it < ITX; it++) {
+    const int q = tid + it * NT, a = q / XW, c = q - a * XW;
+    if (q < NX) xs[a][c] = vx[it] + ip(cx, mc, 2, mf[a][c], a, c);  // cx = mc's region + 2
+  }
+  __syncthreads();
+  for (int q = tid; q < NJ; q += NT) {
+    const int a = q / JW, c = q - a * JW;
+    j1[a][c] = jacobi_2d<T>(mf[a + 1][c + 1], bs[a][c], c1, c2, Aat(Af, PLf, n, jf[a][c]),
+                            [&](int dj, int di) { return xs[a + 1 + dj][c + 1 + di]; });
+  }
+  __syncthreads();
+  T* xo = Q + tk.xout;
+  for (int q = tid; q < TYV * TX; q += NT) {
+    const int a = q / TX, c = q - a * TX;
+    const int J = y0 + a, Ic = x0 + c;
+    if (J >= m - NH || Ic >= n - NH) continue;
+    const T v = jacobi_2d<T>(mf[a + 2][c + 2], bs[a + 1][c + 1], c1, c2,
+                             Aat(Af, PLf, n, jf[a + 1][c + 1]),
+                             [&](int dj, int di) { return j1[a + 1 + dj][c + 1 + di]; });
+    store_img_mask32<T>(xo, nullptr, J, Ic, v, m, n);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_persist: a run of consecutive smoother and residual/restriction phases
 // of the multigrid levels below the finest (the tile lists of what would
 // otherwise be one launch each), executed by one launch of G co-resident
@@ -2287,7 +2491,8 @@ enum Kind {
   L_NSSRC,
   L_NSDOT,
   L_NSFIX,
-  L_FLATOPS
+  L_FLATOPS,
+  L_UP2
 };
 
 struct Launch {
@@ -2365,6 +2570,9 @@ struct Batch {
   // the same for the start level of an F-cycle's V-cycle below the finest
   // (entered from the prolongation)
   bool fuse_ip = true;
+  // two consecutive up-leg smoothings below the finest level in one launch
+  // (k_up2)
+  bool fuse_up = true;
   size_t max_shmem = 0;
   // persistent multigrid launches (tuning.persist workgroups per CU, 0 off)
   int persist = 0;
@@ -2590,13 +2798,39 @@ struct Builder {
       }
     }
     coarse(who, 0, pred);
+    // up-leg: levels below the finest two at a time (k_up2) where the
+    // instance has both, the finer one is not the masked start level and
+    // the finer level's launch is small (at most UP2_TILES tiles): there a
+    // pair costs less than its two latency-bound launches (9.0 against
+    // 5.3 + 5.3 us at 64^2 + 128^2); on larger levels the recomputed coarse
+    // regions cost more than the launch they save (14.7 against 5.3 + 7.7 us
+    // at 256^2 + 512^2)
+    constexpr double UP2_TILES = 256.;
+    std::vector<int> paired(b.ninst, -1);
     for (int dd = 0; dd <= Dmax; dd++) {
       const int ty = pick_ty(who, start, dd);
-      std::vector<Task> ts;
+      int tyf = dd < Dmax ? pick_ty(who, start, dd + 1) : 0;
+      if (tyf) {
+        double cells = 0.;
+        for (int i : who) {
+          const int D = b.H[i].d.nfine - 1 - start[i];
+          if (dd + 1 > D) continue;
+          const LevD& L = b.H[i].d.L[b.H[i].d.nfine - 2 - dd];
+          cells += (double)(L.m - 2 * NH) * (L.n - 2 * NH);
+        }
+        if (cells / ((double)TX * tyf) > UP2_TILES) tyf = 0;
+      }
+      std::vector<Task> ts, ps;
       for (int i : who) {
         const int D = b.H[i].d.nfine - 1 - start[i];
-        if (dd > D) continue;
+        if (dd > D || paired[i] == dd) continue;
         const int l = b.H[i].d.nfine - 1 - dd;
+        if (b.fuse_up && dd < D && l - 1 >= 1 && !(final_top && dd + 1 == D) &&
+            (tyf == 4 || tyf == 8)) {
+          up2_task(i, l - 1, ps, tyf);
+          paired[i] = dd + 1;
+          continue;
+        }
         smooth_task(i, l, 2, (final_top && dd == D) ? 1 : 0, ts, ty);
       }
       if (!ts.empty()) {
@@ -2605,7 +2839,30 @@ struct Builder {
         Ls.pred = pred;
         Ls.ty = ty;
       }
+      if (!ps.empty()) {
+        Launch& Lp = add(L_UP2, ps);
+        Lp.pred = pred;
+        Lp.ty = tyf;
+      }
     }
+  }
+
+  // up-leg smoothings of levels lf + 1 and lf of instance i in one task list
+  // (k_up2; mode 2 on both, unmasked)
+  void up2_task(int i, int lf, std::vector<Task>& out, int ty) {
+    HostInst& h = b.H[i];
+    const int lc = lf + 1;
+    Task t = mk_task(i, lf, 0, 0);
+    t.mode = 2;
+    t.xin2 = h.buf[lc][h.cur[lc]];
+    t.xc = h.buf[lc][h.cur[lc] ^ 1];  // the coarse result, the fine prolongation's source
+    t.xcc = h.buf[lc + 1][h.cur[lc + 1]];
+    h.cur[lc] ^= 1;
+    t.xin = h.buf[lf][h.cur[lf]];
+    t.xout = h.buf[lf][h.cur[lf] ^ 1];
+    h.cur[lf] ^= 1;
+    const LevD& L = lev(i, lf);
+    tiles(out, t, NH, L.m - NH, NH, L.n - NH, ty);
   }
 
   void coarse(const std::vector<int>& who, int mode, int pred) {
@@ -3229,6 +3486,12 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
                      b.dD, BS, b.fault);
         break;
       }
+      case L_UP2:
+        if (L.ty == 4)
+          F2D_LAUNCH((k_up2<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        else
+          F2D_LAUNCH((k_up2<T, 8>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        break;
       case L_NSSRC:
         F2D_LAUNCH(k_ns_source, dim3(g), B2, 0, b.s, TK, b.dI, P, b.dD);
         break;
@@ -3294,6 +3557,15 @@ static void launch_bytes(const Batch& b, const Launch& L, double* alg, double* i
       a += cells * (wrd + 1.) * tz;               // + x written
       const bool fly = t.lev == 0 && d.a0fly;
       im += cells * ((wrd + 1.) * tz + 1. + (fly ? 1. : 5. * tz) + (t.mode >= 2 ? 0.25 * tz : 0.));
+    } else if (L.kind == L_UP2) {
+      // two smoothings in mode 2: x and b read, x written, on the tile's
+      // fine cells and on the coarse cells it lies over (+ the next coarser
+      // x under those)
+      const int cj = std::min(L.ty, lv.m - NH - t.y0), ci = std::min(TX, lv.n - NH - t.x0);
+      const double cells = (double)std::max(cj, 0) * std::max(ci, 0);
+      const double cc = 0.25 * cells;
+      a += (cells + cc) * 3. * tz + 0.25 * cc * tz;
+      im += (cells + cc) * (3. * tz + 1. + 5. * tz) * 1.3 + 0.25 * cc * (tz + 1.);
     } else if (L.kind == L_SMOOTH_RR) {
       // rhs read and smoothed x written on the tile's fine cells, the
       // restricted residual written on its coarse cells
@@ -3535,6 +3807,12 @@ static int build_programs(Batch& b) {
     t.n2 = cl ? d.L[t.lev + 1].n : 0;
     t.msk2 = cl ? d.L[t.lev + 1].msk : 0;
     t.sb = (t.lev == 0 && d.a0fly) ? d.sb0 : -1;
+    const bool cl2 = t.lev + 2 < d.nlev;
+    t.b2 = cl ? d.L[t.lev + 1].b : 0;
+    t.A2 = cl ? d.L[t.lev + 1].A : 0;
+    t.msk3 = cl2 ? d.L[t.lev + 2].msk : 0;
+    t.m3 = cl2 ? d.L[t.lev + 2].m : 0;
+    t.n3 = cl2 ? d.L[t.lev + 2].n : 0;
   }
   ECHK(hipMemcpy(b.dT, b.tasks.data(), b.tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
   if (b.dPh) (void)hipFree(b.dPh);
@@ -4258,8 +4536,8 @@ int f2d_batch_query(f2d_batch* b, int* done) {
 
 int f2d_batch_profile(f2d_batch* b, int kind, int level, int nsteps, double* avg_ms,
                       long long* nlaunch, double* alg_bytes, double* impl_bytes) {
-  const int kinds[6] = {L_SMOOTH, L_RR, L_RHS, L_COARSE, L_PERSIST, L_SMOOTH_RR};
-  if (kind < 0 || kind > 5) return efail(F2D_E_ARG, "kind 0..5");
+  const int kinds[7] = {L_SMOOTH, L_RR, L_RHS, L_COARSE, L_PERSIST, L_SMOOTH_RR, L_UP2};
+  if (kind < 0 || kind > 6) return efail(F2D_E_ARG, "kind 0..6");
   ProfSel ps;
   ps.kind = kinds[kind];
   ps.level = level;

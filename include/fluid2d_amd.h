@@ -388,7 +388,8 @@ int f2d_batch_synchronize(f2d_batch *b);
 /* done = 1 when all work enqueued on the batch's stream has completed */
 int f2d_batch_query(f2d_batch *b, int *done);
 /* Runs nsteps steps eagerly; every launch of kind (0 smoother, 1
- * residual/restriction, 2 advection, 3 coarse levels) at top-down level
+ * residual/restriction, 2 advection, 3 coarse levels, 4 persistent
+ * multigrid, 5 fused down-leg, 6 two-level up-leg) at top-down level
  * `level` (-1: any) is issued 8 times back to back between a pair of HIP
  * events (the repeats leave the state as one launch does).  avg_ms: mean
  * duration per launch (elapsed / 8); alg_bytes / impl_bytes: mean
