@@ -375,6 +375,33 @@ __global__ void __launch_bounds__(256) k_c2c(const Task* __restrict__ tasks,
 // One tile (task ti of `tasks`) of a (64, 4) workgroup; the kernel k_smooth
 // below runs one tile per workgroup, the persistent kernel k_persist any
 // number of them one after another.
+// Phase timestamps of the 64 x 16 smoother tiles (profiling build only,
+// make stamps): thread 0 of each of the first 2048 workgroups of a launch
+// records the real-time clock (100 MHz) at the start, after the tile loads
+// are issued and the step gate read, after the tile is in LDS, after the
+// prolongation, after each sweep; slot 7 holds the compute unit and the
+// tile.  Each launch overwrites the previous one's records.
+#ifdef F2D_PHASE_STAMPS
+__device__ unsigned long long smooth_stamps[2048 * 8];
+#define F2D_STAMP(k)                                                               \
+  do {                                                                             \
+    if (TYV == 16 && threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x < 2048)     \
+      smooth_stamps[blockIdx.x * 8 + (k)] = (unsigned long long)wall_clock64();      \
+  } while (0)
+#define F2D_STAMP_SYNC(k) \
+  do {                    \
+    __syncthreads();      \
+    F2D_STAMP(k);         \
+  } while (0)
+#else
+#define F2D_STAMP(k) \
+  do {               \
+  } while (0)
+#define F2D_STAMP_SYNC(k) \
+  do {                    \
+  } while (0)
+#endif
+
 template <typename T, int TYV>
 __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, const int ti,
                                             const InstD* __restrict__ I, T* Q,
@@ -393,10 +420,16 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
   __shared__ T j1[JH][JW];
   __shared__ T bs[JH][JW];         // right-hand side on the first sweep's region
   __shared__ uint8_t sbs[XH][XW];  // stencil bits (finest level)
+  F2D_STAMP(0);
   const Task tk = tasks[ti];
   // the skip test is read and taken after the tile loads are issued (the
   // task holds the addressing, so the data loads follow the task read)
   const int MODE = tk.mode;
+#ifdef F2D_PHASE_STAMPS
+  if (TYV == 16 && threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x < 2048)
+    smooth_stamps[blockIdx.x * 8 + 7] =
+        ((unsigned long long)__smid() << 32) | (unsigned)(MODE | (tk.y0 << 4) | (tk.x0 << 16));
+#endif
   const int final_mask = tk.flags & 1;
   const InstD& in = I[tk.inst];
   const int m = tk.m, n = tk.n;
@@ -510,6 +543,7 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
     }
   }
   if (skip(dyn, tk.inst, pred)) return;
+  F2D_STAMP(1);
   bool full = true;
   if (lean) {
 #pragma unroll
@@ -592,7 +626,9 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
     }
   }
   // every cell of the tile has the full 9-point stencil: constant weights
+  F2D_STAMP_SYNC(2);
   full_tile = block_all4(fly && full);
+  F2D_STAMP(3);
   const T c1 = T(in.omega), c2 = T(1. - in.omega);
   T* xout = Q + tk.xout;
   const int8_t* mp = final_mask ? B + in.mskp : nullptr;
@@ -648,6 +684,7 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
       j1[ber][bec] = Jac(obe, ber + 1, bec + 1, be,
                          [&](int dj, int di) { return xs[ber + 1 + dj][bec + 1 + di]; });
     __syncthreads();
+    F2D_STAMP(4);
 #pragma unroll
     for (int k = 0; k < IT3; k++) {
       const int rr = ty + 4 * k;
@@ -659,6 +696,7 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
       else
         store_img_mask32<T>(xout, mp, y0 + rr, Ic, v, m, n);
     }
+    F2D_STAMP_SYNC(5);
   };
   auto run = [&](auto Jac) {
     if (lean)
@@ -4370,6 +4408,15 @@ int f2d_batch_set_noslip(f2d_batch* b, int inst, const int8_t* msknoslip, const 
   d.y2bc = y2bc;
   return reprogram(*b);
 }
+
+#ifdef F2D_PHASE_STAMPS
+// the phase timestamps of the last 64 x 16 smoother launch (2048 x 8)
+int f2d_debug_smooth_stamps(unsigned long long* host) {
+  ECHK(hipMemcpyFromSymbol(host, HIP_SYMBOL(f2d::eng::smooth_stamps),
+                           sizeof(unsigned long long) * 2048 * 8));
+  return F2D_OK;
+}
+#endif
 
 int f2d_batch_corner_mask(f2d_batch* b, int inst, int8_t* host) {
   if (inst < 0 || inst >= b->ninst) return efail(F2D_E_ARG, "instance out of range");

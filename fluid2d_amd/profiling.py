@@ -7,6 +7,7 @@ roofline.
     python -m fluid2d_amd.profiling pmc DB COUNTER [PATTERN]
     python -m fluid2d_amd.profiling traffic FETCH_DB WRITE_DB KERNEL GRID_X OUT_JSON [TUNING_JSON]
     python -m fluid2d_amd.profiling sq DB [ROWS]
+    python -m fluid2d_amd.profiling stamps [WORKLOAD]
 
 ``traffic`` scales the per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the
 kernel launches with grid size GRID_X by what the same counters read for
@@ -15,6 +16,12 @@ copy,...``: 5 passes of the library's celltocorner kernel over 256 MiB
 arrays, a streaming kernel with plain global loads; the runtime's copy
 kernel is timed beside it).  The entry for KERNEL is added to OUT_JSON
 (a dict keyed by kernel name), keeping the entries of other kernels.
+
+``stamps`` loads the profiling build of the library (``make -C
+fluid2d_amd/csrc stamps``: per-workgroup phase timestamps in the 64 x 16
+smoother tiles), runs three plain FP64 steps of WORKLOAD (default
+euler-closed-1024) and prints the phase timeline of the last finest-level
+smoother launch.
 """
 import json
 import sqlite3
@@ -106,7 +113,59 @@ def sq_table(path, wave_cycle_rate=None):
     return rows
 
 
+def stamps(workload="euler-closed-1024"):
+    """Phase timeline of the last 64 x 16 smoother launch (profiling build):
+    per-phase durations (us, 100 MHz clock) over the workgroups, the end
+    times, and wall tiles against interior tiles."""
+    import ctypes as C
+    import os
+
+    import numpy as np
+
+    from . import _lib
+    _lib.LIB_PATH = os.path.join(os.path.dirname(_lib.LIB_PATH), "libf2dgpu_stamps.so")
+    from . import workloads
+    from .batch import Batch
+    from .grid import Grid
+    p = workloads.make_param(workload)
+    g = Grid(p)
+    tun = {"coarse_lds_budget": 159744, "min_tiles": 1024, "coarse_wave_cells": 0, "persist": 0}
+    b = Batch([(p, g)], precision="fp64", tuning=tun)
+    for k, v in workloads.initial_fields(workload, g).items():
+        b.set_field(0, k, v)
+    b.set_psi_from_vorticity()
+    b.set_dt(workloads.time_step(workload, g, b.diagnostics(0)["maxspeed"]))
+    b.step(3)
+    lib = _lib.load()
+    buf = (C.c_ulonglong * (2048 * 8))()
+    if lib.f2d_debug_smooth_stamps(buf) != 0:
+        raise RuntimeError("f2d_debug_smooth_stamps failed")
+    a = np.frombuffer(buf, dtype=np.uint64).reshape(2048, 8).astype(np.int64)
+    a = a[a[:, 0] > 0]
+    t = (a[:, :6] - a[:, 0].min()) * 0.01
+    info = a[:, 7]
+    y0, x0 = (info >> 4) & 0xFFF, (info >> 16) & 0xFFFF
+    m, n = g.nyl, g.nxl
+    wall = (y0 <= 3) | (y0 + 16 >= m - 3) | (x0 <= 3) | (x0 + 64 >= n - 3)
+    print("workgroups %d on %d compute units, mode %s" % (len(a), len(np.unique(info >> 32)),
+                                                          np.unique(info & 15).tolist()))
+    names = ["start", "tile loads issued, gate read", "tile in LDS", "prolongation",
+             "first sweep", "second sweep, stores"]
+    print("%-30s %7s %7s %7s %7s" % ("phase (us)", "mean", "p10", "p50", "p90"))
+    for k, nm in enumerate(names):
+        v = t[:, 0] if k == 0 else t[:, k] - t[:, k - 1]
+        print("%-30s %7.2f %7.2f %7.2f %7.2f" % (nm, v.mean(), *np.percentile(v, [10, 50, 90])))
+    e = t[:, 5]
+    print("workgroup end (us): p10 %.2f p50 %.2f p90 %.2f max %.2f" %
+          (*np.percentile(e, [10, 50, 90]), e.max()))
+    print("wall tiles %d: mean end %.2f us; interior tiles: %.2f us" %
+          (wall.sum(), e[wall].mean(), e[~wall].mean()))
+
+
 def main(argv):
+    if argv and argv[0] == "stamps":
+        stamps(*argv[1:2])
+        return 0
     if len(argv) < 2:
         print(__doc__)
         return 2
