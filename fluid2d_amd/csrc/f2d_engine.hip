@@ -716,7 +716,7 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
     const T c3 = c1 / fabs(w[4]);
     run([&](size_t, int r, int q, T bc, auto X) {
       const T v = jacobi_c3_2d<T>(bc, c3, c2, ABitsTile2<T, XW>{&sbs[r][q], w}, X);
-      return ((sbs[r][q] >> 4) & 1) ? v : T(0);
+      return bit_select<T>(sbs[r][q], 4, v);
     });
   } else {
     run([&](size_t c, int, int, T bc, auto X) {

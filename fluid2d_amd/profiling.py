@@ -151,15 +151,21 @@ def stamps(workload="euler-closed-1024"):
                                                           np.unique(info & 15).tolist()))
     names = ["start", "tile loads issued, gate read", "tile in LDS", "prolongation",
              "first sweep", "second sweep, stores"]
-    print("%-30s %7s %7s %7s %7s" % ("phase (us)", "mean", "p10", "p50", "p90"))
+    print("%-30s %7s %7s %7s %7s %8s %8s" % ("phase (us)", "mean", "p10", "p50", "p90", "wall",
+                                              "interior"))
     for k, nm in enumerate(names):
         v = t[:, 0] if k == 0 else t[:, k] - t[:, k - 1]
-        print("%-30s %7.2f %7.2f %7.2f %7.2f" % (nm, v.mean(), *np.percentile(v, [10, 50, 90])))
+        print("%-30s %7.2f %7.2f %7.2f %7.2f %8.2f %8.2f" %
+              (nm, v.mean(), *np.percentile(v, [10, 50, 90]), v[wall].mean(), v[~wall].mean()))
     e = t[:, 5]
     print("workgroup end (us): p10 %.2f p50 %.2f p90 %.2f max %.2f" %
           (*np.percentile(e, [10, 50, 90]), e.max()))
     print("wall tiles %d: mean end %.2f us; interior tiles: %.2f us" %
           (wall.sum(), e[wall].mean(), e[~wall].mean()))
+    cu = info >> 32
+    last = np.argsort(-e)[:8]
+    print("last to end: " + ", ".join("(y0 %d, x0 %d, cu %x, %.2f us, %d on its cu, %d wall)" % (
+        y0[q], x0[q], cu[q], e[q], (cu == cu[q]).sum(), wall[cu == cu[q]].sum()) for q in last))
 
 
 def main(argv):
