@@ -1463,6 +1463,26 @@ struct CL {
   LDSP T *x, *b, *t;  // the residual reuses t (never live at the same time)
 };
 
+// Phase stamps of the coarse kernel (profiling build only, make stamps):
+// thread 0 of workgroup 0 records the shader clock and a phase code at the
+// end of each level visit of the last launch of each mode (stamps[mode]:
+// clock, code pairs; code 100 + k: the real-time clock, 100 MHz).
+#ifdef F2D_PHASE_STAMPS
+__device__ unsigned long long coarse_stamps[2][256];
+#define F2D_CSTAMP(sy, mode, code)                                                     \
+  do {                                                                                \
+    if ((sy).tid == 0 && blockIdx.x == 0 && (sy).ck < 127) {                            \
+      coarse_stamps[(mode)][2 * (sy).ck] = (unsigned long long)clock64();                \
+      coarse_stamps[(mode)][2 * (sy).ck + 1] = (unsigned long long)(code);               \
+      (sy).ck++;                                                                      \
+    }                                                                                 \
+  } while (0)
+#else
+#define F2D_CSTAMP(sy, mode, code) \
+  do {                             \
+  } while (0)
+#endif
+
 // Phases over a level: levels of at most SMALL cells run on wavefront 0
 // alone and end with a wavefront-level barrier (LDS operations of one
 // wavefront execute in order); larger levels use the whole workgroup and a
@@ -1472,6 +1492,7 @@ struct Sync {
   int tid, nt;
   int small_cells;  // levels of at most this many cells run on wavefront 0
   bool prev_small;
+  int ck;           // next phase stamp (profiling build)
   __device__ __forceinline__ bool begin(int cells) {
     const bool small = cells <= small_cells;
     if (!small && prev_small) __syncthreads();
@@ -1816,10 +1837,93 @@ __device__ __forceinline__ bool c_coarsest_wave(const CL<double, TA>& L, double 
   return true;
 }
 
+// a double from lane (lane moved by the DPP control CTRL) of the same row of
+// 16 lanes: two 32-bit DPP moves
+template <int CTRL>
+__device__ __forceinline__ double dpp_get(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// DPP controls on the 4 x 4 torus held by a row of 16 lanes, lane 4 r + c:
+// the value of cell (r - 1, c) (row rotated right by 4 lanes), (r + 1, c)
+// (by 12), (r, c - 1) and (r, c + 1) (quad permutes [3,0,1,2], [1,2,3,0])
+constexpr int DPP_N = 0x124, DPP_S = 0x12C, DPP_W = 0x93, DPP_E = 0x39;
+
+// The coarsest-level solve of a 4 x 4 interior on a row of 16 lanes, lane
+// 4 r + c holding interior cell (r, c) with its operator row, c1 / |A_5|,
+// rhs and mask.  Below the finest level every array holds periodic images
+// in its halo (operator, rhs, mask, and x as stored), so the first sweep's
+// value at a halo position is the value at its image: both sweeps are
+// sweeps on the 4 x 4 torus, and the nine neighbour values of a cell are
+// DPP lane moves within the row (a row rotation, then a quad permute for
+// the diagonals) instead of LDS permutes.  Same arithmetic as
+// c_coarsest_wave, sweep for sweep.  Each of the four rows of wavefront 0
+// computes the same torus; row 0 stores x (interior + images).
+template <typename TA>
+__device__ __forceinline__ bool c_coarsest_torus4(const CL<double, TA>& L, double c1, double c2,
+                                                  Sync& sy, int iters) {
+  typedef double T;
+  const int m = L.m, n = L.n;
+  if (m - 2 * NH != 4 || n - 2 * NH != 4) return false;
+  sy.prev_small = true;  // a workgroup-wide phase after this one synchronises first
+  if (sy.tid >= 64) return true;
+  const int lane = sy.tid & 15, P = m * n;
+  const int j = NH + (lane >> 2), i = NH + (lane & 3), c = j * n + i;
+  ARow<T> a;
+  a.a0 = T(L.A[c]);
+  a.a1 = T(L.A[P + c]);
+  a.a2 = T(L.A[2 * P + c]);
+  a.a3 = T(L.A[3 * P + c]);
+  a.a4 = T(L.A[4 * P + c]);
+  a.a3e = T(L.A[3 * P + c + 1]);
+  a.a2s = T(L.A[2 * P + c + n - 1]);
+  a.a1s = T(L.A[P + c + n]);
+  a.a0s = T(L.A[c + n + 1]);
+  const T c3 = c1 / fabs(a.a4);
+  const T bc = L.b[c];
+  const int8_t mk = L.msk[c];
+  // the nine torus neighbours of v, o[(dj + 1) * 3 + di + 1] = v(r + dj, c + di)
+  auto nbrs = [](T v, T* o) {
+    const T up = dpp_get<DPP_N>(v), dn = dpp_get<DPP_S>(v);
+    o[0] = dpp_get<DPP_W>(up);
+    o[1] = up;
+    o[2] = dpp_get<DPP_E>(up);
+    o[3] = dpp_get<DPP_W>(v);
+    o[4] = v;
+    o[5] = dpp_get<DPP_E>(v);
+    o[6] = dpp_get<DPP_W>(dn);
+    o[7] = dn;
+    o[8] = dpp_get<DPP_E>(dn);
+  };
+  T xv = T(0);
+  for (int it = 0; it < iters; it++) {
+    T v;
+    if (it == 0) {
+      v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int, int) { return T(0); });
+    } else {
+      T xn[9];
+      nbrs(xv, xn);
+      v = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return xn[(dj + 1) * 3 + di + 1]; });
+    }
+    const T tv = mk != 0 ? v : T(0);
+    T tn[9];
+    nbrs(tv, tn);
+    const T v2 = jacobi_c3_2d<T>(bc, c3, c2, a, [&](int dj, int di) { return tn[(dj + 1) * 3 + di + 1]; });
+    xv = mk == 0 ? T(0) : v2;
+  }
+  if (sy.tid < 16) store_img_mask<T>(L.x, nullptr, j, i, xv, m, n);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return true;
+}
+
 // the coarsest-level solve: 16 smoothings of a level entered with x = 0
 // (hierarchy.py:117-118 / 144-145)
 template <typename T, typename TA>
 __device__ __forceinline__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync& sy) {
+  if (c_coarsest_torus4(L, c1, c2, sy, 16)) return;
   if (c_coarsest_wave(L, c1, c2, sy, 16)) return;
   if (!c_smooth_own(L, c1, c2, sy, true, 16, false))
     for (int k = 0; k < 16; k++) c_smooth(L, c1, c2, sy, k == 0);
@@ -1829,7 +1933,7 @@ __device__ __forceinline__ void c_coarsest(const CL<T, TA>& L, T c1, T c2, Sync&
 // (hierarchy.py:98-126)
 template <typename T, typename TA>
 __device__ __forceinline__ void c_vcycle(const CL<T, TA>* Ls, int lev1, int deep, bool zero_first,
-                                         T c1, T c2, Sync& sy) {
+                                         T c1, T c2, Sync& sy, int mode) {
   for (int l = lev1; l < deep; l++) {
     const bool z = l > lev1 || zero_first;
     if (c_smooth_own(Ls[l], c1, c2, sy, z, 1, true)) {
@@ -1838,12 +1942,16 @@ __device__ __forceinline__ void c_vcycle(const CL<T, TA>* Ls, int lev1, int deep
       c_smooth(Ls[l], c1, c2, sy, z);
       c_resid_restrict(Ls[l], Ls[l + 1], false, sy);
     }
+    F2D_CSTAMP(sy, mode, 10 + l);
   }
   c_coarsest(Ls[deep], c1, c2, sy);
+  F2D_CSTAMP(sy, mode, 20);
   for (int l = deep - 1; l >= lev1; l--) {
     c_interp(Ls[l], Ls[l + 1], true, sy);
     c_smooth_level(Ls[l], c1, c2, sy, false);
+    F2D_CSTAMP(sy, mode, 30 + l);
   }
+  (void)mode;
 }
 
 constexpr int COARSE_THREADS = 1024;  // one workgroup per instance holds the CU's LDS anyway
@@ -1866,6 +1974,19 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
   const int tid = threadIdx.x, nt = blockDim.x;
   const int k0 = in.nfine, deep = in.nlev - 1, nl = deep - k0 + 1;
   __shared__ CL<C, T> Ls[MAXLEV];
+  Sync sy;
+  sy.tid = tid;
+  sy.nt = nt;
+  sy.small_cells = small_cells;
+  sy.prev_small = false;
+  sy.ck = 0;
+#ifdef F2D_PHASE_STAMPS
+  if (tid == 0 && blockIdx.x == 0) {
+    coarse_stamps[mode][254] = wall_clock64();
+    coarse_stamps[mode][255] = 100 + nl;
+  }
+#endif
+  F2D_CSTAMP(sy, mode, 0);
   // LDS: per level A (5 planes, pool type), x, b, t (FP64), 16-byte
   // aligned; then the masks.  Every thread computes the layout (thread 0
   // publishes it for the phases) and issues its share of the operator and
@@ -1935,26 +2056,34 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
     __builtin_amdgcn_s_waitcnt(0);
   }
   __syncthreads();
+  F2D_CSTAMP(sy, mode, 1);
   const C c1 = C(in.omega), c2 = C(1. - in.omega);
-  Sync sy;
-  sy.tid = tid;
-  sy.nt = nt;
-  sy.small_cells = small_cells;
-  sy.prev_small = false;
   if (mode == 0) {
-    c_vcycle(Ls, 0, nl - 1, true, c1, c2, sy);
+    c_vcycle(Ls, 0, nl - 1, true, c1, c2, sy, 0);
   } else {
-    for (int l = 0; l < nl - 1; l++) c_resid_restrict(Ls[l], Ls[l + 1], true, sy);
+    for (int l = 0; l < nl - 1; l++) {
+      c_resid_restrict(Ls[l], Ls[l + 1], true, sy);
+      F2D_CSTAMP(sy, 1, 40 + l);
+    }
     c_coarsest(Ls[nl - 1], c1, c2, sy);
+    F2D_CSTAMP(sy, 1, 20);
     for (int l = nl - 2; l >= 0; l--) {
       c_interp(Ls[l], Ls[l + 1], false, sy);
-      c_vcycle(Ls, l, nl - 1, false, c1, c2, sy);
+      F2D_CSTAMP(sy, 1, 50 + l);
+      c_vcycle(Ls, l, nl - 1, false, c1, c2, sy, 1);
     }
   }
   __syncthreads();
   T* xg = Q + tk.xout;
   const int sz = Ls[0].m * Ls[0].n;
   for (int q = tid; q < sz; q += nt) xg[q] = T(Ls[0].x[q]);
+  F2D_CSTAMP(sy, mode, 2);
+#ifdef F2D_PHASE_STAMPS
+  if (tid == 0 && blockIdx.x == 0) {
+    coarse_stamps[mode][252] = wall_clock64();
+    coarse_stamps[mode][253] = 102;
+  }
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -4414,6 +4543,12 @@ int f2d_batch_set_noslip(f2d_batch* b, int inst, const int8_t* msknoslip, const 
 int f2d_debug_smooth_stamps(unsigned long long* host) {
   ECHK(hipMemcpyFromSymbol(host, HIP_SYMBOL(f2d::eng::smooth_stamps),
                            sizeof(unsigned long long) * 2048 * 8));
+  return F2D_OK;
+}
+// the phase stamps of the last coarse-kernel launch of each mode (2 x 256)
+int f2d_debug_coarse_stamps(unsigned long long* host) {
+  ECHK(hipMemcpyFromSymbol(host, HIP_SYMBOL(f2d::eng::coarse_stamps),
+                           sizeof(unsigned long long) * 2 * 256));
   return F2D_OK;
 }
 #endif

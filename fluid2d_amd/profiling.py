@@ -8,6 +8,7 @@ roofline.
     python -m fluid2d_amd.profiling traffic FETCH_DB WRITE_DB KERNEL GRID_X OUT_JSON [TUNING_JSON]
     python -m fluid2d_amd.profiling sq DB [ROWS]
     python -m fluid2d_amd.profiling stamps [WORKLOAD]
+    python -m fluid2d_amd.profiling coarse [WORKLOAD]
 
 ``traffic`` scales the per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the
 kernel launches with grid size GRID_X by what the same counters read for
@@ -21,7 +22,9 @@ kernel is timed beside it).  The entry for KERNEL is added to OUT_JSON
 fluid2d_amd/csrc stamps``: per-workgroup phase timestamps in the 64 x 16
 smoother tiles), runs three plain FP64 steps of WORKLOAD (default
 euler-closed-1024) and prints the phase timeline of the last finest-level
-smoother launch.
+smoother launch.  ``coarse`` does the same for the coarse kernel: the
+shader-clock cycles of each level visit of its last launch of each mode
+(a V-cycle's tail and an F-cycle).
 """
 import json
 import sqlite3
@@ -168,9 +171,75 @@ def stamps(workload="euler-closed-1024"):
         y0[q], x0[q], cu[q], e[q], (cu == cu[q]).sum(), wall[cu == cu[q]].sum()) for q in last))
 
 
+def coarse_stamps(workload="euler-closed-1024"):
+    """Phase breakdown of the last coarse-kernel launch of each mode
+    (profiling build): the shader-clock cycles of each level visit, with the
+    launch's real-time length for the clock rate."""
+    import ctypes as C
+    import os
+
+    import numpy as np
+
+    from . import _lib
+    _lib.LIB_PATH = os.path.join(os.path.dirname(_lib.LIB_PATH), "libf2dgpu_stamps.so")
+    from . import workloads
+    from .batch import Batch
+    from .grid import Grid
+    p = workloads.make_param(workload)
+    g = Grid(p)
+    tun = {"coarse_lds_budget": 159744, "min_tiles": 1024, "coarse_wave_cells": 0, "persist": 0}
+    b = Batch([(p, g)], precision="fp64", tuning=tun)
+    for k, v in workloads.initial_fields(workload, g).items():
+        b.set_field(0, k, v)
+    b.set_psi_from_vorticity()
+    b.set_dt(workloads.time_step(workload, g, b.diagnostics(0)["maxspeed"]))
+    b.step(3)
+    lib = _lib.load()
+    buf = (C.c_ulonglong * 512)()
+    if lib.f2d_debug_coarse_stamps(buf) != 0:
+        raise RuntimeError("f2d_debug_coarse_stamps failed")
+    a = np.frombuffer(buf, dtype=np.uint64).reshape(2, 256).astype(np.int64)
+    names = {0: "start", 1: "operators in LDS, rhs widened", 2: "end (x stored)", 20: "coarsest solve"}
+    for mode in (0, 1):
+        r = a[mode]
+        nl = int(r[255] - 100)
+        wall_us = (r[252] - r[254]) * 0.01
+        pairs = [(int(r[2 * k]), int(r[2 * k + 1])) for k in range(126) if r[2 * k] > 0]
+        if not pairs:
+            continue
+        t0 = pairs[0][0]
+        cyc = pairs[-1][0] - t0
+        mhz = cyc / wall_us if wall_us > 0 else 0.0
+        print("mode %d (%s): %d levels, %.2f us, %d cycles (%.0f MHz)" % (
+            mode, "V-cycle tail" if mode == 0 else "F-cycle", nl, wall_us, cyc, mhz))
+        prev = t0
+        tot = {}
+        for c, code in pairs[1:]:
+            if code in names:
+                nm = names[code]
+            elif 10 <= code < 20:
+                nm = "down, level +%d (smooth, residual, restrict)" % (code - 10)
+            elif 30 <= code < 40:
+                nm = "up, level +%d (interpolate, smooth)" % (code - 30)
+            elif 40 <= code < 50:
+                nm = "restrict b, level +%d" % (code - 40)
+            else:
+                nm = "interpolate onto level +%d" % (code - 50)
+            tot[nm] = tot.get(nm, 0) + (c - prev)
+            if mode == 0:
+                print("  %-46s %7d cycles %6.2f us" % (nm, c - prev, (c - prev) / mhz if mhz else 0))
+            prev = c
+        if mode == 1:
+            for nm, v in tot.items():
+                print("  %-46s %7d cycles %6.2f us (sum)" % (nm, v, v / mhz if mhz else 0))
+
+
 def main(argv):
     if argv and argv[0] == "stamps":
         stamps(*argv[1:2])
+        return 0
+    if argv and argv[0] == "coarse":
+        coarse_stamps(*argv[1:2])
         return 0
     if len(argv) < 2:
         print(__doc__)
