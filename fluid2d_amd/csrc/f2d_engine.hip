@@ -1612,23 +1612,27 @@ struct ARow {
   }
 };
 
-// A cell of the first sweep's region held by its owning thread: position,
-// operator row, c1 / |A_5|, rhs and mask (read once per level visit)
+// An interior cell held by its owning thread: position, operator row,
+// c1 / |A_5|, rhs and mask (read once per level visit)
 template <typename T>
 struct OwnCell {
   int j, i;
   int c;
-  bool own, inner;
+  bool own;
   ARow<T> a;
   T c3, bc;
   int8_t mk;
 };
 
 // Level visits where each thread (of the workgroup, or of wavefront 0 when
-// the level is small) owns at most OWNK cells of the first sweep's region
-// keep those cells' operands in registers: a sweep, and the residual, read
-// only x or t from LDS.  The arithmetic is jacobi_f's and residual_f's,
-// operand for operand (c1 / |A_5| is the same division, done once).
+// the level is small) owns at most OWNK interior cells keep those cells'
+// operands in registers: a sweep, and the residual, read only x or t from
+// LDS.  The arithmetic is jacobi_f's and residual_f's, operand for operand
+// (c1 / |A_5| is the same division, done once).  The first sweep runs on
+// the interior only and stores t with its periodic images: below the finest
+// level the operator, rhs, mask and x hold periodic images in their halo,
+// so the first sweep's value at a halo position (which smoothtwicewithA
+// evaluates there from the halo) is the value at its image.
 constexpr int OWNK = 1;
 
 // `iters` smoothings (two damped-Jacobi sweeps each; the first reads x = 0
@@ -1639,8 +1643,8 @@ template <typename T, typename TA>
 __device__ __forceinline__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Sync& sy, bool zero_in, int iters,
                              bool resid) {
   const int m = L.m, n = L.n, cells = m * n;
-  const int wj = m - 2 * NH + 2, wi = n - 2 * NH + 2, w = wj * wi;
-  // one wavefront when the region fits it or the level is small (LDS
+  const int wi = n - 2 * NH, w = (m - 2 * NH) * wi;
+  // one wavefront when the interior fits it or the level is small (LDS
   // operations of a wavefront execute in order: a wave-level barrier
   // separates the phases)
   const bool wave = w <= 64 || cells <= sy.small_cells;
@@ -1664,10 +1668,9 @@ __device__ __forceinline__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Syn
     const int q = sy.tid + k * st;
     o[k].own = active && q < w;
     const int qj = o[k].own ? qdiv(q, wi) : 0;
-    o[k].j = NH - 1 + qj;
-    o[k].i = NH - 1 + (o[k].own ? q - qj * wi : 0);
+    o[k].j = NH + qj;
+    o[k].i = NH + (o[k].own ? q - qj * wi : 0);
     o[k].c = o[k].j * n + o[k].i;
-    o[k].inner = o[k].own && o[k].j >= NH && o[k].j < m - NH && o[k].i >= NH && o[k].i < n - NH;
     if (o[k].own) {
       const int c = o[k].c;
       o[k].a.a0 = T(L.A[c]);
@@ -1700,12 +1703,12 @@ __device__ __forceinline__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Syn
         else
           v = jacobi_c3_2d<T>(o[k].bc, o[k].c3, c2, o[k].a,
                               [&](int dj, int di) { return x[c + dj * n + di]; });
-        t[c] = o[k].mk == 0 ? T(0) : v;
+        store_img_mask<T>(t, nullptr, o[k].j, o[k].i, o[k].mk == 0 ? T(0) : v, m, n);
       }
     sync();
 #pragma unroll
     for (int k = 0; k < OWNK; k++)
-      if (o[k].inner) {
+      if (o[k].own) {
         const int c = o[k].c;
         const T v = jacobi_c3_2d<T>(o[k].bc, o[k].c3, c2, o[k].a,
                                     [&](int dj, int di) { return t[c + dj * n + di]; });
@@ -1716,7 +1719,7 @@ __device__ __forceinline__ bool c_smooth_own(const CL<T, TA>& L, T c1, T c2, Syn
   if (resid) {
 #pragma unroll
     for (int k = 0; k < OWNK; k++)
-      if (o[k].inner) {
+      if (o[k].own) {
         const int c = o[k].c;
         const T v = residual_2d<T>(o[k].mk, o[k].bc, o[k].a,
                                    [&](int dj, int di) { return x[c + dj * n + di]; });
