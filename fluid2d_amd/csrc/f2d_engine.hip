@@ -2040,15 +2040,22 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
       off += r16(G.m * G.n);
       dma(mk, B + G.msk, G.m * G.n);
     }
+    // the top level's rhs, widened to FP64: its first 8 * nt values are
+    // loaded with the transfers, before the gate is read
+    const LevD& G0 = in.L[k0];
+    const int sz0 = G0.m * G0.n;
+    const T* bg = Q + G0.b;
+    T v0[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v0[k] = tid + k * nt < sz0 ? bg[tid + k * nt] : T(0);
     if (skip(dyn, tk.inst, pred)) {
       __builtin_amdgcn_s_waitcnt(0);  // no transfer into LDS outlives the block
       return;
     }
-    // the top level's rhs, widened to FP64
-    const LevD& G0 = in.L[k0];
-    const int sz0 = G0.m * G0.n;
-    const T* bg = Q + G0.b;
-    for (int base = tid; base < sz0; base += 8 * nt) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (tid + k * nt < sz0) b0[tid + k * nt] = (C)v0[k];
+    for (int base = tid + 8 * nt; base < sz0; base += 8 * nt) {
       T v[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) v[k] = bg[min(base + k * nt, sz0 - 1)];
