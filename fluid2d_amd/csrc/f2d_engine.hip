@@ -967,6 +967,24 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
 // images.  The per-cell arithmetic is jacobi_2d's and residual_2d's, as in
 // k_smooth (mode 1, explicit operator) and k_resid_restrict.
 // ---------------------------------------------------------------------------
+// Phase timestamps of k_smooth_rr (profiling build only, make stamps):
+// thread 0 of each of the first 256 workgroups of the last launch on each
+// level (1..7) records the real-time clock (100 MHz) at the start, after the
+// task read, after the loads are issued and the step gate read, after each
+// sweep, after the residuals and at the end; slot 7 holds the compute unit.
+#ifdef F2D_PHASE_STAMPS
+__device__ unsigned long long rr_stamps[8][256][8];
+#define F2D_RSTAMP(lev, k)                                                  \
+  do {                                                                     \
+    if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x < 256 && (lev) < 8) \
+      rr_stamps[(lev)][blockIdx.x][(k)] = (unsigned long long)wall_clock64();  \
+  } while (0)
+#else
+#define F2D_RSTAMP(lev, k) \
+  do {                     \
+  } while (0)
+#endif
+
 __device__ __forceinline__ int wrap_img(int g, int m) {
   const int M = m - 2 * NH;
   while (g < NH) g += M;
@@ -978,13 +996,20 @@ __device__ __forceinline__ int wrap_img(int g, int m) {
 // x = I(x of the coarser level) (an F-cycle's V-cycle from this level,
 // hierarchy.py:145-151); the prolongation is evaluated at the images too (the
 // coarse x and both masks are stored with their images).
-template <typename T, int TY2V, int X0>
-__global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ tasks,
+// STAGE: the five operator planes on the region (plus the row and the
+// columns the stencil reaches past it) are loaded with the tile, at their
+// images, and the sweeps and residuals read them from LDS: one memory round
+// trip for the launch instead of one per phase (the small launches of the
+// mid levels, whose time is those round trips).  NTH: threads per
+// workgroup, (64, NTH / 64); 1024 for launches of at most one workgroup per
+// CU, where each phase is then one pass over its region instead of several.
+template <typename T, int TY2V, int X0, int STAGE = 0, int NTH = 256>
+__global__ void __launch_bounds__(NTH) k_smooth_rr(const Task* __restrict__ tasks,
                                                    const InstD* __restrict__ I, T* Q,
                                                    const int8_t* __restrict__ B,
                                                    const DynD* __restrict__ dyn, int pred) {
   static_assert(X0 == 1 || (X0 == 3 && TY2V <= 4), "x0 form");
-  constexpr int NT = 256;
+  constexpr int NT = NTH;
   constexpr int RY = 2 * TY2V + 1, RX = 2 * TX2 + 1;  // residual points
   constexpr int H1 = RY + 2, W1 = RX + 2;             // second sweep
   constexpr int H2 = RY + 4, W2 = RX + 4;             // first sweep, rhs, mask
@@ -996,9 +1021,22 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
   __shared__ T x2s[H1][W1];
   __shared__ T rs[RY][RX];
   __shared__ int8_t ms[H2][W2];
-  __shared__ uint32_t cs[H2][W2];  // image cell index of each region position
+  __shared__ uint32_t cs[STAGE ? 1 : H2][STAGE ? 1 : W2];  // image cell index of each region position
+  // staged operator: rows [gy, gy + H2], columns [gx - 1, gx + W2]
+  constexpr int HA = H2 + 1, WA = W2 + 2, NA = HA * WA, ITA = STAGE ? (NA + NT - 1) / NT : 1;
+  __shared__ T As[STAGE ? 5 : 1][STAGE ? HA : 1][STAGE ? WA : 1];
   const int ti = task_index();
+#ifdef F2D_PHASE_STAMPS
+  const unsigned long long st0 = wall_clock64();
+#endif
   const Task tk = tasks[ti];
+#ifdef F2D_PHASE_STAMPS
+  F2D_RSTAMP(tk.lev, 1);
+  if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x < 256 && tk.lev < 8) {
+    rr_stamps[tk.lev][blockIdx.x][0] = st0;
+    rr_stamps[tk.lev][blockIdx.x][7] = ((unsigned long long)__smid() << 32) | gridDim.x;
+  }
+#endif
   const InstD& in = I[tk.inst];
   const int m1 = tk.m, n1 = tk.n, m2 = tk.m2, n2 = tk.n2;
   const size_t PL = (size_t)m1 * n1;
@@ -1020,6 +1058,17 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
     cv[it] = at32(wrap_img(gy + rr, m1), wrap_img(gx + cc, n1), n1);
     bv[it] = ld32(b, cv[it]);
     mv[it] = ld32(msk, cv[it]);
+  }
+  T av[5][ITA];
+  if constexpr (STAGE != 0) {
+#pragma unroll
+    for (int it = 0; it < ITA; it++) {
+      const int q = min(tid + it * NT, NA - 1);
+      const int rr = q / WA, cc = q - rr * WA;
+      const uint32_t e = at32(wrap_img(gy + rr, m1), wrap_img(gx - 1 + cc, n1), n1);
+#pragma unroll
+      for (int k = 0; k < 5; k++) av[k][it] = ld32(A, (uint32_t)k * (uint32_t)PL + e);
+    }
   }
   // X0 3: x = I(xc) on the first sweep's region plus one ring.  Fine row
   // gy - 1 + rr (gy odd) lies over coarse row cy + rr / 2, so the coarse x
@@ -1053,6 +1102,7 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
     }
   }
   if (skip(dyn, tk.inst, pred)) return;
+  F2D_RSTAMP(tk.lev, 2);
 #pragma unroll
   for (int it = 0; it < IT2; it++) {
     const int q = tid + it * NT;
@@ -1060,8 +1110,20 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
       const int rr = q / W2, cc = q - rr * W2;
       bs2[rr][cc] = bv[it];
       ms[rr][cc] = mv[it];
-      cs[rr][cc] = cv[it];
+      if constexpr (STAGE == 0) cs[rr][cc] = cv[it];
     }
+  }
+  if constexpr (STAGE != 0) {
+#pragma unroll
+    for (int it = 0; it < ITA; it++) {
+      const int q = tid + it * NT;
+      if (q < NA) {
+        const int rr = q / WA, cc = q - rr * WA;
+#pragma unroll
+        for (int k = 0; k < 5; k++) As[k][rr][cc] = av[k][it];
+      }
+    }
+    if constexpr (X0 == 1) __syncthreads();  // (X0 3 synchronises below)
   }
   if constexpr (X0 == 3) {
 #pragma unroll
@@ -1105,6 +1167,18 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
     const APlanes<T> Ap{A, PL, c};
     return [=](int k, int dj, int di) { return Ap(k, (ptrdiff_t)dj * n1 + di); };
   };
+  // the operator of region position (rr, cc): from LDS when staged, else at
+  // the position's image in the level's planes
+  auto Aop = [&](int rr, int cc, uint32_t e) {
+    if constexpr (STAGE != 0) {
+      (void)e;
+      return [&, rr, cc](int k, int dj, int di) { return As[k][rr + dj][cc + 1 + di]; };
+    } else {
+      (void)rr;
+      (void)cc;
+      return Aat(e);
+    }
+  };
   // first sweep on the whole region (rhs from registers), from x = 0 or
   // from the prolongation
 #pragma unroll
@@ -1113,23 +1187,26 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
     if (q < NR2) {
       const int rr = q / W2, cc = q - rr * W2;
       if constexpr (X0 == 3)
-        x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aat(cv[it]),
+        x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aop(rr, cc, cv[it]),
                                    [&](int dj, int di) { return x0s[rr + 1 + dj][cc + 1 + di]; });
       else
-        x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aat(cv[it]),
+        x1s[rr][cc] = jacobi_2d<T>(mv[it], bv[it], c1, c2, Aop(rr, cc, cv[it]),
                                    [](int, int) { return T(0); });
     }
   }
   __syncthreads();
+  F2D_RSTAMP(tk.lev, 3);
   // second sweep on the region less one ring
   constexpr int NR1 = H1 * W1;
   for (int q = tid; q < NR1; q += NT) {
     const int r1 = q / W1, c1i = q - r1 * W1;
     const int rr = r1 + 1, cc = c1i + 1;
-    x2s[r1][c1i] = jacobi_2d<T>(ms[rr][cc], bs2[rr][cc], c1, c2, Aat(cs[rr][cc]),
+    x2s[r1][c1i] = jacobi_2d<T>(ms[rr][cc], bs2[rr][cc], c1, c2,
+                                Aop(rr, cc, STAGE ? 0u : cs[STAGE ? 0 : rr][STAGE ? 0 : cc]),
                                 [&](int dj, int di) { return x1s[rr + dj][cc + di]; });
   }
   __syncthreads();
+  F2D_RSTAMP(tk.lev, 4);
   // residuals at the restriction's points; the smoothed x of the owned
   // fine cells (local rows/cols [0, 2*T) off the halo, under the coarse
   // interior) stored with their images
@@ -1138,7 +1215,8 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
   for (int q = tid; q < NRR; q += NT) {
     const int r0 = q / RX, c0 = q - r0 * RX;
     const int rr = r0 + 2, cc = c0 + 2;
-    rs[r0][c0] = residual_2d<T>(ms[rr][cc], bs2[rr][cc], Aat(cs[rr][cc]),
+    rs[r0][c0] = residual_2d<T>(ms[rr][cc], bs2[rr][cc],
+                                Aop(rr, cc, STAGE ? 0u : cs[STAGE ? 0 : rr][STAGE ? 0 : cc]),
                                 [&](int dj, int di) { return x2s[r0 + 1 + dj][c0 + 1 + di]; });
     const int jf = fy0 + r0, if_ = fx0 + c0;
     if (r0 < 2 * TY2V && c0 < 2 * TX2 && jf < m1 - NH && if_ < n1 - NH &&
@@ -1146,6 +1224,7 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
       store_img_mask32<T>(xout, nullptr, jf, if_, x2s[r0 + 1][c0 + 1], m1, n1);
   }
   __syncthreads();
+  F2D_RSTAMP(tk.lev, 5);
   // full weighting onto the coarse tile (+ images)
   T* b2 = Q + in.L[tk.lev + 1].b;
   constexpr int NC = TY2V * TX2;
@@ -1160,6 +1239,10 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
     }
     store_img_mask32<T>(b2, nullptr, J2, I2, v, m2, n2);
   }
+#ifdef F2D_PHASE_STAMPS
+  __syncthreads();
+  F2D_RSTAMP(tk.lev, 6);
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -1177,13 +1260,15 @@ __global__ void __launch_bounds__(256) k_smooth_rr(const Task* __restrict__ task
 // Tile origins are odd (NH plus multiples of the even tile sizes): fine
 // region row rr lies over coarse region row (rr + 1) / 2, and so on down.
 // ---------------------------------------------------------------------------
-template <typename T, int TYV>
-__global__ void __launch_bounds__(256) k_up2(const Task* __restrict__ tasks,
+// NTH: threads per workgroup, (64, NTH / 64); its launches are small (at
+// most 256 tiles), so 1024 threads make each phase one pass over its region.
+template <typename T, int TYV, int NTH = 1024>
+__global__ void __launch_bounds__(NTH) k_up2(const Task* __restrict__ tasks,
                                              const InstD* __restrict__ I, T* Q,
                                              const int8_t* __restrict__ B,
                                              const DynD* __restrict__ dyn, int pred) {
   static_assert(TYV == 4 || TYV == 8, "tile rows");
-  constexpr int NT = 256;
+  constexpr int NT = NTH;
   constexpr int XH = TYV + 4, XW = TX + 4, JH = TYV + 2, JW = TX + 2;  // fine x0, first sweep
   constexpr int CH = TYV / 2 + 4, CW = TX / 2 + 4;  // coarse x the prolongation reads
   constexpr int C1H = CH + 2, C1W = CW + 2;          // coarse first sweep
@@ -3537,6 +3622,8 @@ static void launch_rr(int ty, unsigned g, hipStream_t s, const Task* TK, const I
                        pred, flags, partial);
 }
 
+constexpr unsigned RR_STAGE_TILES = 256;
+
 template <typename T>
 static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof = nullptr) {
   const dim3 B2(64, 4), B1(256);
@@ -3631,13 +3718,21 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
         F2D_LAUNCH(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
       case L_SMOOTH_RR:
+        // launches of at most RR_STAGE_TILES workgroups (one per CU or less)
+        // stage the operator in LDS (k_smooth_rr STAGE)
         if (L.mode == 3) {  // x0 = I(xc): tiles of at most 4 coarse rows
-          if (L.ty == 4)
+          if (L.ty == 4 && g <= RR_STAGE_TILES)
+            F2D_LAUNCH((k_smooth_rr<T, 2, 3, 1, 1024>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI,
+                       Q, b.B, b.dD, L.pred);
+          else if (L.ty == 4)
             F2D_LAUNCH((k_smooth_rr<T, 2, 3>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                        L.pred);
           else
             F2D_LAUNCH((k_smooth_rr<T, 4, 3>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                        L.pred);
+        } else if (L.ty == 4 && g <= RR_STAGE_TILES) {
+          F2D_LAUNCH((k_smooth_rr<T, 2, 1, 1, 1024>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q,
+                     b.B, b.dD, L.pred);
         } else if (L.ty == 4) {
           F2D_LAUNCH((k_smooth_rr<T, 2, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                      L.pred);
@@ -3665,9 +3760,9 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
       }
       case L_UP2:
         if (L.ty == 4)
-          F2D_LAUNCH((k_up2<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+          F2D_LAUNCH((k_up2<T, 4>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
         else
-          F2D_LAUNCH((k_up2<T, 8>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+          F2D_LAUNCH((k_up2<T, 8>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
         break;
       case L_NSSRC:
         F2D_LAUNCH(k_ns_source, dim3(g), B2, 0, b.s, TK, b.dI, P, b.dD);
@@ -4553,6 +4648,12 @@ int f2d_batch_set_noslip(f2d_batch* b, int inst, const int8_t* msknoslip, const 
 int f2d_debug_smooth_stamps(unsigned long long* host) {
   ECHK(hipMemcpyFromSymbol(host, HIP_SYMBOL(f2d::eng::smooth_stamps),
                            sizeof(unsigned long long) * 2048 * 8));
+  return F2D_OK;
+}
+// the phase stamps of k_smooth_rr (8 levels x 256 workgroups x 8)
+int f2d_debug_rr_stamps(unsigned long long* host) {
+  ECHK(hipMemcpyFromSymbol(host, HIP_SYMBOL(f2d::eng::rr_stamps),
+                           sizeof(unsigned long long) * 8 * 256 * 8));
   return F2D_OK;
 }
 // the phase stamps of the last coarse-kernel launch of each mode (2 x 256)

@@ -9,6 +9,7 @@ roofline.
     python -m fluid2d_amd.profiling sq DB [ROWS]
     python -m fluid2d_amd.profiling stamps [WORKLOAD]
     python -m fluid2d_amd.profiling coarse [WORKLOAD]
+    python -m fluid2d_amd.profiling rr [WORKLOAD]
 
 ``traffic`` scales the per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the
 kernel launches with grid size GRID_X by what the same counters read for
@@ -24,7 +25,8 @@ smoother tiles), runs three plain FP64 steps of WORKLOAD (default
 euler-closed-1024) and prints the phase timeline of the last finest-level
 smoother launch.  ``coarse`` does the same for the coarse kernel: the
 shader-clock cycles of each level visit of its last launch of each mode
-(a V-cycle's tail and an F-cycle).
+(a V-cycle's tail and an F-cycle), and ``rr`` for the fused down-leg
+kernel on each level.
 """
 import json
 import sqlite3
@@ -234,9 +236,56 @@ def coarse_stamps(workload="euler-closed-1024"):
                 print("  %-46s %7d cycles %6.2f us (sum)" % (nm, v, v / mhz if mhz else 0))
 
 
+def rr_stamps(workload="euler-closed-1024"):
+    """Phase timeline of the fused down-leg kernel (k_smooth_rr) per level
+    (profiling build): mean phase durations over the first 256 workgroups of
+    the last launch on each level (us, 100 MHz clock)."""
+    import ctypes as C
+    import os
+
+    import numpy as np
+
+    from . import _lib
+    _lib.LIB_PATH = os.path.join(os.path.dirname(_lib.LIB_PATH), "libf2dgpu_stamps.so")
+    from . import workloads
+    from .batch import Batch
+    from .grid import Grid
+    p = workloads.make_param(workload)
+    g = Grid(p)
+    tun = {"coarse_lds_budget": 159744, "min_tiles": 1024, "coarse_wave_cells": 0, "persist": 0}
+    b = Batch([(p, g)], precision="fp64", tuning=tun)
+    for k, v in workloads.initial_fields(workload, g).items():
+        b.set_field(0, k, v)
+    b.set_psi_from_vorticity()
+    b.set_dt(workloads.time_step(workload, g, b.diagnostics(0)["maxspeed"]))
+    b.step(3)
+    lib = _lib.load()
+    buf = (C.c_ulonglong * (8 * 256 * 8))()
+    if lib.f2d_debug_rr_stamps(buf) != 0:
+        raise RuntimeError("f2d_debug_rr_stamps failed")
+    a = np.frombuffer(buf, dtype=np.uint64).reshape(8, 256, 8).astype(np.int64)
+    names = ["task read", "loads issued, gate read", "first sweep", "second sweep",
+             "residuals, x stored", "restriction, end"]
+    print("%-6s %5s %7s" % ("level", "WGs", "start") + "".join(" %11s" % n[:11] for n in names)
+          + " %7s %7s" % ("end p50", "max"))
+    for lev in range(8):
+        r = a[lev]
+        r = r[r[:, 0] > 0]
+        if len(r) == 0:
+            continue
+        ng = int(r[0, 7] & 0xFFFFFFFF)
+        t = (r[:, :7] - r[:, 0].min()) * 0.01
+        ph = [np.mean(t[:, k + 1] - t[:, k]) for k in range(6)]
+        print("%-6d %5d %7.2f" % (lev, ng, np.mean(t[:, 0])) + "".join(" %11.2f" % v for v in ph)
+              + " %7.2f %7.2f" % (np.median(t[:, 6]), t[:, 6].max()))
+
+
 def main(argv):
     if argv and argv[0] == "stamps":
         stamps(*argv[1:2])
+        return 0
+    if argv and argv[0] == "rr":
+        rr_stamps(*argv[1:2])
         return 0
     if argv and argv[0] == "coarse":
         coarse_stamps(*argv[1:2])
