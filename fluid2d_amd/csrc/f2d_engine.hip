@@ -3717,33 +3717,48 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
       case L_GATE:
         F2D_LAUNCH(k_gate, dim3((b.ninst + 255) / 256), B1, 0, b.s, b.dD, b.ninst);
         break;
-      case L_SMOOTH_RR:
+      case L_SMOOTH_RR: {
         // launches of at most RR_STAGE_TILES workgroups (one per CU or less)
-        // stage the operator in LDS (k_smooth_rr STAGE)
+        // run 1024-thread workgroups and stage the operator in LDS
+        const bool st = g <= RR_STAGE_TILES;
+        const dim3 B16(64, 16);
         if (L.mode == 3) {  // x0 = I(xc): tiles of at most 4 coarse rows
-          if (L.ty == 4 && g <= RR_STAGE_TILES)
-            F2D_LAUNCH((k_smooth_rr<T, 2, 3, 1, 1024>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI,
-                       Q, b.B, b.dD, L.pred);
+          if (L.ty == 4 && st)
+            F2D_LAUNCH((k_smooth_rr<T, 2, 3, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
+                       b.dD, L.pred);
           else if (L.ty == 4)
             F2D_LAUNCH((k_smooth_rr<T, 2, 3>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                        L.pred);
+          else if (st)
+            F2D_LAUNCH((k_smooth_rr<T, 4, 3, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
+                       b.dD, L.pred);
           else
             F2D_LAUNCH((k_smooth_rr<T, 4, 3>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                        L.pred);
-        } else if (L.ty == 4 && g <= RR_STAGE_TILES) {
-          F2D_LAUNCH((k_smooth_rr<T, 2, 1, 1, 1024>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q,
-                     b.B, b.dD, L.pred);
         } else if (L.ty == 4) {
-          F2D_LAUNCH((k_smooth_rr<T, 2, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
-                     L.pred);
+          if (st)
+            F2D_LAUNCH((k_smooth_rr<T, 2, 1, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
+                       b.dD, L.pred);
+          else
+            F2D_LAUNCH((k_smooth_rr<T, 2, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                       L.pred);
         } else if (L.ty == 8) {
-          F2D_LAUNCH((k_smooth_rr<T, 4, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
-                     L.pred);
+          if (st)
+            F2D_LAUNCH((k_smooth_rr<T, 4, 1, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
+                       b.dD, L.pred);
+          else
+            F2D_LAUNCH((k_smooth_rr<T, 4, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                       L.pred);
         } else {
-          F2D_LAUNCH((k_smooth_rr<T, TY2, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
-                     L.pred);
+          if (st)
+            F2D_LAUNCH((k_smooth_rr<T, TY2, 1, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
+                       b.dD, L.pred);
+          else
+            F2D_LAUNCH((k_smooth_rr<T, TY2, 1>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+                       L.pred);
         }
         break;
+      }
       case L_PERSIST: {
         const PPhase* PH = b.dPh + L.ph_off;
         BarSlot* BS = b.bar + L.bar;
