@@ -2848,6 +2848,10 @@ struct Batch {
   size_t coarse_budget = 0;    // LDS bytes for k_coarse (0: device maximum)
 };
 
+// launches of the fused down-leg of at most this many tiles run the staged
+// 1024-thread form
+constexpr double RR_STAGE_TILES_B = 256.;
+
 // ---- task helpers ----------------------------------------------------------
 static Task mk_task(int inst, int lev, int y0, int x0) {
   Task t;
@@ -2928,6 +2932,22 @@ struct Builder {
     for (int ty = TY; ty > 4; ty /= 2)
       if (cells / ((double)TX * ty) >= (double)b.min_tiles) return ty;
     return 4;
+  }
+  // tile rows of a fused down-leg phase: ty, or the lowest of 8 and `most`
+  // (rows) that brings the phase to at most RR_STAGE_TILES_B tiles
+  int rr_tile_rows(const std::vector<int>& who, const std::vector<int>& start, int dd, int ty,
+                   int most) const {
+    double cells = 0.;
+    for (int i : who) {
+      const int D = b.H[i].d.nfine - 1 - start[i];
+      if (dd > D) continue;
+      const LevD& L = b.H[i].d.L[b.H[i].d.nfine - 1 - dd];
+      cells += (double)(L.m - 2 * NH) * (L.n - 2 * NH);
+    }
+    if (cells / ((double)TX * ty) <= RR_STAGE_TILES_B) return ty;
+    for (int t = 2 * ty; t <= most; t *= 2)
+      if (cells / ((double)TX * t) <= RR_STAGE_TILES_B) return t;
+    return ty;
   }
   int pick_ty(const std::vector<int>& who, const std::vector<int>& start, int dd) const {
     double cells = 0.;
@@ -3019,6 +3039,10 @@ struct Builder {
     for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1 - start[i]);
     for (int dd = Dmax; dd >= 0; dd--) {
       const int ty = pick_ty(who, start, dd);
+      // the fused down-leg takes taller tiles where that brings its launch
+      // to at most RR_STAGE_TILES tiles (one 1024-thread workgroup per CU,
+      // operator staged in LDS); x0 = I(xc) tiles stay at most 8 rows
+      const int tyr1 = rr_tile_rows(who, start, dd, ty, 16), tyr3 = rr_tile_rows(who, start, dd, ty, 8);
       std::vector<Task> ts, rs, fs, f3;
       for (int i : who) {
         const int D = b.H[i].d.nfine - 1 - start[i];
@@ -3034,7 +3058,7 @@ struct Builder {
           h.cur[l] ^= 1;
           std::vector<Task>& out = mode == 3 ? f3 : fs;
           const size_t n0 = out.size();
-          rr_tasks(i, l, 0, xo, mode == 3 ? xc : -1, out, ty / 2);
+          rr_tasks(i, l, 0, xo, mode == 3 ? xc : -1, out, (mode == 3 ? tyr3 : tyr1) / 2);
           for (size_t k = n0; k < out.size(); k++) out[k].mode = mode;
           continue;
         }
@@ -3055,7 +3079,7 @@ struct Builder {
         if (f->empty()) continue;
         Launch& Lf = add(L_SMOOTH_RR, *f);
         Lf.pred = pred;
-        Lf.ty = ty;
+        Lf.ty = f == &f3 ? tyr3 : tyr1;
         Lf.mode = f == &f3 ? 3 : 1;
       }
     }
