@@ -1267,7 +1267,7 @@ __global__ void __launch_bounds__(NTH) k_up2(const Task* __restrict__ tasks,
                                              const InstD* __restrict__ I, T* Q,
                                              const int8_t* __restrict__ B,
                                              const DynD* __restrict__ dyn, int pred) {
-  static_assert(TYV == 4 || TYV == 8, "tile rows");
+  static_assert(TYV == 4 || TYV == 8 || TYV == 16, "tile rows");
   constexpr int NT = NTH;
   constexpr int XH = TYV + 4, XW = TX + 4, JH = TYV + 2, JW = TX + 2;  // fine x0, first sweep
   constexpr int CH = TYV / 2 + 4, CW = TX / 2 + 4;  // coarse x the prolongation reads
@@ -3084,13 +3084,13 @@ struct Builder {
       }
     }
     coarse(who, 0, pred);
-    // up-leg: levels below the finest two at a time (k_up2) where the
-    // instance has both, the finer one is not the masked start level and
-    // the finer level's launch is small (at most UP2_TILES tiles): there a
-    // pair costs less than its two latency-bound launches (9.0 against
-    // 5.3 + 5.3 us at 64^2 + 128^2); on larger levels the recomputed coarse
-    // regions cost more than the launch they save (14.7 against 5.3 + 7.7 us
-    // at 256^2 + 512^2)
+    // up-leg: levels below the finest two at a time (k_up2, one 1024-thread
+    // workgroup per tile) where the instance has both, the finer one is not
+    // the masked start level and the finer level's launch is small (at most
+    // UP2_TILES tiles, taking tiles of up to 16 rows for that): there a pair
+    // costs less than its two latency-bound launches (64^2 + 128^2, and
+    // 256^2 + 512^2 on 64 x 16 tiles: -0.5 % per 1024^2 step); larger
+    // launches would not be resident in one wave
     constexpr double UP2_TILES = 256.;
     std::vector<int> paired(b.ninst, -1);
     for (int dd = 0; dd <= Dmax; dd++) {
@@ -3104,6 +3104,8 @@ struct Builder {
           const LevD& L = b.H[i].d.L[b.H[i].d.nfine - 2 - dd];
           cells += (double)(L.m - 2 * NH) * (L.n - 2 * NH);
         }
+        // taller tiles where those bring the finer level to UP2_TILES
+        while (tyf < 16 && cells / ((double)TX * tyf) > UP2_TILES) tyf *= 2;
         if (cells / ((double)TX * tyf) > UP2_TILES) tyf = 0;
       }
       std::vector<Task> ts, ps;
@@ -3112,7 +3114,7 @@ struct Builder {
         if (dd > D || paired[i] == dd) continue;
         const int l = b.H[i].d.nfine - 1 - dd;
         if (b.fuse_up && dd < D && l - 1 >= 1 && !(final_top && dd + 1 == D) &&
-            (tyf == 4 || tyf == 8)) {
+            (tyf == 4 || tyf == 8 || tyf == 16)) {
           up2_task(i, l - 1, ps, tyf);
           paired[i] = dd + 1;
           continue;
@@ -3800,8 +3802,10 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
       case L_UP2:
         if (L.ty == 4)
           F2D_LAUNCH((k_up2<T, 4>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
-        else
+        else if (L.ty == 8)
           F2D_LAUNCH((k_up2<T, 8>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        else
+          F2D_LAUNCH((k_up2<T, 16>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
         break;
       case L_NSSRC:
         F2D_LAUNCH(k_ns_source, dim3(g), B2, 0, b.s, TK, b.dI, P, b.dD);
