@@ -247,7 +247,12 @@ def load_traffic(kernel, workload, ty):
         return None, None
     with open(path) as f:
         t = json.load(f)
-    key = "%s<double, %d>" % (kernel, ty)
+    # the kernel's name as the trace shows it: k_smooth<T, rows>,
+    # k_resid_restrict<T, R, coarse rows>
+    if kernel == "k_resid_restrict":
+        key = "%s<double, double, %d>" % (kernel, ty // 2)
+    else:
+        key = "%s<double, %d>" % (kernel, ty)
     if key not in t:
         return None, os.path.relpath(path, REPO) + " (no entry for %s)" % key
     return t[key].get("bytes_per_launch"), os.path.relpath(path, REPO) + " [%s]" % key

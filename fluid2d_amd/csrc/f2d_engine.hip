@@ -273,8 +273,13 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   // edge's left (lower) cell: the first term in FP32 on differences taken
   // in FP64, the unit-field flux F(1) and the sum in FP64, so the rounding
   // scales with the local variation of q.
-  __shared__ double fx[TY][TX + 1];
-  __shared__ double fy[TY + 1][TX];
+  // One tracer: each edge's flux overwrites the edge velocity it was
+  // computed from (the same thread reads and writes the same slot), so the
+  // tile needs no flux arrays of its own (LDS for four workgroups per CU).
+  __shared__ double fxs[NTR > 1 ? TY : 1][NTR > 1 ? TX + 1 : 1];
+  __shared__ double fys[NTR > 1 ? TY + 1 : 1][NTR > 1 ? TX : 1];
+  double(*fx)[TX + 1] = NTR > 1 ? (double(*)[TX + 1])fxs : (double(*)[TX + 1])us;
+  double(*fy)[TX] = NTR > 1 ? (double(*)[TX])fys : (double(*)[TX])vs;
   auto edge_flux = [&](const double* qt, int lq, ptrdiff_t st, double vel) -> double {
     const double ql = MIXED ? qt[lq] : 0.;
     auto qv = [qt, lq, st, ql](int k) { return T(qt[lq + k * st] - ql); };
