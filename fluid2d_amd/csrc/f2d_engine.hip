@@ -273,13 +273,17 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   // edge's left (lower) cell: the first term in FP32 on differences taken
   // in FP64, the unit-field flux F(1) and the sum in FP64, so the rounding
   // scales with the local variation of q.
-  // One tracer: each edge's flux overwrites the edge velocity it was
-  // computed from (the same thread reads and writes the same slot), so the
-  // tile needs no flux arrays of its own (LDS for four workgroups per CU).
-  __shared__ double fxs[NTR > 1 ? TY : 1][NTR > 1 ? TX + 1 : 1];
-  __shared__ double fys[NTR > 1 ? TY + 1 : 1][NTR > 1 ? TX : 1];
-  double(*fx)[TX + 1] = NTR > 1 ? (double(*)[TX + 1])fxs : (double(*)[TX + 1])us;
-  double(*fy)[TX] = NTR > 1 ? (double(*)[TX])fys : (double(*)[TX])vs;
+  // Each edge's flux overwrites the edge velocity it was computed from (the
+  // same thread reads and writes the same slot; with two tracers the thread
+  // keeps its edges' velocities in registers for the second), so the tile
+  // needs no flux arrays of its own: four workgroups per CU with one tracer,
+  // three with two.
+  double(*fx)[TX + 1] = (double(*)[TX + 1])us;
+  double(*fy)[TX] = (double(*)[TX])vs;
+  constexpr int KU = (TY * (TX + 1) + 255) / 256, KV = ((TY + 1) * TX + 255) / 256;
+  double uvel[NTR > 1 ? KU : 1], vvel[NTR > 1 ? KV : 1];
+  (void)uvel;
+  (void)vvel;
   auto edge_flux = [&](const double* qt, int lq, ptrdiff_t st, double vel) -> double {
     const double ql = MIXED ? qt[lq] : 0.;
     auto qv = [qt, lq, st, ql](int k) { return T(qt[lq + k * st] - ql); };
@@ -298,13 +302,34 @@ __global__ void __launch_bounds__(256) k_rhs(const Task* __restrict__ tasks,
   for (int t = 0; t < NTR; t++) {
     const double* qt = qs[t];
     if (t > 0) __syncthreads();  // the previous tracer's divergence read fx, fy
-    for (int q = tid; q < TY * (TX + 1); q += 256) {
-      const int r = q / (TX + 1), e = q - r * (TX + 1);
-      fx[r][e] = edge_flux(qt, (r + 3) * QW + e + 2, 1, us[r * (TX + 1) + e]);
-    }
-    for (int q = tid; q < (TY + 1) * TX; q += 256) {
-      const int e = q / TX, c = q - e * TX;
-      fy[e][c] = edge_flux(qt, (e + 2) * QW + c + 3, (ptrdiff_t)QW, vs[e * TX + c]);
+    if constexpr (NTR == 1) {
+      for (int q = tid; q < TY * (TX + 1); q += 256) {
+        const int r = q / (TX + 1), e = q - r * (TX + 1);
+        fx[r][e] = edge_flux(qt, (r + 3) * QW + e + 2, 1, us[q]);
+      }
+      for (int q = tid; q < (TY + 1) * TX; q += 256) {
+        const int e = q / TX, c = q - e * TX;
+        fy[e][c] = edge_flux(qt, (e + 2) * QW + c + 3, (ptrdiff_t)QW, vs[q]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < KU; k++) {
+        const int q = tid + 256 * k;
+        if (q < TY * (TX + 1)) {
+          const int r = q / (TX + 1), e = q - r * (TX + 1);
+          if (t == 0) uvel[k] = us[q];
+          fx[r][e] = edge_flux(qt, (r + 3) * QW + e + 2, 1, uvel[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KV; k++) {
+        const int q = tid + 256 * k;
+        if (q < (TY + 1) * TX) {
+          const int e = q / TX, c = q - e * TX;
+          if (t == 0) vvel[k] = vs[q];
+          fy[e][c] = edge_flux(qt, (e + 2) * QW + c + 3, (ptrdiff_t)QW, vvel[k]);
+        }
+      }
     }
     __syncthreads();
     for (int r = threadIdx.y; r < TY; r += blockDim.y) {
