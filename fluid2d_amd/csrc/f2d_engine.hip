@@ -432,9 +432,15 @@ __device__ unsigned long long smooth_stamps[2048 * 8];
   } while (0)
 #endif
 
-template <typename T, int TYV>
+// ADD (task flag 4): the stored value is also added to the FP64 plane at
+// task offset xcc (in the P pool; flag 8: in the T pool of an FP64 batch)
+// at every position stored -- x0 += x (hierarchy.py:171) folded into the
+// last smoothing of a refinement V-cycle (no tile of the launch reads that
+// plane); a separate instantiation, so the other smoothings carry none of
+// its code.
+template <typename T, int TYV, bool ADD = false>
 __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, const int ti,
-                                            const InstD* __restrict__ I, T* Q,
+                                            const InstD* __restrict__ I, T* Q, double* P,
                                             const int8_t* __restrict__ B,
                                             const DynD* __restrict__ dyn, int pred) {
   // Block (64, 4).  Each phase first issues the global loads of all its
@@ -508,10 +514,24 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
   bool ok2[IT2];
   T bv[IT2];
   T be = T(0);
+  // the plane the stored values are added to (task flag 4), and its values
+  // at the tile's cells in the unclipped form, loaded with the tile
+  const bool addp = ADD && (tk.flags & 4) != 0;
+  double* pw = nullptr;
+  if constexpr (sizeof(T) == 8) {
+    pw = addp ? ((tk.flags & 8) ? (double*)Q : P) + tk.xcc : nullptr;
+  } else {
+    pw = addp ? P + tk.xcc : nullptr;
+  }
+  double pv[IT3];
   if (lean) {
 #pragma unroll
     for (int k = 0; k < KB; k++) bv[k] = ld32(b, ty + 4 * k < JH ? ob + k * s4 : ob);
     be = ld32(b, obe);
+    if (addp) {
+#pragma unroll
+      for (int k = 0; k < IT3; k++) pv[k] = ld32(pw, ob + (uint32_t)n + 1u + k * s4);
+    }
   } else {
 #pragma unroll
     for (int it = 0; it < IT2; it++) {
@@ -664,6 +684,14 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
   const int8_t* mp = final_mask ? B + in.mskp : nullptr;
   T w[5];
   for (int k = 0; k < 5; k++) w[k] = T(in.w0[k]);
+  // the plane += the value stored at interior (J, I) and at each image
+  auto add_img = [&](int J, int I, T v) {
+    each_img(J, I, m, n, [&](int j, int i) {
+      const uint32_t c = at32(j, i, n);
+      const T sv = mp ? v * T(ld32(mp, c)) : v;
+      st32(pw, c, ld32(pw, c) + (double)sv);
+    });
+  };
   // Jac(c, r, q, b, X): damped-Jacobi value of the cell at global c, tile
   // (r, q) of the x/bits tiles, right-hand side b, neighbour accessor X
   auto sweeps = [&](auto Jac) {
@@ -696,6 +724,7 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
       const T v = Jac(c, rr + 2, cc + 2, bs[rr + 1][cc + 1],
                       [&](int dj, int di) { return j1[rr + 1 + dj][cc + 1 + di]; });
       store_img_mask32<T>(xout, mp, J, Ic, v, m, n);
+      if (addp) add_img(J, Ic, v);
     }
   };
   // the same two sweeps in the unclipped form; stores without images when
@@ -721,10 +750,14 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
       const uint32_t c = ob + (uint32_t)n + 1u + k * s4;  // (y0 + rr, x0 + tx)
       const T v = Jac(c, rr + 2, tx + 2, bs[rr + 1][tx + 1],
                       [&](int dj, int di) { return j1[rr + 1 + dj][tx + 1 + di]; });
-      if (noimg)
-        st32(xout, c, mp ? v * T(ld32(mp, c)) : v);
-      else
+      if (noimg) {
+        const T sv = mp ? v * T(ld32(mp, c)) : v;
+        st32(xout, c, sv);
+        if (addp) st32(pw, c, pv[k] + (double)sv);
+      } else {
         store_img_mask32<T>(xout, mp, y0 + rr, Ic, v, m, n);
+        if (addp) add_img(y0 + rr, Ic, v);
+      }
     }
     F2D_STAMP_SYNC(5);
   };
@@ -758,12 +791,12 @@ __device__ __forceinline__ void smooth_tile(const Task* __restrict__ tasks, cons
   }
 }
 
-template <typename T, int TYV>
+template <typename T, int TYV, bool ADD = false>
 __global__ void __launch_bounds__(256) k_smooth(const Task* __restrict__ tasks,
-                                                const InstD* __restrict__ I, T* Q,
+                                                const InstD* __restrict__ I, T* Q, double* P,
                                                 const int8_t* __restrict__ B,
                                                 const DynD* __restrict__ dyn, int pred) {
-  smooth_tile<T, TYV>(tasks, task_index(), I, Q, B, dyn, pred);
+  smooth_tile<T, TYV, ADD>(tasks, task_index(), I, Q, P, B, dyn, pred);
 }
 
 // ---------------------------------------------------------------------------
@@ -1541,7 +1574,7 @@ __global__ void __launch_bounds__(256) k_persist(const PPhase* __restrict__ ph, 
     const PPhase f = ph[p];
     for (int t = blockIdx.x; t < f.ntask; t += G) {
       if (f.kind == 0)
-        smooth_tile<T, TYV>(tasks + f.toff, t, I, Q, B, dyn, f.pred);
+        smooth_tile<T, TYV>(tasks + f.toff, t, I, Q, nullptr, B, dyn, f.pred);
       else
         rr_tile<T, T, TYV / 2>(tasks + f.toff, t, I, (const T*)Q, Q, B, dyn, f.pred, f.flags,
                                nullptr);
@@ -3026,6 +3059,16 @@ struct Builder {
     }
   }
   // smoother on level l of instance i: input mode, output the other buffer
+  // per instance: the FP64 plane the finest level's up-leg smoothing adds
+  // its result to (x0 += x folded into it), -1 for none; in the T pool of an
+  // FP64 batch when upd_q
+  std::vector<long long> upd_plane;
+  bool upd_q = false;
+  // interior cells of instance i's finest level
+  long long h_cells(int i) const {
+    const LevD& L = b.H[i].d.L[0];
+    return (long long)(L.m - 2 * NH) * (L.n - 2 * NH);
+  }
   Task smooth_task(int i, int l, int mode, int final_mask, std::vector<Task>& out, int ty) {
     HostInst& h = b.H[i];
     Task t = mk_task(i, l, 0, 0);
@@ -3034,6 +3077,10 @@ struct Builder {
     t.xin = h.buf[l][h.cur[l]];
     t.xout = h.buf[l][h.cur[l] ^ 1];
     if (mode == 2 || mode == 3) t.xc = h.buf[l + 1][h.cur[l + 1]];
+    if (l == 0 && mode == 2 && !upd_plane.empty() && upd_plane[i] >= 0) {
+      t.flags |= 4 | (upd_q ? 8 : 0);
+      t.xcc = upd_plane[i];
+    }
     const LevD& L = lev(i, l);
     tiles(out, t, NH, L.m - NH, NH, L.n - NH, ty);
     h.cur[l] ^= 1;
@@ -3153,9 +3200,12 @@ struct Builder {
       }
       if (!ts.empty()) {
         balance(ts, ty);
+        bool adds = false;
+        for (const Task& t : ts) adds = adds || (t.flags & 4);
         Launch& Ls = add(L_SMOOTH, ts);
         Ls.pred = pred;
         Ls.ty = ty;
+        if (adds) Ls.flags |= 16;
       }
       if (!ps.empty()) {
         Launch& Lp = add(L_UP2, ps);
@@ -3366,6 +3416,19 @@ struct Builder {
         }
       }
       coarse(who, 1, 1);
+      // x0 += x (hierarchy.py:171) is folded into each instance's last
+      // smoothing of the finest level (the up-leg of its V-cycle from level
+      // 0, the last of the F-cycle's V-cycles), at every position it stores:
+      // a launch less where the solve is latency-bound (at most 4 M finest
+      // cells; on larger batches the smoothing's read-modify-write of the
+      // plane costs what the separate pass does)
+      double fine_cells = 0.;
+      for (int i : who) fine_cells += (double)h_cells(i);
+      const bool fold_up = fine_cells <= 4194304.;
+      upd_plane.assign(b.ninst, -1);
+      upd_q = psi_q;
+      if (fold_up)
+        for (int i : who) upd_plane[i] = psi(b.H[i]);
       for (int dd = 0; dd <= Dmax; dd++) {
         std::vector<int> sub, st(b.ninst, 0);
         for (int i : who)
@@ -3375,20 +3438,23 @@ struct Builder {
           }
         vcycle(sub, st, 3, 1, false);
       }
-      std::vector<Task> up;
-      for (int i : who) {
-        HostInst& h = b.H[i];
-        const size_t F = (size_t)h.d.m * h.d.n;
-        for (size_t q = 0; q < F; q += FLAT) {
-          Task t = mk_task(i, 0, (int)q, (int)std::min<size_t>(FLAT, F - q));
-          t.xout = psi(h);
-          t.xin = h.buf[0][h.cur[0]];
-          up.push_back(t);
+      upd_plane.clear();
+      if (!fold_up) {
+        std::vector<Task> up;
+        for (int i : who) {
+          HostInst& h = b.H[i];
+          const size_t F = (size_t)h.d.m * h.d.n;
+          for (size_t q = 0; q < F; q += FLAT) {
+            Task t = mk_task(i, 0, (int)q, (int)std::min<size_t>(FLAT, F - q));
+            t.xout = psi(h);
+            t.xin = h.buf[0][h.cur[0]];
+            up.push_back(t);
+          }
         }
+        Launch& U = add(L_UPDATE, up);
+        U.pred = 1;
+        U.flags = psi_q ? 1 : 0;
       }
-      Launch& U = add(L_UPDATE, up);
-      U.pred = 1;
-      U.flags = psi_q ? 1 : 0;
       resid_norm(1, 1);
     }
   }
@@ -3723,12 +3789,22 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
           F2D_LAUNCH((k_c2c<T, T>), dim3(g), B2, 0, b.s, TK, b.dI, Q, Q, b.dD, L.pred);
         break;
       case L_SMOOTH:
-        if (L.ty == 4)
-          F2D_LAUNCH((k_smooth<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+        // launch flag 16: tasks that add their result to a plane (k_smooth ADD)
+        if ((L.flags & 16) && L.ty == 4)
+          F2D_LAUNCH((k_smooth<T, 4, true>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B, b.dD,
+                     L.pred);
+        else if ((L.flags & 16) && L.ty == 8)
+          F2D_LAUNCH((k_smooth<T, 8, true>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B, b.dD,
+                     L.pred);
+        else if (L.flags & 16)
+          F2D_LAUNCH((k_smooth<T, TY, true>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B, b.dD,
+                     L.pred);
+        else if (L.ty == 4)
+          F2D_LAUNCH((k_smooth<T, 4>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B, b.dD, L.pred);
         else if (L.ty == 8)
-          F2D_LAUNCH((k_smooth<T, 8>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);
+          F2D_LAUNCH((k_smooth<T, 8>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B, b.dD, L.pred);
         else
-          F2D_LAUNCH((k_smooth<T, TY>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
+          F2D_LAUNCH((k_smooth<T, TY>), dim3(g), B2, 0, b.s, TK, b.dI, Q, P, b.B, b.dD,
                              L.pred);
         break;
       case L_RR:
