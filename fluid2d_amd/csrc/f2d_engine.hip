@@ -3529,8 +3529,12 @@ struct Builder {
         true);
     solve_defect(who, [](HostInst& h) { return h.d.state + (long long)h.d.m * h.d.n; }, false,
                  [](HostInst& h) { return h.d.work; }, 16, b.maxite, b.tol);
-    // zero mean (perio) and mask, then (u, v)
-    npsum(who, 0, 3);
+    // zero mean (perio: the mean's sum and fold only there) and mask, then
+    // (u, v)
+    std::vector<int> per;
+    for (int i : who)
+      if (b.H[i].d.perio) per.push_back(i);
+    if (!per.empty()) npsum(per, 0, 3);
     add(L_FINPSI, flat(who, [](HostInst& h) { return (long long)h.d.m * h.d.n; }, 1));
     ortho(who, [](HostInst& h) { return h.d.state; }, 1);
   }
