@@ -1007,6 +1007,132 @@ __global__ void __launch_bounds__(256) k_resid_restrict(const Task* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// k_restrict_b2: the F-cycle's descent of b (core/gmg/hierarchy.py:131-151;
+// restrict, core/gmg/fortran_multigrid.f90:501-546) over two levels in one
+// tile program.  Task mode 2: b of level lev restricted onto lev+1 over the
+// region the tile needs (the cells it owns stored with their images, as
+// the one-level restriction stores them), then onto the tile's cells of
+// lev+2 (its b at xc); mode 1: one level, lev onto the tile's cells of
+// lev+1.  Below the finest level a halo value is its periodic image's, so
+// a lev+1 value on the halo is evaluated at its image.  Every value is
+// the restriction of k_resid_restrict with flag 2, operands in the same
+// order, rounded to T before the next level uses it: the results equal two
+// launches of that kernel bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T, int TYO, int NTH>
+__global__ void __launch_bounds__(NTH) k_restrict_b2(const Task* __restrict__ tasks,
+                                                     const InstD* __restrict__ I, T* Q,
+                                                     const int8_t* __restrict__ B,
+                                                     const DynD* __restrict__ dyn, int pred) {
+  constexpr int MY = 2 * TYO + 1, MX = 2 * TX2 + 1;  // region below the output level
+  constexpr int FY = 2 * MY + 1, FX = 2 * MX + 1;    // region of level lev (mode 2)
+  constexpr int NF = FY * FX, ITF = (NF + NTH - 1) / NTH;
+  constexpr int NM = MY * MX, ITM = (NM + NTH - 1) / NTH;
+  constexpr int NO = TYO * TX2, ITO = (NO + NTH - 1) / NTH;
+  __shared__ T fs[FY][FX];
+  __shared__ T ms[MY][MX];
+  const Task tk = tasks[task_index()];
+  const int tid = threadIdx.x;
+  const bool two = tk.mode == 2;
+  const T* bf = Q + tk.b;
+  const int m = tk.m, n = tk.n;     // level lev
+  const int mm = tk.m2, nm = tk.n2;  // level lev+1
+  const int mo = two ? tk.m3 : mm, no = two ? tk.n3 : nm;
+  const int8_t* mko = B + (two ? tk.msk3 : tk.msk2);
+  const int8_t* mkm = B + tk.msk2;
+  // region origins: the output level's tile (y0, x0) reads the level below
+  // from (2 y0 - 3, 2 x0 - 3); in mode 2 that level is lev+1, which reads
+  // lev from twice that again
+  const int my0 = 2 * tk.y0 - 3, mx0 = 2 * tk.x0 - 3;
+  const int fy0 = 2 * my0 - 3, fx0 = 2 * mx0 - 3;
+  // loads issued before the step gate is read: b of the region below the
+  // output (mode 1) or of level lev's region with the lev+1 mask (mode 2),
+  // then the output mask; rows and columns past the last needed point are
+  // clamped and never used.  Level lev's region is read at the images of
+  // its points: the lev+1 halo row mm - NH restricts lev rows m - 3 .. m - 1,
+  // whose images NH .. NH + 2 are the rows its image (row NH) restricts, so
+  // the halo values come out of the region as their images' values.
+  T fv[ITF];
+  int8_t mmv[ITM];
+  T mv[ITM];
+  if (two) {
+#pragma unroll
+    for (int it = 0; it < ITF; it++) {
+      const int q = min(tid + it * NTH, NF - 1);
+      const int r = q / FX, s = q - r * FX;
+      fv[it] = ld32(bf, at32(img(min(fy0 + r, m - 1), m), img(min(fx0 + s, n - 1), n), n));
+    }
+#pragma unroll
+    for (int it = 0; it < ITM; it++) {
+      const int q = min(tid + it * NTH, NM - 1);
+      const int r = q / MX, s = q - r * MX;
+      mmv[it] = ld32(mkm, at32(img(min(my0 + r, mm - NH), mm), img(min(mx0 + s, nm - NH), nm), nm));
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < ITM; it++) {
+      const int q = min(tid + it * NTH, NM - 1);
+      const int r = q / MX, s = q - r * MX;
+      mv[it] = ld32(bf, at32(img(min(my0 + r, m - NH), m), img(min(mx0 + s, n - NH), n), n));
+    }
+  }
+  int8_t mov[ITO];
+#pragma unroll
+  for (int it = 0; it < ITO; it++) {
+    const int q = min(tid + it * NTH, NO - 1);
+    const int ry = q / TX2, rx = q - ry * TX2;
+    mov[it] = ld32(mko, at32(min(tk.y0 + ry, mo - 1), min(tk.x0 + rx, no - 1), no));
+  }
+  if (skip(dyn, tk.inst, pred)) return;
+  if (two) {
+#pragma unroll
+    for (int it = 0; it < ITF; it++) {
+      const int q = tid + it * NTH;
+      if (q < NF) fs[q / FX][q % FX] = fv[it];
+    }
+    __syncthreads();
+    T* bm = Q + tk.b2;
+#pragma unroll
+    for (int it = 0; it < ITM; it++) {
+      const int q = tid + it * NTH;
+      if (q >= NM) continue;
+      const int r = q / MX, s = q - r * MX;
+      const int J = my0 + r, Ii = mx0 + s;
+      if (J > mm - NH || Ii > nm - NH) continue;  // beyond the last needed point
+      const bool halo = J == mm - NH || Ii == nm - NH;
+      T v = T(0);
+      if (mmv[it] != 0)
+        v = restrict_r<T>([&](int dj, int di) { return fs[2 * r + 1 + dj][2 * s + 1 + di]; });
+      ms[r][s] = v;
+      // owned: under the output tile's interior cells, off the halo
+      if (r < 2 * TYO && s < 2 * TX2 && !halo && tk.y0 + r / 2 < mo - NH &&
+          tk.x0 + s / 2 < no - NH)
+        store_img_mask32<T>(bm, nullptr, J, Ii, v, mm, nm);
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < ITM; it++) {
+      const int q = tid + it * NTH;
+      if (q < NM) ms[q / MX][q % MX] = mv[it];
+    }
+  }
+  __syncthreads();
+  T* bo = Q + (two ? tk.xc : tk.b2);
+#pragma unroll
+  for (int it = 0; it < ITO; it++) {
+    const int q = tid + it * NTH;
+    if (q >= NO) continue;
+    const int ry = q / TX2, rx = q - ry * TX2;
+    const int J = tk.y0 + ry, Ii = tk.x0 + rx;
+    if (J >= mo - NH || Ii >= no - NH) continue;
+    T v = T(0);
+    if (mov[it] != 0)
+      v = restrict_r<T>([&](int dj, int di) { return ms[2 * ry + 1 + dj][2 * rx + 1 + di]; });
+    store_img_mask32<T>(bo, nullptr, J, Ii, v, mo, no);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_smooth_rr: the down-leg of a V-cycle on a level below the finest entered
 // with x = 0 (hierarchy.py:103-108 for lev > lev1: smooth, residual,
 // restrict) or, at the start level of an F-cycle's V-cycle, with
@@ -2817,7 +2943,8 @@ enum Kind {
   L_NSDOT,
   L_NSFIX,
   L_FLATOPS,
-  L_UP2
+  L_UP2,
+  L_RB2
 };
 
 struct Launch {
@@ -2914,6 +3041,9 @@ struct Batch {
 // launches of the fused down-leg of at most this many tiles run the staged
 // 1024-thread form
 constexpr double RR_STAGE_TILES_B = 256.;
+// two-level restrictions of b: launches of at most RB2_TILES tiles of 8
+// output rows take tiles of RB2_TY_SMALL rows
+constexpr int RB2_TILES = 256, RB2_TY_SMALL = 4;
 
 // ---- task helpers ----------------------------------------------------------
 static Task mk_task(int inst, int lev, int y0, int x0) {
@@ -3098,6 +3228,25 @@ struct Builder {
     const int nc = (C.n - 2 * NH + TX2 - 1) / TX2;
     int row = 0;
     for (int y = NH; y < C.m - NH; y += ty2, row++)
+      for (int k = 0; k < nc; k++) {
+        Task u = t;
+        u.y0 = y;
+        u.x0 = NH + ((k + 5 * row) % nc) * TX2;
+        out.push_back(u);
+      }
+  }
+
+  // b of level l restricted onto l + mode (k_restrict_b2): tiles of
+  // ty x TX2 over the interior of level l + mode, columns rotated per row
+  // as in tiles()
+  void rb_tasks(int i, int l, int mode, std::vector<Task>& out, int ty) {
+    const LevD& C = lev(i, l + mode);
+    Task t = mk_task(i, l, 0, 0);
+    t.mode = mode;
+    t.xc = mode == 2 ? C.b : 0;  // b of level l + 2
+    const int nc = (C.n - 2 * NH + TX2 - 1) / TX2;
+    int row = 0;
+    for (int y = NH; y < C.m - NH; y += ty, row++)
       for (int k = 0; k < nc; k++) {
         Task u = t;
         u.y0 = y;
@@ -3392,27 +3541,56 @@ struct Builder {
     resid_norm(0, 0);
     for (int it = 0; it < maxite; it++) {
       // F-cycle: b restricted down the fine levels (b0 -> b1 came with the
-      // residual), coarse tail, then V-cycles from each fine level upward
+      // residual), coarse tail, then V-cycles from each fine level upward.
+      // Level l = nfine - 1 - dd of an instance goes to l + 1 at
+      // bottom-aligned step dd; steps dd and dd - 1 share a k_restrict_b2
+      // launch (one-level launches where the program's mid-level phases
+      // are merged into persistent launches)
       int Dmax = 0;
       for (int i : who) Dmax = std::max(Dmax, b.H[i].d.nfine - 1);
-      for (int dd = Dmax - 1; dd >= 0; dd--) {
-        double cells = 0.;
-        for (int i : who) {
-          const int l = b.H[i].d.nfine - 1 - dd;
-          if (l >= 1) cells += (double)(b.H[i].d.L[l].m - 2 * NH) * (b.H[i].d.L[l].n - 2 * NH);
+      for (int dd = Dmax - 1; dd >= 0; dd -= (b.persist > 0 ? 1 : 2)) {
+        if (b.persist > 0) {
+          double cells = 0.;
+          for (int i : who) {
+            const int l = b.H[i].d.nfine - 1 - dd;
+            if (l >= 1) cells += (double)(b.H[i].d.L[l].m - 2 * NH) * (b.H[i].d.L[l].n - 2 * NH);
+          }
+          const int ty = pick_ty_cells(cells);
+          std::vector<Task> rs;
+          for (int i : who) {
+            const int l = b.H[i].d.nfine - 1 - dd;
+            if (l >= 1) rr_tasks(i, l, 0, -1, -1, rs, ty / 2);
+          }
+          if (!rs.empty()) {
+            Launch& L = add(L_RR, rs);
+            L.flags = 2;
+            L.pred = 1;
+            L.ty = ty;
+          }
+          continue;
         }
-        const int ty = pick_ty_cells(cells);
+        // output tiles of 8 rows, or of RB2_TY_SMALL rows where 8 would give
+        // at most RB2_TILES tiles (latency-bound launches: more, shorter tiles)
         std::vector<Task> rs;
-        for (int i : who) {
-          const int l = b.H[i].d.nfine - 1 - dd;
-          if (l < 1) continue;
-          rr_tasks(i, l, 0, -1, -1, rs, ty / 2);
+        int rty = 8;
+        for (int ty : {8, RB2_TY_SMALL}) {
+          rs.clear();
+          rty = ty;
+          for (int i : who) {
+            const int l = b.H[i].d.nfine - 1 - dd;
+            if (dd >= 1 && l >= 1)
+              rb_tasks(i, l, 2, rs, ty);
+            else if (dd >= 1 && l == 0)
+              rb_tasks(i, 1, 1, rs, ty);
+            else if (dd == 0 && l >= 1)
+              rb_tasks(i, l, 1, rs, ty);
+          }
+          if (ty == 8 && (int)rs.size() > RB2_TILES) break;
         }
         if (!rs.empty()) {
-          Launch& L = add(L_RR, rs);
-          L.flags = 2;
+          Launch& L = add(L_RB2, rs);
           L.pred = 1;
-          L.ty = ty;
+          L.ty = rty;
         }
       }
       coarse(who, 1, 1);
@@ -3909,6 +4087,14 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
                      b.dD, BS, b.fault);
         break;
       }
+      case L_RB2:
+        if (L.ty == 8)
+          F2D_LAUNCH((k_restrict_b2<T, 8, 1024>), dim3(g), dim3(1024), 0, b.s, TK, b.dI, Q, b.B,
+                     b.dD, L.pred);
+        else
+          F2D_LAUNCH((k_restrict_b2<T, RB2_TY_SMALL, 1024>), dim3(g), dim3(1024), 0, b.s, TK,
+                     b.dI, Q, b.B, b.dD, L.pred);
+        break;
       case L_UP2:
         if (L.ty == 4)
           F2D_LAUNCH((k_up2<T, 4>), dim3(g), dim3(64, 16), 0, b.s, TK, b.dI, Q, b.B, b.dD, L.pred);

@@ -299,10 +299,16 @@ def test_persistent_multigrid_equals_launch_per_phase(precision):
                 for i in range(len(group)):
                     b.set_dt(dt_for(b, i, b.diagnostics(i)["maxspeed"]), i)
                 b.step(8)
-                out.append((b.info(0)["launches_per_step"], [b.state(i) for i in range(len(group))]))
+                out.append((b.info(0)["launches_per_step"], [b.state(i) for i in range(len(group))],
+                            b.profile("persist", -1, 1)["launches"]))
                 assert b.tuning()["persist"] == persist
                 del b
-            assert out[1][0] < out[0][0], "persistent program has %d launches, per-phase %d" % (
+            # the persistent program runs its merged phases in k_persist, in
+            # at most the per-phase program's launches (whose F-cycle descent
+            # of b already takes two levels a launch)
+            assert out[1][2] > 0 and out[0][2] == 0, "persistent launches %d / %d" % (
+                out[1][2], out[0][2])
+            assert out[1][0] <= out[0][0], "persistent program has %d launches, per-phase %d" % (
                 out[1][0], out[0][0])
             for a, m in zip(out[0][1], out[1][1]):
                 assert (a == m).all()
