@@ -90,6 +90,7 @@ struct Task {
   // operator, the next coarser level's x, mask and size
   long long xin2, b2, A2, xcc, msk3;
   int m3, n3;
+  int slot;  // the task's index in its launch's list as built (partial-sum slot)
 };
 
 }  // namespace eng

@@ -48,15 +48,20 @@ __device__ __forceinline__ bool skip(const DynD* dyn, int inst, int pred) {
 __device__ __forceinline__ bool idle(const DynD* dyn, int inst) { return !dyn[inst].run; }
 
 // Task of this workgroup.  Workgroups are dealt to the 8 XCDs round-robin
-// (block b runs on XCD b mod 8); mapping each XCD to a contiguous run of
-// the task list (tasks are ordered by instance, level and tile row) keeps
-// neighbouring tiles -- and the same tiles in consecutive launches -- on
-// one XCD and its L2.
+// (block b runs on XCD b mod 8); each XCD takes a contiguous run of the
+// task list as the builder made it (tasks ordered by instance, level and
+// tile row), which keeps neighbouring tiles -- and the same tiles in
+// consecutive launches -- on one XCD and its L2.  The device copy of a
+// tile launch's task list is stored in dispatch order (xcd_task below, at
+// upload), so workgroup b reads task b: no grid size is read (a memory
+// round trip before the task read), and a task's place in the builder's
+// list (partial-sum slot) is its field slot.
 constexpr int NXCD = 8;
-__device__ __forceinline__ int task_index() {
-  const int b = blockIdx.x, n = gridDim.x;
+__device__ __forceinline__ int task_index() { return blockIdx.x; }
+// builder's task index of workgroup b of an n-workgroup launch
+__host__ __device__ inline int xcd_task(int b, int n) {
   const int q = n / NXCD, r = n % NXCD, x = b % NXCD, i = b / NXCD;
-  return x * q + min(x, r) + i;
+  return x * q + (x < r ? x : r) + i;
 }
 
 // AND of a predicate over the 256 threads of a (64, 4) workgroup: one vote
@@ -895,8 +900,8 @@ __device__ __forceinline__ void rr_tile(const Task* __restrict__ tasks, const in
   }
   if (skip(dyn, tk.inst, pred)) {
     if ((flags & 12) && tid == 0) {
-      partial[2 * ti] = 0.;
-      partial[2 * ti + 1] = 0.;
+      partial[2 * tk.slot] = 0.;
+      partial[2 * tk.slot + 1] = 0.;
     }
     return;
   }
@@ -976,8 +981,8 @@ __device__ __forceinline__ void rr_tile(const Task* __restrict__ tasks, const in
       a += red[0][w];
       c += red[1][w];
     }
-    partial[2 * ti] = a;
-    partial[2 * ti + 1] = c;
+    partial[2 * tk.slot] = a;
+    partial[2 * tk.slot + 1] = c;
   }
   T* b2 = Q + in.L[tk.lev + 1].b;
 #pragma unroll
@@ -2241,13 +2246,17 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
                                                 const InstD* __restrict__ I, T* Q,
                                                 const int8_t* __restrict__ B,
                                                 const DynD* __restrict__ dyn, int pred, int mode,
-                                                int small_cells) {
+                                                int small_cells, int inst_first) {
   typedef double C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   LDSP unsigned char* lds = (LDSP unsigned char*)lds_raw;
   const int ti = task_index();
-  const Task tk = tasks[ti];
-  const InstD& in = I[tk.inst];
+  const Task& tk = tasks[ti];
+  // the instance: first + task index where the launch's instances are
+  // consecutive (inst_first >= 0), so that the descriptor read, and the
+  // operator transfers addressed by it, need not wait for the task read
+  const int inst = inst_first >= 0 ? inst_first + ti : tk.inst;
+  const InstD& in = I[inst];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int k0 = in.nfine, deep = in.nlev - 1, nl = deep - k0 + 1;
   __shared__ CL<C, T> Ls[MAXLEV];
@@ -2322,7 +2331,7 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse(const Task* __restric
     T v0[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) v0[k] = tid + k * nt < sz0 ? bg[tid + k * nt] : T(0);
-    if (skip(dyn, tk.inst, pred)) {
+    if (skip(dyn, inst, pred)) {
       __builtin_amdgcn_s_waitcnt(0);  // no transfer into LDS outlives the block
       return;
     }
@@ -2541,7 +2550,7 @@ __global__ void __launch_bounds__(256) k_ns_dot(const Task* __restrict__ tasks,
   if (tid < 2) {
     double x = red[tid][0];
     for (int q = 1; q < (int)(blockDim.x * blockDim.y) / 64; q++) x = x + red[tid][q];
-    partial[(size_t)ti * 2 + tid] = x;
+    partial[(size_t)tk.slot * 2 + tid] = x;
   }
 }
 
@@ -2717,7 +2726,7 @@ __global__ void __launch_bounds__(256) k_npsum(const Task* __restrict__ tasks,
         if ((lane % (2 * w)) == 0) r = r + o;
       }
     }
-    if (lane == 0) partial[slot[ti] + ch] = r;
+    if (lane == 0) partial[slot[tk.slot] + ch] = r;
   }
 }
 
@@ -2782,7 +2791,7 @@ __global__ void __launch_bounds__(256) k_diag(const Task* __restrict__ tasks,
     double x = red[tid][0];
     for (int q = 1; q < (int)(blockDim.x * blockDim.y) / 64; q++)
       x = tid == 0 ? fmax(x, red[tid][q]) : x + red[tid][q];
-    partial[(size_t)ti * 8 + tid] = x;
+    partial[(size_t)tk.slot * 8 + tid] = x;
   }
 }
 
@@ -3398,6 +3407,10 @@ struct Builder {
     L.mode = mode;
     L.pred = pred;
     L.shmem = sh;
+    // consecutive instances: the kernel takes instance first + task index
+    L.inst0 = who.empty() ? -1 : who[0];
+    for (size_t k = 0; k < who.size(); k++)
+      if (who[k] != who[0] + (int)k) L.inst0 = -1;
   }
 
   using Off = std::function<long long(HostInst&)>;
@@ -3998,7 +4011,7 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
         break;
       case L_COARSE:
         F2D_LAUNCH(k_coarse<T>, dim3(g), dim3(COARSE_THREADS), L.shmem, b.s, TK, b.dI, Q, b.B,
-                           b.dD, L.pred, L.mode, b.coarse_wave_cells);
+                           b.dD, L.pred, L.mode, b.coarse_wave_cells, L.inst0);
         break;
       case L_ORTHO:
         if (L.flags == 1)
@@ -4425,7 +4438,26 @@ static int build_programs(Batch& b) {
     t.m3 = cl2 ? d.L[t.lev + 2].m : 0;
     t.n3 = cl2 ? d.L[t.lev + 2].n : 0;
   }
-  ECHK(hipMemcpy(b.dT, b.tasks.data(), b.tasks.size() * sizeof(Task), hipMemcpyHostToDevice));
+  // the device copy: each launch's task list in dispatch order (workgroup k
+  // of a tile launch takes the builder's task xcd_task(k, n); the coarse
+  // kernel's workgroup k takes task k, its instance first + k), each task's
+  // slot its index in the builder's list.  Persistent phases keep the
+  // builder's order (the persistent kernel walks them itself).
+  std::vector<Task> dev(b.tasks);
+  for (size_t k = 0; k < dev.size(); k++) dev[k].slot = 0;
+  std::vector<char> placed(b.tasks.size(), 0);
+  for (auto* pr : progs)
+    for (const Launch& L : *pr) {
+      if (L.ntask <= 0 || L.kind == L_FOLD || L.kind == L_GATE || L.kind == L_PERSIST) continue;
+      if (placed[L.toff]) continue;
+      for (int k = 0; k < L.ntask; k++) {
+        const int src = L.kind == L_COARSE ? k : xcd_task(k, L.ntask);
+        dev[L.toff + k] = b.tasks[L.toff + src];
+        dev[L.toff + k].slot = src;
+        placed[L.toff + k] = 1;
+      }
+    }
+  ECHK(hipMemcpy(b.dT, dev.data(), dev.size() * sizeof(Task), hipMemcpyHostToDevice));
   if (b.dPh) (void)hipFree(b.dPh);
   if (b.bar) (void)hipFree(b.bar);
   b.dPh = nullptr;
