@@ -3911,9 +3911,11 @@ struct Builder {
 // Optional timing of one launch kind (eager runs only): every matching
 // launch is issued PROF_REP times back to back between one pair of events
 // on the batch stream, so the events' own latency is spread over the
-// repeats.  Every launch kind reads buffers it does not write (smoother
+// repeats.  The launches read buffers they do not write (smoother
 // ping-pong buffers, residual into its own buffer, tendencies from stage
-// states), so a repeat leaves the state as one launch does.
+// states), so a repeat leaves the state as one launch does -- except a
+// smoothing that also adds its result into the solution (launch flag 16,
+// x0 += x): it is never timed, since a repeat would add it again.
 constexpr int PROF_REP = 8;
 struct ProfSel {
   int kind = -1, level = -1;
@@ -3953,7 +3955,7 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
     const unsigned g = (unsigned)L.ntask;
     if (L.kind != L_FOLD && L.kind != L_GATE && g == 0) continue;
     bool timed = false;
-    if (prof && L.kind == prof->kind &&
+    if (prof && L.kind == prof->kind && !(L.kind == L_SMOOTH && (L.flags & 16)) &&
         (prof->level < 0 || (L.ntask > 0 && b.tasks[L.toff].lev == prof->level))) {
       hipEvent_t e0, e1;
       ECHK(hipEventCreate(&e0));
