@@ -3052,6 +3052,10 @@ struct Batch {
 // 1024-thread form (two workgroups per CU: 512 tiles of 64 x 8 on the 512^2
 // level take 0.8 % less per 1024^2 step than 256 tiles of 64 x 16)
 constexpr double RR_STAGE_TILES_B = 512.;
+// the smallest fused down-leg launches (at most this many tiles of 4 rows:
+// the 128^2 and 64^2 levels at 1024^2) take tiles of 2 fine rows, twice the
+// workgroups, each with half the cells (-0.6 % per 1024^2 step)
+constexpr double RR_SMALL_TILES = 64.;
 // two-level restrictions of b: launches of at most RB2_TILES tiles of 8
 // output rows take tiles of RB2_TY_SMALL rows
 constexpr int RB2_TILES = 256, RB2_TY_SMALL = 4;
@@ -3137,8 +3141,9 @@ struct Builder {
       if (cells / ((double)TX * ty) >= (double)b.min_tiles) return ty;
     return 4;
   }
-  // tile rows of a fused down-leg phase: ty, or the lowest of 8 and `most`
-  // (rows) that brings the phase to at most RR_STAGE_TILES_B tiles
+  // tile rows of a fused down-leg phase: 2 on the smallest phases, else ty,
+  // or the lowest of 8 and `most` (rows) that brings the phase to at most
+  // RR_STAGE_TILES_B tiles
   int rr_tile_rows(const std::vector<int>& who, const std::vector<int>& start, int dd, int ty,
                    int most) const {
     double cells = 0.;
@@ -3148,6 +3153,7 @@ struct Builder {
       const LevD& L = b.H[i].d.L[b.H[i].d.nfine - 1 - dd];
       cells += (double)(L.m - 2 * NH) * (L.n - 2 * NH);
     }
+    if (cells / ((double)TX * 4) <= RR_SMALL_TILES) return 2;
     if (cells / ((double)TX * ty) <= RR_STAGE_TILES_B) return ty;
     for (int t = 2 * ty; t <= most; t *= 2)
       if (cells / ((double)TX * t) <= RR_STAGE_TILES_B) return t;
@@ -4054,7 +4060,10 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
         const bool st = g <= RR_STAGE_TILES;
         const dim3 B16(64, 16);
         if (L.mode == 3) {  // x0 = I(xc): tiles of at most 4 coarse rows
-          if (L.ty == 4 && st)
+          if (L.ty == 2)
+            F2D_LAUNCH((k_smooth_rr<T, 1, 3, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
+                       b.dD, L.pred);
+          else if (L.ty == 4 && st)
             F2D_LAUNCH((k_smooth_rr<T, 2, 3, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
                        b.dD, L.pred);
           else if (L.ty == 4)
@@ -4066,6 +4075,9 @@ static int run_prog_t(Batch& b, const std::vector<Launch>& prog, ProfSel* prof =
           else
             F2D_LAUNCH((k_smooth_rr<T, 4, 3>), dim3(g), B2, 0, b.s, TK, b.dI, Q, b.B, b.dD,
                        L.pred);
+        } else if (L.ty == 2) {
+          F2D_LAUNCH((k_smooth_rr<T, 1, 1, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
+                     b.dD, L.pred);
         } else if (L.ty == 4) {
           if (st)
             F2D_LAUNCH((k_smooth_rr<T, 2, 1, 1, 1024>), dim3(g), B16, 0, b.s, TK, b.dI, Q, b.B,
