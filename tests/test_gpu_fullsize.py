@@ -3,8 +3,10 @@
 - euler-closed-1024 (1024 x 1024 closed box, k^-3 turbulence) and
   boussinesq-rt-2048x1024 (2048 x 1024 channel, Rayleigh-Taylor start): the
   FP64 batched engine steps twice from the workload's start and matches the
-  oracle stepped with the same dt, every field within rel-L2 1e-12 (the
-  oracle finishes these two steps in a few seconds).
+  oracle stepped with the same dt, every field bit for bit (and so within
+  rel-L2 1e-12; the oracle finishes these two steps in a few seconds); the 1024 x 1024 case
+  also with the benchmark's launch shapes (tiles for at least 1024
+  workgroups per finest-level launch, the coarse kernel's LDS budget).
 - The converged multigrid solve on the 1024 x 1024 closed box equals the
   DST-I solution of the 9-point Laplacian (psi = 0 on the boundary corners)
   within 1e-10 of its maximum: a size-independent check that needs no
@@ -29,12 +31,22 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-@pytest.mark.parametrize("name", ["euler-closed-1024", "boussinesq-rt-2048x1024"])
-def test_engine_full_size_vs_oracle(name):
-    print("case", name, flush=True)
+BENCH_TUNING = {"coarse_lds_budget": 159744, "min_tiles": 1024, "coarse_wave_cells": 0,
+                "persist": 0}
+
+
+@pytest.mark.parametrize("name,tuning", [("euler-closed-1024", None),
+                                         ("euler-closed-1024", BENCH_TUNING),
+                                         ("boussinesq-rt-2048x1024", None)],
+                         ids=["euler-closed-1024", "euler-closed-1024-bench-tiles",
+                              "boussinesq-rt-2048x1024"])
+def test_engine_full_size_vs_oracle(name, tuning):
+    print("case", name, tuning, flush=True)
     p = workloads.make_param(name)
     g = F.Grid(p)
-    b = Batch([(p, g)])
+    b = Batch([(p, g)], tuning=tuning)
+    if tuning is not None:
+        assert b.tuning()["min_tiles"] == tuning["min_tiles"]
     o = orc.OracleModel(p.modelname, g.nx, g.ny, g.Lx, g.Ly, geometry=g.geometry,
                         diffusion=p.diffusion, Kdiff=p.Kdiff, gravity=p.gravity)
     for k, v in workloads.initial_fields(name, g).items():
@@ -51,6 +63,8 @@ def test_engine_full_size_vs_oracle(name):
     s = b.state(0)
     for k, vname in enumerate(o.varname_list):
         assert rel(s[k], o.state[k]) <= 1e-12, vname
+        # the FP64 engine reproduces the oracle's operations in order
+        assert np.array_equal(s[k], o.state[k]), vname
     if p.modelname == "euler":
         assert b.diagnostics(0)["maxspeed"] == pytest.approx(o.diagnostics()["maxspeed"],
                                                              rel=1e-12)
