@@ -273,6 +273,32 @@ def test_mixed_precision_other_schemes(order, method, scheme):
         assert r <= 1e-6, "field %d rel-L2 %.3e" % (k, r)
 
 
+def test_profiled_steps_equal_plain_steps():
+    """Kernel timing (Batch.profile) issues every timed launch 8 times back
+    to back, so each timed launch must leave the state as one launch does:
+    steps run under the timing of every launch kind equal plain steps bit
+    for bit (the finest-level smoothing that also adds its result into psi,
+    x0 += x, is never repeated)."""
+    sp = [spec("euler", 256, 256, 1.0, 1.0, "closed")]
+    kinds = ["smooth", "resid_restrict", "rhs", "coarse", "smooth_rr", "up2"]
+    out = []
+    for profiled in (False, True):
+        b = Batch(sp, precision="fp64", tuning={"min_tiles": 1024})
+        init_fields(b, 0, 77)
+        b.set_psi_from_vorticity()
+        b.set_dt(dt_for(b, 0, b.diagnostics(0)["maxspeed"]))
+        b.step(1)
+        if profiled:
+            for k in kinds:
+                pr = b.profile(k, -1, 1)
+                assert pr["launches"] > 0, k
+        else:
+            b.step(len(kinds))
+        out.append(b.state(0))
+        del b
+    assert (out[0] == out[1]).all()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_persistent_multigrid_equals_launch_per_phase(precision):
     """Runs of mid-level smoother and residual/restriction phases as one
